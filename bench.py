@@ -1,0 +1,219 @@
+#!/usr/bin/env python3
+"""PM KDK throughput on one MI355X: mixed precision vs FP64, 512^3 clustered.
+
+Workload (the configuration the headline metric is quoted on): 512^3 cells and
+512^3 dark-matter particles in the periodic unit box, 20% uniform and 80% in 2000
+Plummer spheres (log-uniform scale radii 1-8 cells), velocities N(0, 0.01^2),
+4 pi G = 1, a = 1, dt = 1e-3, multigrid to rel 1e-10 (FP64 residual).  A step is
+one canonical KDK step: kick-drift, tile sort, deposit, RHS, solve (previous phi
+as guess), kick, periodic shift.  Inputs are resident in HBM before the timed
+region.
+
+Prints one JSON line (driver contract) with:
+  value          mixed-precision particle-steps/s
+  fp64           the FP64 path's particle-steps/s and V-cycles
+  accel_rel_l2   |g_mixed - g_fp64|_2 / |g_fp64|_2 over all particles, same state
+  roofline       the dominant kernel (largest total time) against 8 TB/s
+  stages         per-kernel-class mean time and achieved GB/s
+  cpu_baseline   the CPU restatement (oracle/) timed on this host on a sample
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = "PM KDK particle-steps/s at 512^3, mixed vs FP64 GPU at ≤1e-6 accel rel-L2"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def kernel_bytes(name, ncell, npart, halo_cells, mixed_sz):
+    """Algorithmic (compulsory) bytes of one launch of a kernel class at level 0."""
+    sz = mixed_sz
+    N, P = ncell, npart
+    table = {
+        "smooth.d0": (sz + sz / 2 + sz / 2) * N,          # read e (all), r (colour), write e (colour)
+        "resid_restrict.d0": (2 * sz + sz / 8) * N,        # read e, r; write coarse r
+        "prolong.d0": (2 * sz + sz / 8) * N,               # read/write e, read coarse e
+        "fill.d0": sz * N,
+        "residual": (16 + sz) * N,                         # read phi, f (FP64); write r
+        "update": (16 + sz) * N,                           # read/write phi, read e
+        "deposit": 24 * P + 8 * N + 8 * halo_cells,        # read positions; write own + halos
+        "fold": 8 * N + 8 * halo_cells + 16 * N,           # read own + halos; write rho, f
+        "rhs": 16 * N,
+        "kick_drift": 96 * P + 4 * P + 8 * N,              # read/write x,v; key; read phi
+        "kick": 72 * P + 8 * N,                            # read x,v; write v; read phi
+        "scatter": 4 * P + 2 * 56 * P,                     # key; move x,v,id
+        "keys": 24 * P + 4 * P,
+    }
+    return table.get(name)
+
+
+def run_mode(ctx, precision, steps, warmup, dt, tol, use_graph):
+    ctx.set_precision(precision)
+    if warmup:
+        ctx.pm_steps(warmup, dt, rel_tol=tol, use_graph=use_graph)
+    ctx.sync()
+    ctx.timing_reset()
+    ctx.timing_enable(True)
+    t0 = time.perf_counter()
+    it = ctx.pm_steps(steps, dt, rel_tol=tol, use_graph=use_graph)
+    ctx.sync()
+    t1 = time.perf_counter()
+    ctx.timing_enable(False)
+    return t1 - t0, int(it[0])
+
+
+def stage_table(ctx, names, ncell, npart, halo_cells, sz, elapsed_ms):
+    out = {}
+    for nm in names:
+        cnt, mean, tot = ctx.timing(nm)
+        if cnt == 0:
+            continue
+        b = kernel_bytes(nm, ncell, npart, halo_cells, sz)
+        e = {"launches": cnt, "mean_ms": round(mean, 4), "total_ms": round(tot, 2),
+             "share": round(tot / elapsed_ms, 4)}
+        if b is not None and mean > 0:
+            e["gbs"] = round(b / (mean * 1e-3) / 1e9, 1)
+            e["bytes_per_launch"] = int(b)
+        out[nm] = e
+    return out
+
+
+def cpu_baseline(n_cpu, threads):
+    import oracle
+    from nyx_amd import ics
+    oracle.set_num_threads(threads)
+    pos, vel, m = ics.plummer_clustered(n_cpu, ics.SEED_BASE + 2, n_spheres=max(1, int(2000 * (n_cpu / 512) ** 3)))
+    n = (n_cpu,) * 3
+    pos = [np.ascontiguousarray(a) for a in pos]
+    vel = [np.ascontiguousarray(a) for a in vel]
+    phi, _, _ = oracle.solve_for_phi(n, (0, 0, 0), (1, 1, 1), pos, m_const=m, rel_tol=1e-10)
+    t0 = time.perf_counter()
+    _, st = oracle.pm_step(n, (0, 0, 0), (1, 1, 1), pos, vel, phi, m_const=m, dt=1e-3, rel_tol=1e-10)
+    t = time.perf_counter() - t0
+    return {"value": round(n_cpu ** 3 / t, 1), "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"one canonical KDK step (deposit, MG to rel 1e-10 with {int(st[0])} V-cycles, "
+                      f"gradient, kick-drift, kick) of the same clustered recipe at {n_cpu}^3 cells/particles "
+                      f"({max(1, int(2000 * (n_cpu / 512) ** 3))} Plummer spheres), FP64, {t:.2f} s; "
+                      f"stage s: deposit {st[4]:.2f} solve {st[5]:.2f} grav+kick {st[6]:.2f}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--cpu-n", type=int, default=192)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graph", action="store_true")
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--json-out", default=None)
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md, "
+              "'How the path would shard'); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import nyx_amd
+    from nyx_amd import ics
+    n = a.n
+    dt = 1e-3
+    log(f"generating {n}^3 clustered particles")
+    t = time.perf_counter()
+    pos, vel, m = ics.plummer_clustered(n, ics.SEED_BASE + 2, n_spheres=max(1, int(2000 * (n / 512) ** 3)))
+    log(f"  {time.perf_counter() - t:.1f} s")
+    ctx = nyx_amd.PMContext(nyx_amd.BoxGeom((n, n, n)), precision="fp64")
+    ctx.upload(0, pos, vel, m_const=m)
+    del pos, vel
+    npart = n ** 3
+    ncell = n ** 3
+    log("device bytes", ctx.device_bytes())
+
+    # accuracy: same particles, same density; FP64 solve vs mixed solve
+    log("FP64 initial solve")
+    ctx.set_precision("fp64")
+    ctx.pm_init(1.0, 1.0, a.tol)
+    g64 = ctx.gather_accel()
+    ctx.set_precision("mixed")
+    it_m, _ = ctx.solve_for_phi(1.0, 1.0, a.tol, 0.0, use_guess=False)
+    gmx = ctx.gather_accel()
+    rel_l2 = float(np.sqrt(np.sum((gmx - g64) ** 2)) / np.sqrt(np.sum(g64 ** 2)))
+    del g64, gmx
+    log(f"accel rel-L2 mixed vs fp64 = {rel_l2:.3e}  (mixed cold-start cycles {int(it_m[0])})")
+
+    names = ["smooth.d0", "resid_restrict.d0", "prolong.d0", "fill.d0", "residual", "update", "deposit", "fold",
+             "rhs", "kick_drift", "kick", "scatter", "scan", "keys", "bottom"] + \
+            [f"{k}.d{d}" for k in ("smooth", "resid_restrict", "prolong", "fill") for d in range(1, 12)]
+    tile = 8
+    halo_cells = (ncell // tile ** 3) * ((tile + 1) ** 3 - tile ** 3)
+
+    log(f"mixed: {a.warmup} warmup + {a.steps} timed steps")
+    t_mix, it_mix = run_mode(ctx, "mixed", a.steps, a.warmup, dt, a.tol, a.graph)
+    st_mix = stage_table(ctx, names, ncell, npart, halo_cells, 4, t_mix * 1e3)
+    log(f"  {t_mix / a.steps * 1e3:.2f} ms/step, {it_mix} V-cycles")
+    log(f"fp64: {a.warmup} warmup + {a.steps} timed steps")
+    t_64, it_64 = run_mode(ctx, "fp64", a.steps, a.warmup, dt, a.tol, a.graph)
+    st_64 = stage_table(ctx, names, ncell, npart, halo_cells, 8, t_64 * 1e3)
+    log(f"  {t_64 / a.steps * 1e3:.2f} ms/step, {it_64} V-cycles")
+    ctx.close()
+
+    v_mix = npart * a.steps / t_mix
+    v_64 = npart * a.steps / t_64
+    dom = max(st_mix.items(), key=lambda kv: kv[1]["total_ms"])
+    dname, d = dom
+    b = kernel_bytes(dname, ncell, npart, halo_cells, 4)
+    achieved = b / (d["mean_ms"] * 1e-3) / 1e9 if b else None
+    roof = {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 1) if achieved else None,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+            "traffic": None, "bytes_per_launch": int(b) if b else None, "mean_launch_ms": d["mean_ms"]}
+
+    cpu = None
+    if not a.no_cpu:
+        try:
+            aff = len(os.sched_getaffinity(0))
+        except Exception:
+            aff = os.cpu_count() or 1
+        threads = max(1, min(16, aff))
+        log(f"CPU baseline at {a.cpu_n}^3 on {threads} threads")
+        cpu = cpu_baseline(a.cpu_n, threads)
+        log("  ", cpu["value"], "particle-steps/s")
+
+    out = {
+        "metric": METRIC, "value": round(v_mix, 1), "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(t_mix / a.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"{n}^3 cells, {n}^3 particles, 20% uniform + 80% in "
+                               f"{max(1, int(2000 * (n / 512) ** 3))} Plummer spheres, mixed precision "
+                               f"(FP32 V-cycles in FP64 refinement, FP64 particles), dt 1e-3, rel tol {a.tol:g}",
+                   "cells": ncell, "particles": npart, "graph": bool(a.graph)},
+        "fp64": {"value": round(v_64, 1), "ms_per_step": round(t_64 / a.steps * 1e3, 3), "vcycles": it_64,
+                 "dtype": "f64"},
+        "mixed_over_fp64": round(v_mix / v_64, 3),
+        "vcycles_mixed": it_mix,
+        "accel_rel_l2": rel_l2,
+        "roofline": roof,
+        "stages_mixed": st_mix,
+        "stages_fp64": st_64,
+        "cpu_baseline": cpu,
+    }
+    line = json.dumps(out)
+    if a.json_out:
+        with open(a.json_out, "w") as fh:
+            fh.write(line + "\n")
+    print(line, flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,184 @@
+/*
+ * nyxpm.h -- C ABI of the MI355X particle-mesh gravity library.
+ *
+ * The library replaces, on one periodic level, the hot path Nyx drives through
+ * its particle-container and Gravity interfaces.  Each entry point names the
+ * reference interface it stands in for (paths relative to the Nyx source tree):
+ *
+ *   nyxpm_assign_density      NyxParticleContainerBase::AssignDensitySingleLevel /
+ *                             AssignDensity        (Source/Particle/NyxParticleContainer.H:43-46, 90-97)
+ *   nyxpm_sum_particle_mass   NyxParticleContainerBase::sumParticleMass
+ *                                                  (NyxParticleContainer.H:38, 70-79)
+ *   nyxpm_solve_for_phi       Gravity::solve_for_phi with Gravity::CorrectRhsUsingOffset and
+ *                             Gravity::solve_with_MLMG (Source/Gravity/Gravity.H:54-56;
+ *                             Gravity.cpp:267-315, 1207-1244, 1318-1416)
+ *   nyxpm_solve_poisson       Gravity::solve_with_MLMG alone (Gravity.H:43-48)
+ *   nyxpm_get_grav_vector     Gravity::get_new_grav_vector / get_old_grav_vector
+ *                                                  (Gravity.H:84-85; Gravity.cpp:781-850)
+ *   nyxpm_gather_accel        the acceleration update_dm_particle_single interpolates
+ *                                                  (DarkMatterParticleContainer.cpp:276-292)
+ *   nyxpm_move_kick_drift     DarkMatterParticleContainer::moveKickDrift
+ *                                                  (DarkMatterParticleContainer.H:31-32; .cpp:75-181)
+ *   nyxpm_move_kick           DarkMatterParticleContainer::moveKick
+ *                                                  (DarkMatterParticleContainer.H:33-34; .cpp:183-247)
+ *   nyxpm_redistribute        NyxParticleContainerBase::RedistributeLocal (periodic shift only)
+ *                                                  (NyxParticleContainer.H:123-136)
+ *   nyxpm_pm_init             the phi solve of Nyx::post_init (Source/Driver/Nyx.cpp:1958-2006)
+ *   nyxpm_pm_steps            the particle-only advance of Nyx::advance_particles_only
+ *                             followed by post_timestep's redistribute
+ *                                                  (Source/TimeStep/Nyx_advance_particles.cpp:9-272;
+ *                                                   Source/Driver/Nyx.cpp:1421-1436)
+ *
+ * Conventions
+ *  - Every function returns 0 on success or a NYXPM_ERR_* code; the library never
+ *    aborts the process.  nyxpm_last_error() gives a message for the last failure.
+ *  - Host pointers passed in are borrowed for the duration of the call.  The
+ *    library owns every device buffer; particle and field data stay resident in
+ *    HBM between calls.  Entry points with the _device suffix take device
+ *    pointers instead (stream-ordered on the context's stream).
+ *  - One context per host thread; a context is not internally synchronised.
+ *    All work is ordered on the context's stream; nyxpm_sync() waits for it.
+ *  - Cell fields are x-fastest, no ghost cells: index (k*ny + j)*nx + i.  Vector
+ *    fields are component-major (3 consecutive cell fields).
+ *  - A context may hold several independent periodic boxes ("batch"); box-indexed
+ *    entry points take the box number, the others act on every box.
+ */
+#ifndef NYXPM_H
+#define NYXPM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NYXPM_OK 0
+#define NYXPM_ERR_ARG 1          /* invalid argument */
+#define NYXPM_ERR_HIP 2          /* a HIP runtime call failed */
+#define NYXPM_ERR_NOMEM 3        /* device allocation failed */
+#define NYXPM_ERR_NOT_CONVERGED 4/* multigrid did not reach its tolerance within max_iter */
+#define NYXPM_ERR_UNSUPPORTED 5  /* geometry the solver does not handle (see message) */
+#define NYXPM_ERR_STATE 6        /* call out of order (e.g. no particles uploaded) */
+
+#define NYXPM_FP64 0   /* FP64 everywhere: the drop-in path */
+#define NYXPM_MIXED 1  /* FP32 V-cycles inside FP64 iterative refinement */
+
+typedef struct nyxpm_ctx nyxpm_ctx;
+
+/* One periodic box: Geometry's n_cell, prob_lo, prob_hi (all directions periodic). */
+typedef struct nyxpm_geom {
+    int n_cell[3];
+    double prob_lo[3];
+    double prob_hi[3];
+} nyxpm_geom;
+
+typedef struct nyxpm_opts {
+    int precision;     /* NYXPM_FP64 or NYXPM_MIXED */
+    int nu1, nu2;      /* pre/post smoothing sweeps (MLMG defaults 2, 2); 0 = default */
+    int max_iter;      /* V-cycle limit (MLMG default 200); 0 = default */
+    int tile[3];       /* deposit/gather tile shape in cells; 0 = default (8,8,8) */
+    int verbose;       /* 0 = silent */
+} nyxpm_opts;
+
+/* ---- lifecycle ---------------------------------------------------------- */
+int nyxpm_create(nyxpm_ctx **ctx, const nyxpm_geom *geom, const nyxpm_opts *opts);
+int nyxpm_create_batch(nyxpm_ctx **ctx, int nbox, const nyxpm_geom *geoms, const nyxpm_opts *opts);
+int nyxpm_destroy(nyxpm_ctx *ctx);
+int nyxpm_set_stream(nyxpm_ctx *ctx, void *hip_stream);
+int nyxpm_sync(nyxpm_ctx *ctx);
+const char *nyxpm_last_error(const nyxpm_ctx *ctx);
+int nyxpm_num_boxes(const nyxpm_ctx *ctx);
+int nyxpm_set_precision(nyxpm_ctx *ctx, int precision);
+
+/* ---- particles ---------------------------------------------------------- */
+/* Replace the particles of box `box` (host arrays of length np).  m may be NULL:
+ * then every particle has mass m_const.  Particles keep their upload index as id. */
+int nyxpm_particles_upload(nyxpm_ctx *ctx, int box, int64_t np,
+                           const double *x, const double *y, const double *z,
+                           const double *vx, const double *vy, const double *vz,
+                           const double *m, double m_const);
+/* Same, device pointers. */
+int nyxpm_particles_upload_device(nyxpm_ctx *ctx, int box, int64_t np,
+                                  const double *x, const double *y, const double *z,
+                                  const double *vx, const double *vy, const double *vz,
+                                  const double *m, double m_const);
+/* Copy box `box`'s particles to host arrays, in upload (id) order.  Any pointer
+ * may be NULL to skip that component. */
+int nyxpm_particles_download(nyxpm_ctx *ctx, int box,
+                             double *x, double *y, double *z,
+                             double *vx, double *vy, double *vz);
+int64_t nyxpm_num_particles(const nyxpm_ctx *ctx, int box);
+
+/* ---- AssignDensity / sumParticleMass ------------------------------------ */
+/* Deposit every box's particles with CIC weights (deterministic), periodic,
+ * times 1/cell volume.  rho_out (host, n cells of box `box`) may be NULL to keep
+ * the result on the device only; box = -1 with rho_out NULL deposits all boxes. */
+int nyxpm_assign_density(nyxpm_ctx *ctx, int box, double *rho_out);
+int nyxpm_sum_particle_mass(nyxpm_ctx *ctx, int box, double *mass_out);
+
+/* ---- Gravity ------------------------------------------------------------ */
+/* solve_for_phi on every box: Rhs = density of the last assign_density (or
+ * nyxpm_set_density); Rhs -= mass_offset; Rhs -= vol_weight_sum(Rhs);
+ * Rhs *= -four_pi_G; Rhs *= 1/a; then V-cycles from phi (use_guess != 0) or from
+ * zero until |Rhs - L phi|_inf <= max(abs_tol, rel_tol*max(|Rhs|_inf, |r0|_inf)).
+ * iters / final_resnorm (length nbox each, may be NULL) receive per-box results. */
+int nyxpm_solve_for_phi(nyxpm_ctx *ctx, double a, double four_pi_G, double rel_tol,
+                        double abs_tol, int use_guess, int *iters, double *final_resnorm);
+/* solve_with_MLMG on box `box`: L phi = rhs for a given right-hand side (host
+ * array, mean-free).  phi_io (host) holds the guess on entry, the solution on exit. */
+int nyxpm_solve_poisson(nyxpm_ctx *ctx, int box, const double *rhs, double *phi_io,
+                        double rel_tol, double abs_tol, int fixed_iters, int *iters,
+                        double *norms /* 4: resnorm0, rhsnorm, final, target; may be NULL */);
+/* Accelerations interpolated to box `box`'s particles from the current phi, as
+ * moveKick/moveKickDrift use them (update_dm_particle_single's val,
+ * DarkMatterParticleContainer.cpp:276-292), without changing the particles:
+ * g_out holds 3*np doubles, component-major, in upload (id) order. */
+int nyxpm_gather_accel(nyxpm_ctx *ctx, int box, double *g_out);
+/* Cell-centred gravity g = average of face gradients of phi (3 fields, host). */
+int nyxpm_get_grav_vector(nyxpm_ctx *ctx, int box, double *g_out);
+/* Field access for box `box` (host arrays of n cells). */
+int nyxpm_get_phi(nyxpm_ctx *ctx, int box, double *phi_out);
+int nyxpm_set_phi(nyxpm_ctx *ctx, int box, const double *phi_in);
+int nyxpm_set_density(nyxpm_ctx *ctx, int box, const double *rho_in);
+int nyxpm_get_rhs(nyxpm_ctx *ctx, int box, double *rhs_out);
+
+/* ---- particle update ---------------------------------------------------- */
+/* moveKickDrift: v = (a_old v + dt/2 g(x)) / a_half; x += dt/a_half v, g from the
+ * current phi (gradient fused into the interpolation). */
+int nyxpm_move_kick_drift(nyxpm_ctx *ctx, double dt, double a_old, double a_half);
+/* moveKick: v = (a_half v + dt/2 g(x)) / a_new, then the periodic position shift
+ * of Redistribute. */
+int nyxpm_move_kick(nyxpm_ctx *ctx, double dt, double a_new, double a_half);
+/* Periodic position shift alone (RedistributeLocal on one level). */
+int nyxpm_redistribute(nyxpm_ctx *ctx);
+
+/* ---- whole step --------------------------------------------------------- */
+/* Before the first step: phi for the initial positions (from a zero guess). */
+int nyxpm_pm_init(nyxpm_ctx *ctx, double a, double four_pi_G, double rel_tol, double abs_tol);
+/* nsteps KDK steps with fixed dt: KD with the stored phi, deposit, solve (previous
+ * phi as guess), K, wrap.  nyx_faithful != 0 re-solves the old phi from a zero
+ * guess at the start of each step, as Nyx_advance_particles.cpp:148-170 does.
+ * use_graph != 0 captures the step into a hipGraph once and replays it (the
+ * V-cycle count is then decided on the device; max_iter cycles are recorded).
+ * iters (length nbox, may be NULL) receives the V-cycles of the last step. */
+int nyxpm_pm_steps(nyxpm_ctx *ctx, int nsteps, double dt, double a_old, double a_new,
+                   double four_pi_G, double rel_tol, double abs_tol, int nyx_faithful,
+                   int use_graph, int *iters);
+
+/* ---- measurement -------------------------------------------------------- */
+/* Kernel-class timing with HIP events on the context stream.  When enabled, each
+ * launch of a timed class is bracketed by events; nyxpm_timing_get returns the
+ * number of launches and the mean duration in milliseconds for `name`
+ * ("smooth", "resid_restrict", "prolong", "residual", "deposit", "kick_drift",
+ * "kick", "sort", "fold_rhs", "bottom"). */
+int nyxpm_timing_enable(nyxpm_ctx *ctx, int on);
+int nyxpm_timing_get(nyxpm_ctx *ctx, const char *name, int64_t *count, double *mean_ms,
+                     double *total_ms);
+int nyxpm_timing_reset(nyxpm_ctx *ctx);
+/* Bytes of device memory the context holds. */
+int64_t nyxpm_device_bytes(const nyxpm_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NYXPM_H */

@@ -1,0 +1,245 @@
+// mg_kernels.hip -- geometric multigrid for the periodic 7-point Poisson operator
+// (the MLMG/MLPoisson V-cycle Nyx calls from Gravity::solve_with_MLMG,
+// Source/Gravity/Gravity.cpp:1318-1416), as gfx950 kernels.
+//
+// Every kernel takes a table of CellSeg pieces (one per box and level it works
+// on) and an `active` flag per box, so one launch serves a batch of boxes and a
+// box whose solve has converged costs nothing.  The arithmetic of each cell
+// update is written in the same order as the CPU restatement so that, with FMA
+// contraction off, the FP64 iterates agree bit for bit.
+//
+// Roofline: all of these are HBM-bound stencils (arithmetic intensity well below
+// 1 flop/byte).  Compulsory bytes per fine cell (FP64 / FP32):
+//   red-black half sweep  8 (e) + 8 (r) + 8 (e write)   / 4+4+4
+//   residual + restrict   8 (e) + 8 (r) + 1 (coarse)    / 4+4+0.5
+//   prolong + add         8 (e) + 8 (e write) + 1       / 4+4+0.5
+#include "nyxpm_internal.h"
+
+namespace nyxpm {
+
+template <typename T>
+__device__ __forceinline__ T lap7(const T* a, int i, int j, int k, int nx, int ny, int nz,
+                                  T dhx, T dhy, T dhz) {
+    int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
+    int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
+    int km = k == 0 ? nz - 1 : k - 1, kp = k == nz - 1 ? 0 : k + 1;
+    T c = a[cidx(i, j, k, nx, ny)];
+    T lx = dhx * (a[cidx(im, j, k, nx, ny)] - T(2) * c + a[cidx(ip, j, k, nx, ny)]);
+    T ly = dhy * (a[cidx(i, jm, k, nx, ny)] - T(2) * c + a[cidx(i, jp, k, nx, ny)]);
+    T lz = dhz * (a[cidx(i, j, km, nx, ny)] - T(2) * c + a[cidx(i, j, kp, nx, ny)]);
+    return lx + ly + lz;
+}
+
+// a[c] = v on every cell of each piece.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_fill(const CellSeg* segs, int nseg, const int* active, T* a, T v) {
+    const int s = find_seg(segs, nseg, blockIdx.x);
+    const CellSeg sg = segs[s];
+    if (active && !active[sg.box]) return;
+    const long long n = (long long)sg.n[0] * sg.n[1] * sg.n[2];
+    const long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x;
+    if (q < n) a[sg.off + q] = v;
+}
+
+// One red-black Gauss-Seidel half sweep (cells with (i+j+k+colour) even):
+//   res = r - gamma*e - dhx*(e(i-1)+e(i+1)) - dhy*(..) - dhz*(..);  e += res/gamma
+// One thread per cell of the colour.  Needs even nx (true above the bottom).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_gsrb_half(const CellSeg* segs, int nseg, const int* active,
+                                                      T* __restrict__ e, const T* __restrict__ r, int colour) {
+    const int s = find_seg(segs, nseg, blockIdx.x);
+    const CellSeg& sg = segs[s];
+    if (!active[sg.box]) return;
+    const int nx = sg.n[0], ny = sg.n[1], nz = sg.n[2];
+    const int hx = nx >> 1;
+    const long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x;
+    if (q >= (long long)hx * ny * nz) return;
+    const int ih = (int)(q % hx);
+    const long long t = q / hx;
+    const int j = (int)(t % ny), k = (int)(t / ny);
+    const int i = 2 * ih + ((j + k + colour) & 1);
+    const T dhx = T(sg.dh[0]), dhy = T(sg.dh[1]), dhz = T(sg.dh[2]);
+    const T gamma = T(sg.gamma), inv_gamma = T(sg.inv_gamma);
+    T* E = e + sg.off;
+    const T* R = r + sg.off;
+    int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
+    int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
+    int km = k == 0 ? nz - 1 : k - 1, kp = k == nz - 1 ? 0 : k + 1;
+    const size_t c = cidx(i, j, k, nx, ny);
+    T ec = E[c];
+    T res = R[c] - gamma * ec
+          - dhx * (E[cidx(im, j, k, nx, ny)] + E[cidx(ip, j, k, nx, ny)])
+          - dhy * (E[cidx(i, jm, k, nx, ny)] + E[cidx(i, jp, k, nx, ny)])
+          - dhz * (E[cidx(i, j, km, nx, ny)] + E[cidx(i, j, kp, nx, ny)]);
+    E[c] = ec + res * inv_gamma;
+}
+
+// Residual of the correction, restricted: r_c(I) = 0.125 * sum_children (r - L e),
+// children summed k, then j, then i innermost.  One thread per coarse cell.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_resid_restrict(const CellSeg* segs, int nseg, const int* active,
+                                                           const T* __restrict__ e, T* __restrict__ r) {
+    const int s = find_seg(segs, nseg, blockIdx.x);
+    const CellSeg& sg = segs[s];
+    if (!active[sg.box]) return;
+    const int nx = sg.n[0], ny = sg.n[1], nz = sg.n[2];
+    const int cx = nx >> 1, cy = ny >> 1, cz = nz >> 1;
+    const long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x;
+    if (q >= (long long)cx * cy * cz) return;
+    const int I = (int)(q % cx);
+    const long long t = q / cx;
+    const int J = (int)(t % cy), K = (int)(t / cy);
+    const T dhx = T(sg.dh[0]), dhy = T(sg.dh[1]), dhz = T(sg.dh[2]);
+    const T* E = e + sg.off;
+    const T* R = r + sg.off;
+    T acc = T(0);
+    for (int kk = 0; kk < 2; ++kk)
+        for (int jj = 0; jj < 2; ++jj)
+            for (int ii = 0; ii < 2; ++ii) {
+                int i = 2 * I + ii, j = 2 * J + jj, k = 2 * K + kk;
+                T rc = R[cidx(i, j, k, nx, ny)] - lap7(E, i, j, k, nx, ny, nz, dhx, dhy, dhz);
+                acc += rc;
+            }
+    r[sg.coff + q] = T(0.125) * acc;
+}
+
+// e_fine += e_coarse(parent): piecewise-constant prolongation.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_prolong(const CellSeg* segs, int nseg, const int* active, T* e) {
+    const int s = find_seg(segs, nseg, blockIdx.x);
+    const CellSeg& sg = segs[s];
+    if (!active[sg.box]) return;
+    const int nx = sg.n[0], ny = sg.n[1], nz = sg.n[2];
+    const long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x;
+    if (q >= (long long)nx * ny * nz) return;
+    const int i = (int)(q % nx);
+    const long long t = q / nx;
+    const int j = (int)(t % ny), k = (int)(t / ny);
+    const T ec = e[sg.coff + cidx(i >> 1, j >> 1, k >> 1, nx >> 1, ny >> 1)];
+    e[sg.off + q] = e[sg.off + q] + ec;
+}
+
+// Exact bottom solve: e[c] = sum_{c'} G[(c - c') mod n] r[c'], c' in natural order.
+// One block per box; r and G staged in LDS (at most 4096 cells each).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_bottom(const CellSeg* segs, int nseg, const int* active,
+                                                   T* __restrict__ e, const T* __restrict__ r,
+                                                   const T* __restrict__ G) {
+    const CellSeg& sg = segs[blockIdx.x];
+    if (!active[sg.box]) return;
+    __shared__ T sr[kBottomMaxCells];
+    __shared__ T sG[kBottomMaxCells];
+    const int nx = sg.n[0], ny = sg.n[1], nz = sg.n[2];
+    const int nc = nx * ny * nz;
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) {
+        sr[c] = r[sg.off + c];
+        sG[c] = G[sg.coff + c];
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) {
+        const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+        T acc = T(0);
+        for (int qz = 0; qz < nz; ++qz) {
+            int dz = cz - qz; dz = dz < 0 ? dz + nz : dz;
+            for (int qy = 0; qy < ny; ++qy) {
+                int dy = cy - qy; dy = dy < 0 ? dy + ny : dy;
+                for (int qx = 0; qx < nx; ++qx) {
+                    int dxx = cx - qx; dxx = dxx < 0 ? dxx + nx : dxx;
+                    acc += sG[(dz * ny + dy) * nx + dxx] * sr[(qz * ny + qy) * nx + qx];
+                }
+            }
+        }
+        e[sg.off + c] = acc;
+    }
+}
+
+// Level-0 residual r = f - L phi (FP64), stored as T, with max-norms of r (and of
+// f when nf != nullptr) reduced per box.  sg.off addresses the MG pool, the
+// fields phi/f live in the level-0 pools at foff (passed in coff).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_residual0(const CellSeg* segs, int nseg, const int* active,
+                                                      const double* __restrict__ f,
+                                                      const double* __restrict__ phi, T* __restrict__ r,
+                                                      unsigned long long* nr, unsigned long long* nf) {
+    const int s = find_seg(segs, nseg, blockIdx.x);
+    const CellSeg& sg = segs[s];
+    if (!active[sg.box]) return;
+    const int nx = sg.n[0], ny = sg.n[1], nz = sg.n[2];
+    const long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x;
+    double ar = 0.0, af = 0.0;
+    if (q < (long long)nx * ny * nz) {
+        const int i = (int)(q % nx);
+        const long long t = q / nx;
+        const int j = (int)(t % ny), k = (int)(t / ny);
+        const double* P = phi + sg.coff;
+        const double fv = f[sg.coff + q];
+        const double v = fv - lap7(P, i, j, k, nx, ny, nz, sg.dh[0], sg.dh[1], sg.dh[2]);
+        r[sg.off + q] = T(v);
+        ar = fabs(v);
+        af = fabs(fv);
+    }
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        ar = fmax(ar, __shfl_xor(ar, o));
+        af = fmax(af, __shfl_xor(af, o));
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        atomicMax(&nr[sg.box], dbits(ar));
+        if (nf) atomicMax(&nf[sg.box], dbits(af));
+    }
+}
+
+// phi += e (level 0).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_update0(const CellSeg* segs, int nseg, const int* active,
+                                                    double* __restrict__ phi, const T* __restrict__ e) {
+    const int s = find_seg(segs, nseg, blockIdx.x);
+    const CellSeg& sg = segs[s];
+    if (!active[sg.box]) return;
+    const long long n = (long long)sg.n[0] * sg.n[1] * sg.n[2];
+    const long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x;
+    if (q < n) phi[sg.coff + q] = phi[sg.coff + q] + (double)e[sg.off + q];
+}
+
+// Convergence bookkeeping per box (MLMG stop rule), on the device so that a
+// captured graph can run a data-dependent number of cycles.
+//   init: target = max(abs_tol, max(rel_tol, 1e-16) * max(|r0|, |f|)),
+//         active = |r0| > target (or fixed_iters > 0)
+//   iter: iters++, active = |r| > target && iters < limit
+// state: [0] resnorm0 [1] rhsnorm [2] final [3] target
+__global__ void k_check(int nbox, int init, int fixed_iters, int max_iter, double rel_tol,
+                        double abs_tol, int* active, int* iters, int* failed,
+                        unsigned long long* nr, unsigned long long* nf, double* state) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbox) return;
+    if (init) {
+        const double r0 = __longlong_as_double((long long)nr[b]);
+        const double fn = __longlong_as_double((long long)nf[b]);
+        const double mx = r0 > fn ? r0 : fn;
+        const double rt = rel_tol > 1e-16 ? rel_tol : 1e-16;
+        double target = rt * mx;
+        if (abs_tol > target) target = abs_tol;
+        state[4 * b + 0] = r0;
+        state[4 * b + 1] = fn;
+        state[4 * b + 2] = r0;
+        state[4 * b + 3] = target;
+        iters[b] = 0;
+        failed[b] = 0;
+        active[b] = fixed_iters > 0 ? 1 : (r0 > target ? 1 : 0);
+    } else if (active[b]) {
+        const double rn = __longlong_as_double((long long)nr[b]);
+        const int it = iters[b] + 1;
+        iters[b] = it;
+        state[4 * b + 2] = rn;
+        if (fixed_iters > 0) {
+            active[b] = it < fixed_iters ? 1 : 0;
+        } else {
+            const int more = rn > state[4 * b + 3];
+            active[b] = (more && it < max_iter) ? 1 : 0;
+            failed[b] = (more && it >= max_iter) ? 1 : 0;
+        }
+    }
+    nr[b] = 0ull;
+    nf[b] = 0ull;
+}
+
+}  // namespace nyxpm

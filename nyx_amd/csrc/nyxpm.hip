@@ -1,0 +1,1414 @@
+// nyxpm.hip -- host side of the MI355X particle-mesh library and its C ABI
+// (include/nyxpm.h).  One translation unit: the kernel files are included here.
+//
+// The host code owns every device buffer, builds the launch tables for a batch
+// of periodic boxes, and drives the step in the order of
+// Nyx::advance_particles_only (Source/TimeStep/Nyx_advance_particles.cpp:137-268):
+//   moveKickDrift -> AssignDensity -> solve_for_phi -> moveKick -> Redistribute.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/nyxpm.h"
+#include "nyxpm_internal.h"
+#include "mg_kernels.hip"
+#include "particle_kernels.hip"
+
+using namespace nyxpm;
+
+namespace {
+
+struct Level {
+    int n[3];
+    double dh[3];
+    double gamma, inv_gamma;
+    long long off;  // offset in the MG pools
+    long long nc;
+};
+
+struct Box {
+    int n[3];
+    double plo[3], phi[3], dx[3], dxi[3];
+    double rhi[3], len[3];
+    double inv_vol, dv, prob_size;
+    int nlev;
+    Level lev[kMaxLevels];
+    long long cell_off;
+    int tile[3], ntile[3];
+    long long tile_off, ntiles;
+    long long part_off = 0, np = 0;
+    bool has_mass = false;
+    double m_const = 0.0, mass_total = 0.0, mass_offset = 0.0;
+    int fx_exp = 0;
+    long long G_off;
+};
+
+struct EventPair {
+    hipEvent_t a, b;
+};
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+};
+
+// Largest double below hi whose cell index stays < n (upper end kept by the
+// periodic shift).
+double domain_rhi(double lo, double hi, int n) {
+    const double dxi = 1.0 / ((hi - lo) / (double)n);
+    double r = std::nextafter(hi, -INFINITY);
+    while ((int)std::floor((r - lo) * dxi) >= n) r = std::nextafter(r, -INFINITY);
+    return r;
+}
+
+// Coarsening chain: halve all dimensions while all are even and stay >= 2.
+int mg_levels(const int n[3], int dims[][3]) {
+    int m[3] = {n[0], n[1], n[2]};
+    int L = 0;
+    for (;;) {
+        dims[L][0] = m[0];
+        dims[L][1] = m[1];
+        dims[L][2] = m[2];
+        ++L;
+        if (L >= kMaxLevels) break;
+        bool ok = true;
+        for (int d = 0; d < 3; ++d)
+            if ((m[d] & 1) || m[d] / 2 < 2) ok = false;
+        if (!ok) break;
+        for (int d = 0; d < 3; ++d) m[d] /= 2;
+    }
+    return L;
+}
+
+// Pseudo-inverse of the periodic 7-point operator on the bottom level, as a
+// convolution kernel: G[c] = (1/N) sum_{k!=0} cos(2 pi sum_d ((k_d c_d) mod n_d)/n_d)
+// / lambda_k, lambda_k = sum_d dh_d (2 cos(2 pi k_d/n_d) - 2).
+void bottom_green(const int n[3], const double dh[3], double* G) {
+    const double twopi = 2.0 * M_PI;
+    const int N = n[0] * n[1] * n[2];
+    for (int cz = 0; cz < n[2]; ++cz)
+        for (int cy = 0; cy < n[1]; ++cy)
+            for (int cx = 0; cx < n[0]; ++cx) {
+                double acc = 0.0;
+                for (int kz = 0; kz < n[2]; ++kz)
+                    for (int ky = 0; ky < n[1]; ++ky)
+                        for (int kx = 0; kx < n[0]; ++kx) {
+                            if (kx == 0 && ky == 0 && kz == 0) continue;
+                            double lam = dh[0] * (2.0 * cos(twopi * kx / n[0]) - 2.0)
+                                       + dh[1] * (2.0 * cos(twopi * ky / n[1]) - 2.0)
+                                       + dh[2] * (2.0 * cos(twopi * kz / n[2]) - 2.0);
+                            double th = twopi * ((double)((kx * cx) % n[0]) / n[0]
+                                               + (double)((ky * cy) % n[1]) / n[1]
+                                               + (double)((kz * cz) % n[2]) / n[2]);
+                            acc += cos(th) / lam;
+                        }
+                G[((size_t)cz * n[1] + cy) * n[0] + cx] = acc / (double)N;
+            }
+}
+
+inline int nblocks(long long n, int per = kBlock) { return (int)((n + per - 1) / per); }
+
+}  // namespace
+
+struct nyxpm_ctx {
+    std::string err;
+    int verbose = 0;
+    int precision = NYXPM_FP64;
+    int nu1 = 2, nu2 = 2, max_iter = 200;
+    int graph_cycles = 24;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::vector<Box> boxes;
+    int dmax = 1;  // max levels over boxes
+    long long ncell_total = 0, ntiles_total = 0, npart_total = 0, mg_total = 0, G_total = 0;
+    int halo_stride = 0;
+    int tile[3] = {8, 8, 8};
+
+    // particles (two buffers for the tile sort)
+    PartPtrs P[2] = {};
+    int cur = 0;
+    int* key = nullptr;
+    long long part_cap = 0;
+    bool any_mass = false;
+    bool keys_valid = false, sorted_valid = false, density_valid = false;
+    bool phi_valid = false;
+    bool f1_valid = false;  // f holds rho - mass_offset and partial holds its block sums
+
+    // tiles
+    int* counts = nullptr;
+    int* cursor = nullptr;
+    long long* tstart = nullptr;
+    long long* bsum = nullptr;
+    int scan_blocks = 0;
+
+    // level-0 fields
+    double *own = nullptr, *halo = nullptr, *rho = nullptr, *f = nullptr, *phi = nullptr, *partial = nullptr;
+    double* gtmp = nullptr;  // 3 fields of the largest box (grav vector / downloads)
+    // MG pools
+    double *r64 = nullptr, *e64 = nullptr, *G64 = nullptr;
+    float *r32 = nullptr, *e32 = nullptr, *G32 = nullptr;
+    // per-box device state
+    int *active = nullptr, *iters = nullptr, *failed = nullptr, *err_flag = nullptr;
+    unsigned long long *nr = nullptr, *nf = nullptr;
+    double *state = nullptr, *solv = nullptr, *dv_size = nullptr;
+    BoxDev* dboxes = nullptr;
+    // host mirror of per-box flags
+    int* h_flags = nullptr;  // pinned: active[nbox] iters[nbox] failed[nbox] err
+    double* h_state = nullptr;
+
+    // launch tables
+    CellSeg* seg_pool = nullptr;
+    std::vector<CellSeg> seg_host;
+    struct Table {
+        int first = 0, count = 0, blocks = 0;
+    };
+    std::vector<Table> t_full, t_half, t_coarse;  // per depth (non-bottom boxes)
+    Table t_bottom, t_l0;
+    BoxSeg* pseg = nullptr;
+    int pseg_n = 0, pseg_blocks = 0;
+    BoxSeg* tseg = nullptr;
+    int tseg_n = 0, tseg_blocks = 0;
+
+    // timing
+    bool timing = false;
+    std::map<std::string, std::vector<EventPair>> pending;
+    std::map<std::string, std::pair<long long, double>> timed;
+    std::vector<hipEvent_t> event_pool;
+
+    // graph
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    double g_key[6] = {0, 0, 0, 0, 0, 0};
+    int g_prec = -1;
+
+    long long device_bytes = 0;
+
+    CellSeg* segs(const Table& t) const { return seg_pool + t.first; }
+};
+
+#define FAIL(ctx, code, ...)                         \
+    do {                                             \
+        char _b[512];                                \
+        snprintf(_b, sizeof(_b), __VA_ARGS__);       \
+        (ctx)->err = _b;                             \
+        return (code);                               \
+    } while (0)
+
+#define HIPCK(ctx, call)                                                                      \
+    do {                                                                                      \
+        hipError_t _e = (call);                                                               \
+        if (_e != hipSuccess) FAIL(ctx, NYXPM_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(_e)); \
+    } while (0)
+
+#define LAUNCHCK(ctx)                                                                          \
+    do {                                                                                       \
+        hipError_t _e = hipGetLastError();                                                     \
+        if (_e != hipSuccess) FAIL(ctx, NYXPM_ERR_HIP, "kernel launch failed: %s", hipGetErrorString(_e)); \
+    } while (0)
+
+namespace {
+
+template <typename T>
+int dalloc(nyxpm_ctx* c, T** p, size_t n) {
+    *p = nullptr;
+    if (n == 0) return NYXPM_OK;
+    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        FAIL(c, NYXPM_ERR_NOMEM, "hipMalloc of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
+    }
+    c->device_bytes += (long long)(n * sizeof(T));
+    return NYXPM_OK;
+}
+
+template <typename T>
+void dfree(T*& p) {
+    if (p) hipFree((void*)p);
+    p = nullptr;
+}
+
+// Timing: bracket a launch of class `name` with events when enabled.
+struct Timed {
+    nyxpm_ctx* c;
+    std::string name;
+    hipEvent_t a = nullptr, b = nullptr;
+    hipEvent_t get() {
+        if (!c->event_pool.empty()) {
+            hipEvent_t e = c->event_pool.back();
+            c->event_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        hipEventCreate(&e);
+        return e;
+    }
+    Timed(nyxpm_ctx* c_, const char* n_) : c(c_), name(n_) {
+        if (c->timing) {
+            a = get();
+            b = get();
+            hipEventRecord(a, c->stream);
+        }
+    }
+    ~Timed() {
+        if (c->timing) {
+            hipEventRecord(b, c->stream);
+            c->pending[name].push_back({a, b});
+        }
+    }
+};
+
+int collect_timing(nyxpm_ctx* c) {
+    if (c->pending.empty()) return NYXPM_OK;
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    for (auto& kv : c->pending) {
+        auto& acc = c->timed[kv.first];
+        for (auto& ep : kv.second) {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, ep.a, ep.b);
+            acc.first += 1;
+            acc.second += ms;
+            c->event_pool.push_back(ep.a);
+            c->event_pool.push_back(ep.b);
+        }
+    }
+    c->pending.clear();
+    return NYXPM_OK;
+}
+
+char g_name_buf[64];
+const char* lvl_name(const char* base, int d) {
+    snprintf(g_name_buf, sizeof(g_name_buf), "%s.d%d", base, d);
+    return g_name_buf;
+}
+
+// ---------------------------------------------------------------------------
+// setup
+// ---------------------------------------------------------------------------
+
+int setup_boxes(nyxpm_ctx* c, int nbox, const nyxpm_geom* geoms) {
+    c->boxes.resize(nbox);
+    long long cell = 0, tiles = 0, mg = 0, Gtot = 0;
+    c->dmax = 1;
+    for (int b = 0; b < nbox; ++b) {
+        Box& B = c->boxes[b];
+        const nyxpm_geom& g = geoms[b];
+        for (int d = 0; d < 3; ++d) {
+            if (g.n_cell[d] < 2) FAIL(c, NYXPM_ERR_ARG, "box %d: n_cell[%d]=%d must be >= 2", b, d, g.n_cell[d]);
+            if (!(g.prob_hi[d] > g.prob_lo[d])) FAIL(c, NYXPM_ERR_ARG, "box %d: prob_hi <= prob_lo", b);
+            B.n[d] = g.n_cell[d];
+            B.plo[d] = g.prob_lo[d];
+            B.phi[d] = g.prob_hi[d];
+            B.dx[d] = (B.phi[d] - B.plo[d]) / (double)B.n[d];
+            B.dxi[d] = 1.0 / B.dx[d];
+            B.rhi[d] = domain_rhi(B.plo[d], B.phi[d], B.n[d]);
+            B.len[d] = B.phi[d] - B.plo[d];
+            B.tile[d] = std::min(c->tile[d], B.n[d]);
+            B.ntile[d] = (B.n[d] + B.tile[d] - 1) / B.tile[d];
+        }
+        B.dv = B.dx[0] * B.dx[1] * B.dx[2];
+        B.inv_vol = 1.0 / B.dv;
+        B.prob_size = (B.phi[0] - B.plo[0]) * (B.phi[1] - B.plo[1]) * (B.phi[2] - B.plo[2]);
+        int dims[kMaxLevels][3];
+        B.nlev = mg_levels(B.n, dims);
+        long long nb = (long long)dims[B.nlev - 1][0] * dims[B.nlev - 1][1] * dims[B.nlev - 1][2];
+        if (nb > kBottomMaxCells)
+            FAIL(c, NYXPM_ERR_UNSUPPORTED,
+                 "box %d: %dx%dx%d coarsens only to %dx%dx%d (%lld cells); the exact bottom solve "
+                 "takes at most %d cells -- use dimensions of the form m*2^p with small m",
+                 b, B.n[0], B.n[1], B.n[2], dims[B.nlev - 1][0], dims[B.nlev - 1][1], dims[B.nlev - 1][2], nb,
+                 kBottomMaxCells);
+        for (int l = 0; l < B.nlev; ++l) {
+            Level& L = B.lev[l];
+            for (int d = 0; d < 3; ++d) {
+                L.n[d] = dims[l][d];
+                double dxl = B.dx[d] * (double)(B.n[d] / dims[l][d]);
+                double dxil = 1.0 / dxl;
+                L.dh[d] = dxil * dxil;
+            }
+            L.gamma = -2.0 * (L.dh[0] + L.dh[1] + L.dh[2]);
+            L.inv_gamma = 1.0 / L.gamma;
+            L.nc = (long long)L.n[0] * L.n[1] * L.n[2];
+            L.off = mg;
+            mg += L.nc;
+        }
+        B.G_off = Gtot;
+        Gtot += nb;
+        B.cell_off = cell;
+        cell += B.lev[0].nc;
+        B.tile_off = tiles;
+        B.ntiles = (long long)B.ntile[0] * B.ntile[1] * B.ntile[2];
+        tiles += B.ntiles;
+        c->dmax = std::max(c->dmax, B.nlev);
+    }
+    c->ncell_total = cell;
+    c->ntiles_total = tiles;
+    c->mg_total = mg;
+    c->G_total = Gtot;
+    c->halo_stride = (c->tile[0] + 1) * (c->tile[1] + 1) * (c->tile[2] + 1) - c->tile[0] * c->tile[1] * c->tile[2];
+    return NYXPM_OK;
+}
+
+int build_tables(nyxpm_ctx* c) {
+    std::vector<CellSeg>& S = c->seg_host;
+    S.clear();
+    const int nbox = (int)c->boxes.size();
+    auto mk = [&](int b, int l, long long per) {
+        const Box& B = c->boxes[b];
+        const Level& L = B.lev[l];
+        CellSeg s{};
+        s.box = b;
+        for (int d = 0; d < 3; ++d) {
+            s.n[d] = L.n[d];
+            s.dh[d] = L.dh[d];
+        }
+        s.gamma = L.gamma;
+        s.inv_gamma = L.inv_gamma;
+        s.off = L.off;
+        s.coff = (l + 1 < B.nlev) ? B.lev[l + 1].off : 0;
+        s.nblk = nblocks(per);
+        return s;
+    };
+    auto add_table = [&](std::vector<CellSeg> v) {
+        nyxpm_ctx::Table t;
+        t.first = (int)S.size();
+        t.count = (int)v.size();
+        int blk = 0;
+        for (auto& s : v) {
+            s.blk0 = blk;
+            blk += s.nblk;
+            S.push_back(s);
+        }
+        t.blocks = blk;
+        return t;
+    };
+    c->t_full.assign(c->dmax, {});
+    c->t_half.assign(c->dmax, {});
+    c->t_coarse.assign(c->dmax, {});
+    for (int d = 0; d + 1 < c->dmax; ++d) {
+        std::vector<CellSeg> vf, vh, vc;
+        for (int b = 0; b < nbox; ++b) {
+            const Box& B = c->boxes[b];
+            if (d + 1 >= B.nlev) continue;
+            const Level& L = B.lev[d];
+            vf.push_back(mk(b, d, L.nc));
+            vh.push_back(mk(b, d, L.nc / 2));
+            vc.push_back(mk(b, d, L.nc / 8));
+        }
+        c->t_full[d] = add_table(vf);
+        c->t_half[d] = add_table(vh);
+        c->t_coarse[d] = add_table(vc);
+    }
+    {
+        std::vector<CellSeg> vb;
+        for (int b = 0; b < nbox; ++b) {
+            const Box& B = c->boxes[b];
+            CellSeg s = mk(b, B.nlev - 1, 1);
+            s.coff = B.G_off;
+            s.nblk = 1;
+            vb.push_back(s);
+        }
+        c->t_bottom = add_table(vb);
+    }
+    {
+        std::vector<CellSeg> v0;
+        for (int b = 0; b < nbox; ++b) {
+            CellSeg s = mk(b, 0, c->boxes[b].lev[0].nc);
+            s.coff = c->boxes[b].cell_off;  // level-0 field pools
+            v0.push_back(s);
+        }
+        c->t_l0 = add_table(v0);
+    }
+    int rc = dalloc(c, &c->seg_pool, S.size());
+    if (rc) return rc;
+    HIPCK(c, hipMemcpy(c->seg_pool, S.data(), S.size() * sizeof(CellSeg), hipMemcpyHostToDevice));
+
+    // tiles (one block per tile)
+    std::vector<BoxSeg> ts;
+    int blk = 0;
+    for (int b = 0; b < nbox; ++b) {
+        BoxSeg s{b, blk, (int)c->boxes[b].ntiles, 0};
+        blk += s.nblk;
+        ts.push_back(s);
+    }
+    rc = dalloc(c, &c->tseg, ts.size());
+    if (rc) return rc;
+    HIPCK(c, hipMemcpy(c->tseg, ts.data(), ts.size() * sizeof(BoxSeg), hipMemcpyHostToDevice));
+    c->tseg_n = nbox;
+    c->tseg_blocks = blk;
+    return NYXPM_OK;
+}
+
+int upload_boxdev(nyxpm_ctx* c) {
+    std::vector<BoxDev> v(c->boxes.size());
+    std::vector<double> dvs(2 * c->boxes.size());
+    for (size_t b = 0; b < c->boxes.size(); ++b) {
+        const Box& B = c->boxes[b];
+        BoxDev& D = v[b];
+        for (int d = 0; d < 3; ++d) {
+            D.n[d] = B.n[d];
+            D.tile[d] = B.tile[d];
+            D.ntile[d] = B.ntile[d];
+            D.plo[d] = B.plo[d];
+            D.dxi[d] = B.dxi[d];
+            D.rhi[d] = B.rhi[d];
+            D.len[d] = B.len[d];
+        }
+        D.fx_exp = B.fx_exp;
+        D.cell_off = B.cell_off;
+        D.tile_off = B.tile_off;
+        D.part_off = B.part_off;
+        D.np = B.np;
+        D.inv_vol = B.inv_vol;
+        D.mass_offset = B.mass_offset;
+        D.m_const = B.m_const;
+        D.has_mass = B.has_mass ? 1 : 0;
+        D.halo_stride = c->halo_stride;
+        dvs[2 * b] = B.dv;
+        dvs[2 * b + 1] = B.prob_size;
+    }
+    HIPCK(c, hipMemcpyAsync(c->dboxes, v.data(), v.size() * sizeof(BoxDev), hipMemcpyHostToDevice, c->stream));
+    HIPCK(c, hipMemcpyAsync(c->dv_size, dvs.data(), dvs.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    return NYXPM_OK;
+}
+
+int build_pseg(nyxpm_ctx* c) {
+    std::vector<BoxSeg> ps;
+    int blk = 0;
+    for (size_t b = 0; b < c->boxes.size(); ++b) {
+        BoxSeg s{(int)b, blk, std::max(1, nblocks(c->boxes[b].np)), 0};
+        blk += s.nblk;
+        ps.push_back(s);
+    }
+    if (!c->pseg) {
+        int rc = dalloc(c, &c->pseg, ps.size());
+        if (rc) return rc;
+    }
+    HIPCK(c, hipMemcpy(c->pseg, ps.data(), ps.size() * sizeof(BoxSeg), hipMemcpyHostToDevice));
+    c->pseg_n = (int)ps.size();
+    c->pseg_blocks = blk;
+    return NYXPM_OK;
+}
+
+int alloc_fields(nyxpm_ctx* c) {
+    const int nbox = (int)c->boxes.size();
+    int rc;
+    long long maxc = 0;
+    for (auto& B : c->boxes) maxc = std::max(maxc, B.lev[0].nc);
+    if ((rc = dalloc(c, &c->own, c->ncell_total))) return rc;
+    if ((rc = dalloc(c, &c->rho, c->ncell_total))) return rc;
+    if ((rc = dalloc(c, &c->f, c->ncell_total))) return rc;
+    if ((rc = dalloc(c, &c->phi, c->ncell_total))) return rc;
+    if ((rc = dalloc(c, &c->gtmp, 3 * maxc))) return rc;
+    if ((rc = dalloc(c, &c->halo, (size_t)c->ntiles_total * c->halo_stride))) return rc;
+    if ((rc = dalloc(c, &c->partial, (size_t)c->t_l0.blocks))) return rc;
+    if ((rc = dalloc(c, &c->counts, c->ntiles_total))) return rc;
+    if ((rc = dalloc(c, &c->cursor, c->ntiles_total))) return rc;
+    if ((rc = dalloc(c, &c->tstart, c->ntiles_total + 1))) return rc;
+    c->scan_blocks = nblocks(c->ntiles_total, kBlock * kScanItems);
+    if ((rc = dalloc(c, &c->bsum, c->scan_blocks + 1))) return rc;
+    if ((rc = dalloc(c, &c->r64, c->mg_total))) return rc;
+    if ((rc = dalloc(c, &c->e64, c->mg_total))) return rc;
+    if ((rc = dalloc(c, &c->G64, c->G_total))) return rc;
+    if ((rc = dalloc(c, &c->G32, c->G_total))) return rc;
+    if ((rc = dalloc(c, &c->active, nbox))) return rc;
+    if ((rc = dalloc(c, &c->iters, nbox))) return rc;
+    if ((rc = dalloc(c, &c->failed, nbox))) return rc;
+    if ((rc = dalloc(c, &c->err_flag, 1))) return rc;
+    if ((rc = dalloc(c, &c->nr, nbox))) return rc;
+    if ((rc = dalloc(c, &c->nf, nbox))) return rc;
+    if ((rc = dalloc(c, &c->state, 4 * nbox))) return rc;
+    if ((rc = dalloc(c, &c->solv, nbox))) return rc;
+    if ((rc = dalloc(c, &c->dv_size, 2 * nbox))) return rc;
+    if ((rc = dalloc(c, &c->dboxes, nbox))) return rc;
+    HIPCK(c, hipHostMalloc((void**)&c->h_flags, (3 * nbox + 1) * sizeof(int), hipHostMallocDefault));
+    HIPCK(c, hipHostMalloc((void**)&c->h_state, 4 * nbox * sizeof(double), hipHostMallocDefault));
+    HIPCK(c, hipMemset(c->phi, 0, c->ncell_total * sizeof(double)));
+    HIPCK(c, hipMemset(c->nr, 0, nbox * sizeof(unsigned long long)));
+    HIPCK(c, hipMemset(c->nf, 0, nbox * sizeof(unsigned long long)));
+    HIPCK(c, hipMemset(c->err_flag, 0, sizeof(int)));
+    HIPCK(c, hipMemset(c->counts, 0, c->ntiles_total * sizeof(int)));
+    // bottom Green's functions
+    std::vector<double> G(c->G_total);
+    std::vector<float> G32(c->G_total);
+    for (auto& B : c->boxes) {
+        const Level& L = B.lev[B.nlev - 1];
+        bottom_green(L.n, L.dh, G.data() + B.G_off);
+    }
+    for (long long q = 0; q < c->G_total; ++q) G32[q] = (float)G[q];
+    HIPCK(c, hipMemcpy(c->G64, G.data(), G.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCK(c, hipMemcpy(c->G32, G32.data(), G32.size() * sizeof(float), hipMemcpyHostToDevice));
+    return NYXPM_OK;
+}
+
+int ensure_mixed_pools(nyxpm_ctx* c) {
+    int rc;
+    if (!c->r32 && (rc = dalloc(c, &c->r32, c->mg_total))) return rc;
+    if (!c->e32 && (rc = dalloc(c, &c->e32, c->mg_total))) return rc;
+    return NYXPM_OK;
+}
+
+void free_particles(nyxpm_ctx* c) {
+    for (int s = 0; s < 2; ++s) {
+        dfree(c->P[s].x); dfree(c->P[s].y); dfree(c->P[s].z);
+        dfree(c->P[s].vx); dfree(c->P[s].vy); dfree(c->P[s].vz);
+        dfree(c->P[s].m); dfree(c->P[s].id);
+    }
+    dfree(c->key);
+}
+
+int alloc_particle_set(nyxpm_ctx* c, PartPtrs& P, long long n, bool mass) {
+    int rc;
+    if ((rc = dalloc(c, &P.x, n))) return rc;
+    if ((rc = dalloc(c, &P.y, n))) return rc;
+    if ((rc = dalloc(c, &P.z, n))) return rc;
+    if ((rc = dalloc(c, &P.vx, n))) return rc;
+    if ((rc = dalloc(c, &P.vy, n))) return rc;
+    if ((rc = dalloc(c, &P.vz, n))) return rc;
+    if (mass && (rc = dalloc(c, &P.m, n))) return rc;
+    if ((rc = dalloc(c, &P.id, n))) return rc;
+    return NYXPM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// device phases
+// ---------------------------------------------------------------------------
+
+int err_check(nyxpm_ctx* c) {
+    int e = 0;
+    HIPCK(c, hipMemcpyAsync(&e, c->err_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    if (e) {
+        hipMemsetAsync(c->err_flag, 0, sizeof(int), c->stream);
+        FAIL(c, NYXPM_ERR_STATE,
+             "particle outside one periodic image of its box or binning inconsistent (flags 0x%x)", e);
+    }
+    return NYXPM_OK;
+}
+
+int phase_keys(nyxpm_ctx* c) {
+    PartPtrs& A = c->P[c->cur];
+    {
+        Timed t(c, "keys");
+        hipLaunchKernelGGL(k_fill_int, dim3(nblocks(c->ntiles_total)), dim3(kBlock), 0, c->stream, c->counts,
+                           c->ntiles_total, 0);
+        hipLaunchKernelGGL(k_keys, dim3(c->pseg_blocks), dim3(kBlock), 0, c->stream, c->pseg, c->pseg_n, c->dboxes,
+                           A.x, A.y, A.z, c->key, c->counts, c->err_flag);
+    }
+    LAUNCHCK(c);
+    c->keys_valid = true;
+    return NYXPM_OK;
+}
+
+int phase_sort(nyxpm_ctx* c) {
+    PartPtrs& A = c->P[c->cur];
+    PartPtrs& B = c->P[1 - c->cur];
+    {
+        Timed t(c, "scan");
+        hipLaunchKernelGGL(k_scan_reduce, dim3(c->scan_blocks), dim3(kBlock), 0, c->stream, c->counts,
+                           c->ntiles_total, c->bsum);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, c->stream, c->bsum, c->scan_blocks);
+        hipLaunchKernelGGL(k_scan_down, dim3(c->scan_blocks), dim3(kBlock), 0, c->stream, c->counts,
+                           c->ntiles_total, c->bsum, c->tstart, c->cursor);
+    }
+    {
+        Timed t(c, "scatter");
+        hipLaunchKernelGGL(k_scatter, dim3(c->pseg_blocks), dim3(kBlock), 0, c->stream, c->pseg, c->pseg_n,
+                           c->dboxes, c->key, c->tstart, c->cursor, A, B);
+    }
+    LAUNCHCK(c);
+    c->cur = 1 - c->cur;
+    c->sorted_valid = true;
+    c->keys_valid = false;
+    return NYXPM_OK;
+}
+
+int phase_deposit(nyxpm_ctx* c) {
+    if (!c->sorted_valid) {
+        int rc;
+        if (!c->keys_valid && (rc = phase_keys(c))) return rc;
+        if ((rc = phase_sort(c))) return rc;
+    }
+    PartPtrs& A = c->P[c->cur];
+    const size_t lds = 2 * sizeof(unsigned long long) * (c->tile[0] + 1) * (c->tile[1] + 1) * (c->tile[2] + 1);
+    {
+        Timed t(c, "deposit");
+        hipLaunchKernelGGL(k_deposit, dim3(c->tseg_blocks), dim3(kBlock), lds, c->stream, c->tseg, c->tseg_n,
+                           c->dboxes, c->tstart, A.x, A.y, A.z, A.m, c->own, c->halo, c->err_flag);
+    }
+    {
+        Timed t(c, "fold");
+        hipLaunchKernelGGL(k_fold, dim3(c->t_l0.blocks), dim3(kBlock), 0, c->stream, c->segs(c->t_l0),
+                           c->t_l0.count, c->dboxes, c->own, c->halo, c->rho, c->f, c->partial);
+    }
+    LAUNCHCK(c);
+    c->density_valid = true;
+    c->f1_valid = true;
+    return NYXPM_OK;
+}
+
+// f from the current density: solvability offset and scaling.
+int phase_rhs(nyxpm_ctx* c, double a, double four_pi_G) {
+    const double Ggravity = -four_pi_G;
+    const double inv_a = 1.0 / a;
+    {
+        Timed t(c, "rhs");
+        if (!c->f1_valid)
+            hipLaunchKernelGGL(k_rhs1, dim3(c->t_l0.blocks), dim3(kBlock), 0, c->stream, c->segs(c->t_l0),
+                               c->t_l0.count, c->dboxes, c->rho, c->f, c->partial);
+        hipLaunchKernelGGL(k_rhs_sum, dim3(c->t_l0.count), dim3(kBlock), 0, c->stream, c->segs(c->t_l0),
+                           c->partial, c->dv_size, c->solv);
+        hipLaunchKernelGGL(k_rhs_final, dim3(c->t_l0.blocks), dim3(kBlock), 0, c->stream, c->segs(c->t_l0),
+                           c->t_l0.count, c->dboxes, c->solv, c->f, Ggravity, inv_a);
+    }
+    LAUNCHCK(c);
+    c->f1_valid = false;
+    return NYXPM_OK;
+}
+
+template <typename T>
+struct Pools;
+template <>
+struct Pools<double> {
+    static double* r(nyxpm_ctx* c) { return c->r64; }
+    static double* e(nyxpm_ctx* c) { return c->e64; }
+    static double* G(nyxpm_ctx* c) { return c->G64; }
+};
+template <>
+struct Pools<float> {
+    static float* r(nyxpm_ctx* c) { return c->r32; }
+    static float* e(nyxpm_ctx* c) { return c->e32; }
+    static float* G(nyxpm_ctx* c) { return c->G32; }
+};
+
+template <typename T>
+int vcycle(nyxpm_ctx* c) {
+    T* r = Pools<T>::r(c);
+    T* e = Pools<T>::e(c);
+    const int D = c->dmax;
+    for (int d = 0; d + 1 < D; ++d) {
+        auto& tf = c->t_full[d];
+        auto& th = c->t_half[d];
+        auto& tc = c->t_coarse[d];
+        if (tf.count == 0) continue;
+        {
+            Timed t(c, lvl_name("fill", d));
+            hipLaunchKernelGGL(k_fill<T>, dim3(tf.blocks), dim3(kBlock), 0, c->stream, c->segs(tf), tf.count,
+                               c->active, e, T(0));
+        }
+        for (int s = 0; s < c->nu1; ++s)
+            for (int col = 0; col < 2; ++col) {
+                Timed t(c, lvl_name("smooth", d));
+                hipLaunchKernelGGL(k_gsrb_half<T>, dim3(th.blocks), dim3(kBlock), 0, c->stream, c->segs(th),
+                                   th.count, c->active, e, r, col);
+            }
+        {
+            Timed t(c, lvl_name("resid_restrict", d));
+            hipLaunchKernelGGL(k_resid_restrict<T>, dim3(tc.blocks), dim3(kBlock), 0, c->stream, c->segs(tc),
+                               tc.count, c->active, e, r);
+        }
+    }
+    {
+        Timed t(c, "bottom");
+        hipLaunchKernelGGL(k_bottom<T>, dim3(c->t_bottom.count), dim3(kBlock), 0, c->stream, c->segs(c->t_bottom),
+                           c->t_bottom.count, c->active, e, r, Pools<T>::G(c));
+    }
+    for (int d = D - 2; d >= 0; --d) {
+        auto& tf = c->t_full[d];
+        auto& th = c->t_half[d];
+        if (tf.count == 0) continue;
+        {
+            Timed t(c, lvl_name("prolong", d));
+            hipLaunchKernelGGL(k_prolong<T>, dim3(tf.blocks), dim3(kBlock), 0, c->stream, c->segs(tf), tf.count,
+                               c->active, e);
+        }
+        for (int s = 0; s < c->nu2; ++s)
+            for (int col = 0; col < 2; ++col) {
+                Timed t(c, lvl_name("smooth", d));
+                hipLaunchKernelGGL(k_gsrb_half<T>, dim3(th.blocks), dim3(kBlock), 0, c->stream, c->segs(th),
+                                   th.count, c->active, e, r, col);
+            }
+    }
+    LAUNCHCK(c);
+    return NYXPM_OK;
+}
+
+template <typename T>
+int residual(nyxpm_ctx* c, bool with_f) {
+    Timed t(c, "residual");
+    hipLaunchKernelGGL(k_residual0<T>, dim3(c->t_l0.blocks), dim3(kBlock), 0, c->stream, c->segs(c->t_l0),
+                       c->t_l0.count, c->active, c->f, c->phi, Pools<T>::r(c), c->nr, with_f ? c->nf : nullptr);
+    return NYXPM_OK;
+}
+
+template <typename T>
+int update(nyxpm_ctx* c) {
+    Timed t(c, "update");
+    hipLaunchKernelGGL(k_update0<T>, dim3(c->t_l0.blocks), dim3(kBlock), 0, c->stream, c->segs(c->t_l0),
+                       c->t_l0.count, c->active, c->phi, Pools<T>::e(c));
+    return NYXPM_OK;
+}
+
+int check(nyxpm_ctx* c, bool init, int fixed_iters, double rel, double abs_tol) {
+    const int nbox = (int)c->boxes.size();
+    hipLaunchKernelGGL(k_check, dim3(nblocks(nbox, 64)), dim3(64), 0, c->stream, nbox, init ? 1 : 0, fixed_iters,
+                       c->max_iter, rel, abs_tol, c->active, c->iters, c->failed, c->nr, c->nf, c->state);
+    LAUNCHCK(c);
+    return NYXPM_OK;
+}
+
+int read_flags(nyxpm_ctx* c, bool* any_active) {
+    const int nbox = (int)c->boxes.size();
+    HIPCK(c, hipMemcpyAsync(c->h_flags, c->active, nbox * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    bool a = false;
+    for (int b = 0; b < nbox; ++b) a = a || c->h_flags[b];
+    *any_active = a;
+    return NYXPM_OK;
+}
+
+// One multigrid solve of L phi = f on every box, from the current phi.  With
+// `graph_cycles` > 0 the number of cycles issued is fixed (device flags stop
+// converged boxes) and nothing is read back, so the sequence can be captured.
+template <typename T>
+int solve_impl(nyxpm_ctx* c, double rel, double abs_tol, int fixed_iters, int graph_cycles) {
+    const int nbox = (int)c->boxes.size();
+    int rc;
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(kBlock), 0, c->stream, c->active, (long long)nbox, 1);
+    if ((rc = residual<T>(c, true))) return rc;
+    if ((rc = check(c, true, fixed_iters, rel, abs_tol))) return rc;
+    if (graph_cycles > 0) {
+        for (int it = 0; it < graph_cycles; ++it) {
+            if ((rc = vcycle<T>(c))) return rc;
+            if ((rc = update<T>(c))) return rc;
+            if ((rc = residual<T>(c, false))) return rc;
+            if ((rc = check(c, false, fixed_iters, rel, abs_tol))) return rc;
+        }
+        return NYXPM_OK;
+    }
+    const int limit = fixed_iters > 0 ? fixed_iters : c->max_iter;
+    for (int it = 0; it <= limit; ++it) {
+        bool any;
+        if ((rc = read_flags(c, &any))) return rc;
+        if (!any) break;
+        if ((rc = vcycle<T>(c))) return rc;
+        if ((rc = update<T>(c))) return rc;
+        if ((rc = residual<T>(c, false))) return rc;
+        if ((rc = check(c, false, fixed_iters, rel, abs_tol))) return rc;
+    }
+    return NYXPM_OK;
+}
+
+int solve(nyxpm_ctx* c, double rel, double abs_tol, int fixed_iters, int graph_cycles) {
+    if (c->precision == NYXPM_MIXED) {
+        int rc = ensure_mixed_pools(c);
+        if (rc) return rc;
+        return solve_impl<float>(c, rel, abs_tol, fixed_iters, graph_cycles);
+    }
+    return solve_impl<double>(c, rel, abs_tol, fixed_iters, graph_cycles);
+}
+
+// Per-box results of the last solve; NYXPM_ERR_NOT_CONVERGED if any box failed.
+int solve_results(nyxpm_ctx* c, int* iters, double* final_res, bool allow_active) {
+    const int nbox = (int)c->boxes.size();
+    HIPCK(c, hipMemcpyAsync(c->h_flags, c->active, nbox * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipMemcpyAsync(c->h_flags + nbox, c->iters, nbox * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipMemcpyAsync(c->h_flags + 2 * nbox, c->failed, nbox * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipMemcpyAsync(c->h_state, c->state, 4 * nbox * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    int bad = -1;
+    for (int b = 0; b < nbox; ++b) {
+        if (iters) iters[b] = c->h_flags[nbox + b];
+        if (final_res) final_res[b] = c->h_state[4 * b + 2];
+        if (c->h_flags[2 * nbox + b] || (!allow_active && c->h_flags[b])) bad = b;
+    }
+    if (bad >= 0)
+        FAIL(c, NYXPM_ERR_NOT_CONVERGED, "box %d: multigrid stopped at |r|=%g > target %g after %d cycles", bad,
+             c->h_state[4 * bad + 2], c->h_state[4 * bad + 3], c->h_flags[nbox + bad]);
+    return NYXPM_OK;
+}
+
+template <bool MOVE>
+int phase_kick(nyxpm_ctx* c, double dt, double a_prev, double a_cur) {
+    PartPtrs& A = c->P[c->cur];
+    if (MOVE) {
+        Timed t(c, "kick_drift");
+        hipLaunchKernelGGL(k_fill_int, dim3(nblocks(c->ntiles_total)), dim3(kBlock), 0, c->stream, c->counts,
+                           c->ntiles_total, 0);
+        hipLaunchKernelGGL(k_kick<true>, dim3(c->pseg_blocks), dim3(kBlock), 0, c->stream, c->pseg, c->pseg_n,
+                           c->dboxes, c->phi, A.x, A.y, A.z, A.vx, A.vy, A.vz, dt, a_prev, a_cur, c->key,
+                           c->counts, c->err_flag);
+        c->keys_valid = true;
+    } else {
+        Timed t(c, "kick");
+        hipLaunchKernelGGL(k_kick<false>, dim3(c->pseg_blocks), dim3(kBlock), 0, c->stream, c->pseg, c->pseg_n,
+                           c->dboxes, c->phi, A.x, A.y, A.z, A.vx, A.vy, A.vz, dt, a_prev, a_cur, c->key,
+                           c->counts, c->err_flag);
+        c->keys_valid = false;
+    }
+    LAUNCHCK(c);
+    c->sorted_valid = false;
+    c->density_valid = false;
+    return NYXPM_OK;
+}
+
+int check_box(nyxpm_ctx* c, int box) {
+    if (box < 0 || box >= (int)c->boxes.size()) FAIL(c, NYXPM_ERR_ARG, "box %d out of range", box);
+    return NYXPM_OK;
+}
+
+int require_particles(nyxpm_ctx* c) {
+    if (c->npart_total <= 0 || !c->P[0].x) FAIL(c, NYXPM_ERR_STATE, "no particles uploaded");
+    return NYXPM_OK;
+}
+
+// The canonical KDK step, in stream order; with graph_cycles > 0 nothing is
+// read back (capturable).
+int step_body(nyxpm_ctx* c, double dt, double a_old, double a_new, double four_pi_G, double rel, double abs_tol,
+              int nyx_faithful, int graph_cycles) {
+    int rc;
+    const double a_half = 0.5 * (a_old + a_new);
+    if (nyx_faithful) {
+        // Nyx_advance_particles.cpp:148-170: old phi re-solved from a zero guess
+        if ((rc = phase_keys(c))) return rc;
+        if ((rc = phase_sort(c))) return rc;
+        if ((rc = phase_deposit(c))) return rc;
+        if ((rc = phase_rhs(c, a_old, four_pi_G))) return rc;
+        hipLaunchKernelGGL(k_fill_dbl, dim3(nblocks(c->ncell_total)), dim3(kBlock), 0, c->stream, c->phi,
+                           c->ncell_total, 0.0);
+        if ((rc = solve(c, rel, abs_tol, 0, graph_cycles))) return rc;
+        if (graph_cycles == 0 && (rc = solve_results(c, nullptr, nullptr, false))) return rc;
+    }
+    if ((rc = phase_kick<true>(c, dt, a_old, a_half))) return rc;
+    if ((rc = phase_sort(c))) return rc;
+    if ((rc = phase_deposit(c))) return rc;
+    if ((rc = phase_rhs(c, a_new, four_pi_G))) return rc;
+    if ((rc = solve(c, rel, abs_tol, 0, graph_cycles))) return rc;
+    if ((rc = phase_kick<false>(c, dt, a_half, a_new))) return rc;
+    return NYXPM_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+int nyxpm_create_batch(nyxpm_ctx** out, int nbox, const nyxpm_geom* geoms, const nyxpm_opts* opts) {
+    if (!out) return NYXPM_ERR_ARG;
+    *out = nullptr;
+    if (nbox < 1 || !geoms) return NYXPM_ERR_ARG;
+    nyxpm_ctx* c = new nyxpm_ctx();
+    if (opts) {
+        if (opts->precision == NYXPM_MIXED) c->precision = NYXPM_MIXED;
+        if (opts->nu1 > 0) c->nu1 = opts->nu1;
+        if (opts->nu2 > 0) c->nu2 = opts->nu2;
+        if (opts->max_iter > 0) c->max_iter = opts->max_iter;
+        for (int d = 0; d < 3; ++d)
+            if (opts->tile[d] > 0) c->tile[d] = opts->tile[d];
+        c->verbose = opts->verbose;
+    }
+    *out = c;
+    for (int d = 0; d < 3; ++d)
+        if (c->tile[d] < 1 || c->tile[d] > 32) FAIL(c, NYXPM_ERR_ARG, "tile[%d]=%d outside [1,32]", d, c->tile[d]);
+    if ((c->tile[0] + 1) * (c->tile[1] + 1) * (c->tile[2] + 1) * 16 > 64 * 1024)
+        FAIL(c, NYXPM_ERR_ARG, "tile %dx%dx%d needs more than 64 KiB of LDS", c->tile[0], c->tile[1], c->tile[2]);
+    hipError_t e = hipStreamCreate(&c->stream);
+    if (e != hipSuccess) FAIL(c, NYXPM_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    c->own_stream = true;
+    int rc;
+    if ((rc = setup_boxes(c, nbox, geoms))) return rc;
+    if ((rc = build_tables(c))) return rc;
+    if ((rc = alloc_fields(c))) return rc;
+    if ((rc = build_pseg(c))) return rc;
+    if ((rc = upload_boxdev(c))) return rc;
+    return NYXPM_OK;
+}
+
+int nyxpm_create(nyxpm_ctx** out, const nyxpm_geom* geom, const nyxpm_opts* opts) {
+    return nyxpm_create_batch(out, 1, geom, opts);
+}
+
+int nyxpm_destroy(nyxpm_ctx* c) {
+    if (!c) return NYXPM_OK;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->gexec) hipGraphExecDestroy(c->gexec);
+    if (c->graph) hipGraphDestroy(c->graph);
+    free_particles(c);
+    dfree(c->counts); dfree(c->cursor); dfree(c->tstart); dfree(c->bsum);
+    dfree(c->own); dfree(c->halo); dfree(c->rho); dfree(c->f); dfree(c->phi); dfree(c->partial); dfree(c->gtmp);
+    dfree(c->r64); dfree(c->e64); dfree(c->G64); dfree(c->r32); dfree(c->e32); dfree(c->G32);
+    dfree(c->active); dfree(c->iters); dfree(c->failed); dfree(c->err_flag); dfree(c->nr); dfree(c->nf);
+    dfree(c->state); dfree(c->solv); dfree(c->dv_size); dfree(c->dboxes);
+    dfree(c->seg_pool); dfree(c->pseg); dfree(c->tseg);
+    if (c->h_flags) hipHostFree(c->h_flags);
+    if (c->h_state) hipHostFree(c->h_state);
+    for (auto& kv : c->pending)
+        for (auto& ep : kv.second) { hipEventDestroy(ep.a); hipEventDestroy(ep.b); }
+    for (auto ev : c->event_pool) hipEventDestroy(ev);
+    if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return NYXPM_OK;
+}
+
+int nyxpm_set_stream(nyxpm_ctx* c, void* s) {
+    if (!c) return NYXPM_ERR_ARG;
+    if (c->own_stream && c->stream) {
+        hipStreamSynchronize(c->stream);
+        hipStreamDestroy(c->stream);
+    }
+    c->own_stream = false;
+    c->stream = (hipStream_t)s;
+    if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
+    return NYXPM_OK;
+}
+
+int nyxpm_sync(nyxpm_ctx* c) {
+    if (!c) return NYXPM_ERR_ARG;
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    return err_check(c);
+}
+
+const char* nyxpm_last_error(const nyxpm_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int nyxpm_num_boxes(const nyxpm_ctx* c) { return c ? (int)c->boxes.size() : 0; }
+
+int nyxpm_set_precision(nyxpm_ctx* c, int precision) {
+    if (!c || (precision != NYXPM_FP64 && precision != NYXPM_MIXED)) return NYXPM_ERR_ARG;
+    c->precision = precision;
+    return NYXPM_OK;
+}
+
+int64_t nyxpm_num_particles(const nyxpm_ctx* c, int box) {
+    if (!c || box < 0 || box >= (int)c->boxes.size()) return -1;
+    return c->boxes[box].np;
+}
+
+int64_t nyxpm_device_bytes(const nyxpm_ctx* c) { return c ? c->device_bytes : 0; }
+
+static int upload_impl(nyxpm_ctx* c, int box, int64_t np, const double* x, const double* y, const double* z,
+                       const double* vx, const double* vy, const double* vz, const double* m, double m_const,
+                       hipMemcpyKind kind) {
+    if (!c) return NYXPM_ERR_ARG;
+    int rc;
+    if ((rc = check_box(c, box))) return rc;
+    if (np < 0 || np > (int64_t)0x7fffffff) FAIL(c, NYXPM_ERR_ARG, "np=%lld out of range", (long long)np);
+    if (np > 0 && (!x || !y || !z || !vx || !vy || !vz)) FAIL(c, NYXPM_ERR_ARG, "null particle array");
+    if (!m && !(m_const >= 0.0)) FAIL(c, NYXPM_ERR_ARG, "m_const must be >= 0");
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    const int nbox = (int)c->boxes.size();
+    // new layout
+    std::vector<long long> new_off(nbox), old_off(nbox), old_np(nbox);
+    long long tot = 0;
+    for (int b = 0; b < nbox; ++b) {
+        old_off[b] = c->boxes[b].part_off;
+        old_np[b] = c->boxes[b].np;
+        new_off[b] = tot;
+        tot += (b == box) ? np : c->boxes[b].np;
+    }
+    const bool need_mass = c->any_mass || (m != nullptr);
+    PartPtrs N[2] = {};
+    int* nkey = nullptr;
+    long long cap = std::max(tot, 1LL);
+    for (int s = 0; s < 2; ++s)
+        if ((rc = alloc_particle_set(c, N[s], cap, need_mass))) return rc;
+    if ((rc = dalloc(c, &nkey, cap))) return rc;
+    // keep the other boxes' particles
+    PartPtrs& O = c->P[c->cur];
+    for (int b = 0; b < nbox; ++b) {
+        if (b == box || old_np[b] == 0) continue;
+        const size_t nb8 = old_np[b] * sizeof(double);
+        const long long so = old_off[b], d = new_off[b];
+        HIPCK(c, hipMemcpyAsync(N[0].x + d, O.x + so, nb8, hipMemcpyDeviceToDevice, c->stream));
+        HIPCK(c, hipMemcpyAsync(N[0].y + d, O.y + so, nb8, hipMemcpyDeviceToDevice, c->stream));
+        HIPCK(c, hipMemcpyAsync(N[0].z + d, O.z + so, nb8, hipMemcpyDeviceToDevice, c->stream));
+        HIPCK(c, hipMemcpyAsync(N[0].vx + d, O.vx + so, nb8, hipMemcpyDeviceToDevice, c->stream));
+        HIPCK(c, hipMemcpyAsync(N[0].vy + d, O.vy + so, nb8, hipMemcpyDeviceToDevice, c->stream));
+        HIPCK(c, hipMemcpyAsync(N[0].vz + d, O.vz + so, nb8, hipMemcpyDeviceToDevice, c->stream));
+        HIPCK(c, hipMemcpyAsync(N[0].id + d, O.id + so, old_np[b] * sizeof(int), hipMemcpyDeviceToDevice, c->stream));
+        if (need_mass) {
+            if (O.m) {
+                HIPCK(c, hipMemcpyAsync(N[0].m + d, O.m + so, nb8, hipMemcpyDeviceToDevice, c->stream));
+            } else {
+                std::vector<double> mm(old_np[b], c->boxes[b].m_const);
+                HIPCK(c, hipMemcpyAsync(N[0].m + d, mm.data(), nb8, hipMemcpyHostToDevice, c->stream));
+                HIPCK(c, hipStreamSynchronize(c->stream));
+            }
+        }
+    }
+    const long long d = new_off[box];
+    const size_t n8 = (size_t)np * sizeof(double);
+    if (np > 0) {
+        HIPCK(c, hipMemcpyAsync(N[0].x + d, x, n8, kind, c->stream));
+        HIPCK(c, hipMemcpyAsync(N[0].y + d, y, n8, kind, c->stream));
+        HIPCK(c, hipMemcpyAsync(N[0].z + d, z, n8, kind, c->stream));
+        HIPCK(c, hipMemcpyAsync(N[0].vx + d, vx, n8, kind, c->stream));
+        HIPCK(c, hipMemcpyAsync(N[0].vy + d, vy, n8, kind, c->stream));
+        HIPCK(c, hipMemcpyAsync(N[0].vz + d, vz, n8, kind, c->stream));
+        if (need_mass) {
+            if (m) {
+                HIPCK(c, hipMemcpyAsync(N[0].m + d, m, n8, kind, c->stream));
+            } else {
+                std::vector<double> mm(np, m_const);
+                HIPCK(c, hipMemcpyAsync(N[0].m + d, mm.data(), n8, hipMemcpyHostToDevice, c->stream));
+                HIPCK(c, hipStreamSynchronize(c->stream));
+            }
+        }
+        hipLaunchKernelGGL(k_iota, dim3(nblocks(np)), dim3(kBlock), 0, c->stream, d, (long long)np, N[0].id);
+        LAUNCHCK(c);
+    }
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    free_particles(c);
+    c->P[0] = N[0];
+    c->P[1] = N[1];
+    c->key = nkey;
+    c->cur = 0;
+    c->part_cap = cap;
+    c->any_mass = need_mass;
+    for (int b = 0; b < nbox; ++b) {
+        c->boxes[b].part_off = new_off[b];
+        if (b == box) c->boxes[b].np = np;
+    }
+    c->npart_total = tot;
+    // mass bookkeeping for the box (sumParticleMass and the fixed-point unit)
+    Box& B = c->boxes[box];
+    double mref = 0.0, total = 0.0;
+    if (m) {
+        std::vector<double> hm;
+        const double* mp = m;
+        if (kind == hipMemcpyDeviceToDevice) {
+            hm.resize(np);
+            HIPCK(c, hipMemcpy(hm.data(), m, n8, hipMemcpyDeviceToHost));
+            mp = hm.data();
+        }
+        for (int64_t p = 0; p < np; ++p) {
+            if (!(mp[p] >= 0.0)) FAIL(c, NYXPM_ERR_ARG, "particle %lld has negative or NaN mass", (long long)p);
+            total += mp[p];
+            mref = std::max(mref, mp[p]);
+        }
+        B.has_mass = true;
+    } else {
+        total = (double)np * m_const;
+        mref = m_const;
+        B.has_mass = false;
+    }
+    // boxes uploaded earlier without masses keep m_const; with a mass array present
+    // every box reads the array
+    for (auto& BB : c->boxes) BB.has_mass = need_mass;
+    B.m_const = m ? 0.0 : m_const;
+    B.mass_total = total;
+    B.mass_offset = total / B.prob_size;
+    B.fx_exp = mref > 0.0 ? std::ilogb(mref) : 0;
+    if ((rc = upload_boxdev(c))) return rc;
+    if ((rc = build_pseg(c))) return rc;
+    c->keys_valid = c->sorted_valid = c->density_valid = false;
+    if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
+    return NYXPM_OK;
+}
+
+int nyxpm_particles_upload(nyxpm_ctx* c, int box, int64_t np, const double* x, const double* y, const double* z,
+                           const double* vx, const double* vy, const double* vz, const double* m, double m_const) {
+    return upload_impl(c, box, np, x, y, z, vx, vy, vz, m, m_const, hipMemcpyHostToDevice);
+}
+
+int nyxpm_particles_upload_device(nyxpm_ctx* c, int box, int64_t np, const double* x, const double* y,
+                                  const double* z, const double* vx, const double* vy, const double* vz,
+                                  const double* m, double m_const) {
+    return upload_impl(c, box, np, x, y, z, vx, vy, vz, m, m_const, hipMemcpyDeviceToDevice);
+}
+
+int nyxpm_particles_download(nyxpm_ctx* c, int box, double* x, double* y, double* z, double* vx, double* vy,
+                             double* vz) {
+    if (!c) return NYXPM_ERR_ARG;
+    int rc;
+    if ((rc = check_box(c, box))) return rc;
+    const Box& B = c->boxes[box];
+    if (B.np == 0) return NYXPM_OK;
+    PartPtrs& A = c->P[c->cur];
+    double* src[6] = {A.x, A.y, A.z, A.vx, A.vy, A.vz};
+    double* dst[6] = {x, y, z, vx, vy, vz};
+    // staging: the other particle buffer is free between phases
+    double* tmp = c->P[1 - c->cur].x;
+    for (int q = 0; q < 6; ++q) {
+        if (!dst[q]) continue;
+        hipLaunchKernelGGL(k_unsort, dim3(nblocks(B.np)), dim3(kBlock), 0, c->stream, B.part_off, B.np, A.id,
+                           src[q], tmp);
+        LAUNCHCK(c);
+        HIPCK(c, hipMemcpyAsync(dst[q], tmp, B.np * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCK(c, hipStreamSynchronize(c->stream));
+    }
+    return err_check(c);
+}
+
+int nyxpm_assign_density(nyxpm_ctx* c, int box, double* rho_out) {
+    if (!c) return NYXPM_ERR_ARG;
+    int rc;
+    if ((rc = require_particles(c))) return rc;
+    if (box >= 0 && (rc = check_box(c, box))) return rc;
+    if (rho_out && box < 0) FAIL(c, NYXPM_ERR_ARG, "rho_out needs a box index");
+    if ((rc = phase_deposit(c))) return rc;
+    if (rho_out) {
+        const Box& B = c->boxes[box];
+        HIPCK(c, hipMemcpyAsync(rho_out, c->rho + B.cell_off, B.lev[0].nc * sizeof(double), hipMemcpyDeviceToHost,
+                                c->stream));
+    }
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    return err_check(c);
+}
+
+int nyxpm_sum_particle_mass(nyxpm_ctx* c, int box, double* mass_out) {
+    if (!c || !mass_out) return NYXPM_ERR_ARG;
+    int rc;
+    if ((rc = check_box(c, box))) return rc;
+    *mass_out = c->boxes[box].mass_total;
+    return NYXPM_OK;
+}
+
+int nyxpm_set_density(nyxpm_ctx* c, int box, const double* rho_in) {
+    if (!c || !rho_in) return NYXPM_ERR_ARG;
+    int rc;
+    if ((rc = check_box(c, box))) return rc;
+    const Box& B = c->boxes[box];
+    HIPCK(c, hipMemcpyAsync(c->rho + B.cell_off, rho_in, B.lev[0].nc * sizeof(double), hipMemcpyHostToDevice,
+                            c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    c->density_valid = true;
+    c->f1_valid = false;
+    return NYXPM_OK;
+}
+
+int nyxpm_solve_for_phi(nyxpm_ctx* c, double a, double four_pi_G, double rel_tol, double abs_tol, int use_guess,
+                        int* iters, double* final_resnorm) {
+    if (!c) return NYXPM_ERR_ARG;
+    if (!(a > 0.0)) FAIL(c, NYXPM_ERR_ARG, "a must be > 0");
+    if (!c->density_valid) FAIL(c, NYXPM_ERR_STATE, "no density: call nyxpm_assign_density first");
+    int rc;
+    if ((rc = phase_rhs(c, a, four_pi_G))) return rc;
+    if (!use_guess)
+        hipLaunchKernelGGL(k_fill_dbl, dim3(nblocks(c->ncell_total)), dim3(kBlock), 0, c->stream, c->phi,
+                           c->ncell_total, 0.0);
+    if ((rc = solve(c, rel_tol, abs_tol, 0, 0))) return rc;
+    if ((rc = solve_results(c, iters, final_resnorm, false))) return rc;
+    c->phi_valid = true;
+    return err_check(c);
+}
+
+int nyxpm_solve_poisson(nyxpm_ctx* c, int box, const double* rhs, double* phi_io, double rel_tol, double abs_tol,
+                        int fixed_iters, int* iters, double* norms) {
+    if (!c || !rhs || !phi_io) return NYXPM_ERR_ARG;
+    int rc;
+    if ((rc = check_box(c, box))) return rc;
+    if (c->boxes.size() != 1) FAIL(c, NYXPM_ERR_ARG, "nyxpm_solve_poisson works on single-box contexts");
+    const Box& B = c->boxes[box];
+    HIPCK(c, hipMemcpyAsync(c->f + B.cell_off, rhs, B.lev[0].nc * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCK(c, hipMemcpyAsync(c->phi + B.cell_off, phi_io, B.lev[0].nc * sizeof(double), hipMemcpyHostToDevice,
+                            c->stream));
+    if ((rc = solve(c, rel_tol, abs_tol, fixed_iters, 0))) return rc;
+    int it = 0;
+    double fin = 0;
+    rc = solve_results(c, &it, &fin, false);
+    if (iters) *iters = it;
+    if (norms) for (int q = 0; q < 4; ++q) norms[q] = c->h_state[q];
+    HIPCK(c, hipMemcpyAsync(phi_io, c->phi + B.cell_off, B.lev[0].nc * sizeof(double), hipMemcpyDeviceToHost,
+                            c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    c->phi_valid = true;
+    return rc;
+}
+
+int nyxpm_get_grav_vector(nyxpm_ctx* c, int box, double* g_out) {
+    if (!c || !g_out) return NYXPM_ERR_ARG;
+    int rc;
+    if ((rc = check_box(c, box))) return rc;
+    const Box& B = c->boxes[box];
+    hipLaunchKernelGGL(k_grav_vector, dim3(nblocks(B.lev[0].nc)), dim3(kBlock), 0, c->stream, c->dboxes, box, c->phi,
+                       c->gtmp);
+    LAUNCHCK(c);
+    HIPCK(c, hipMemcpyAsync(g_out, c->gtmp, 3 * B.lev[0].nc * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    return NYXPM_OK;
+}
+
+int nyxpm_gather_accel(nyxpm_ctx* c, int box, double* g_out) {
+    if (!c || !g_out) return NYXPM_ERR_ARG;
+    int rc;
+    if ((rc = check_box(c, box))) return rc;
+    if ((rc = require_particles(c))) return rc;
+    const Box& B = c->boxes[box];
+    if (B.np == 0) return NYXPM_OK;
+    PartPtrs& A = c->P[c->cur];
+    double* tmp = nullptr;
+    if ((rc = dalloc(c, &tmp, 3 * B.np))) return rc;
+    hipLaunchKernelGGL(k_gather, dim3(nblocks(B.np)), dim3(kBlock), 0, c->stream, c->dboxes, box, c->phi, A.x, A.y,
+                       A.z, A.id, tmp, c->err_flag);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(g_out, tmp, 3 * B.np * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(tmp);
+    c->device_bytes -= 3 * B.np * (long long)sizeof(double);
+    if (e != hipSuccess) FAIL(c, NYXPM_ERR_HIP, "gather_accel: %s", hipGetErrorString(e));
+    return err_check(c);
+}
+
+static int field_io(nyxpm_ctx* c, int box, double* field, double* host, bool to_host) {
+    if (!c || !host) return NYXPM_ERR_ARG;
+    int rc;
+    if ((rc = check_box(c, box))) return rc;
+    const Box& B = c->boxes[box];
+    if (to_host)
+        HIPCK(c, hipMemcpyAsync(host, field + B.cell_off, B.lev[0].nc * sizeof(double), hipMemcpyDeviceToHost,
+                                c->stream));
+    else
+        HIPCK(c, hipMemcpyAsync(field + B.cell_off, host, B.lev[0].nc * sizeof(double), hipMemcpyHostToDevice,
+                                c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    return NYXPM_OK;
+}
+
+int nyxpm_get_phi(nyxpm_ctx* c, int box, double* out) { return field_io(c, box, c ? c->phi : nullptr, out, true); }
+int nyxpm_set_phi(nyxpm_ctx* c, int box, const double* in) {
+    int rc = field_io(c, box, c ? c->phi : nullptr, const_cast<double*>(in), false);
+    if (!rc) c->phi_valid = true;
+    return rc;
+}
+int nyxpm_get_rhs(nyxpm_ctx* c, int box, double* out) { return field_io(c, box, c ? c->f : nullptr, out, true); }
+
+int nyxpm_move_kick_drift(nyxpm_ctx* c, double dt, double a_old, double a_half) {
+    if (!c) return NYXPM_ERR_ARG;
+    int rc;
+    if ((rc = require_particles(c))) return rc;
+    if (!(a_half > 0.0)) FAIL(c, NYXPM_ERR_ARG, "a_half must be > 0");
+    if ((rc = phase_kick<true>(c, dt, a_old, a_half))) return rc;
+    return nyxpm_sync(c);
+}
+
+int nyxpm_move_kick(nyxpm_ctx* c, double dt, double a_new, double a_half) {
+    if (!c) return NYXPM_ERR_ARG;
+    int rc;
+    if ((rc = require_particles(c))) return rc;
+    if (!(a_new > 0.0)) FAIL(c, NYXPM_ERR_ARG, "a_new must be > 0");
+    if ((rc = phase_kick<false>(c, dt, a_half, a_new))) return rc;
+    return nyxpm_sync(c);
+}
+
+int nyxpm_redistribute(nyxpm_ctx* c) {
+    if (!c) return NYXPM_ERR_ARG;
+    int rc;
+    if ((rc = require_particles(c))) return rc;
+    PartPtrs& A = c->P[c->cur];
+    hipLaunchKernelGGL(k_wrap, dim3(c->pseg_blocks), dim3(kBlock), 0, c->stream, c->pseg, c->pseg_n, c->dboxes, A.x,
+                       A.y, A.z, c->err_flag);
+    LAUNCHCK(c);
+    c->keys_valid = c->sorted_valid = c->density_valid = false;
+    return nyxpm_sync(c);
+}
+
+int nyxpm_pm_init(nyxpm_ctx* c, double a, double four_pi_G, double rel_tol, double abs_tol) {
+    if (!c) return NYXPM_ERR_ARG;
+    int rc;
+    if ((rc = require_particles(c))) return rc;
+    if ((rc = phase_deposit(c))) return rc;
+    if ((rc = phase_rhs(c, a, four_pi_G))) return rc;
+    hipLaunchKernelGGL(k_fill_dbl, dim3(nblocks(c->ncell_total)), dim3(kBlock), 0, c->stream, c->phi,
+                       c->ncell_total, 0.0);
+    if ((rc = solve(c, rel_tol, abs_tol, 0, 0))) return rc;
+    if ((rc = solve_results(c, nullptr, nullptr, false))) return rc;
+    c->phi_valid = true;
+    return err_check(c);
+}
+
+int nyxpm_pm_steps(nyxpm_ctx* c, int nsteps, double dt, double a_old, double a_new, double four_pi_G,
+                   double rel_tol, double abs_tol, int nyx_faithful, int use_graph, int* iters) {
+    if (!c || nsteps < 0) return NYXPM_ERR_ARG;
+    int rc;
+    if ((rc = require_particles(c))) return rc;
+    if (!c->phi_valid && !nyx_faithful) FAIL(c, NYXPM_ERR_STATE, "call nyxpm_pm_init before stepping");
+    if (!(a_old > 0.0) || !(a_new > 0.0)) FAIL(c, NYXPM_ERR_ARG, "scale factors must be > 0");
+    for (int s = 0; s < nsteps; ++s) {
+        if (!use_graph) {
+            if ((rc = step_body(c, dt, a_old, a_new, four_pi_G, rel_tol, abs_tol, nyx_faithful, 0))) return rc;
+            if ((rc = solve_results(c, iters, nullptr, false))) return rc;
+            continue;
+        }
+        const double key[6] = {dt, a_old, a_new, four_pi_G, rel_tol, abs_tol + (nyx_faithful ? 1e300 : 0.0)};
+        bool same = c->gexec && c->g_prec == c->precision && std::memcmp(key, c->g_key, sizeof(key)) == 0;
+        if (!same) {
+            if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+            if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
+            if (c->precision == NYXPM_MIXED && (rc = ensure_mixed_pools(c))) return rc;
+            const bool was_timing = c->timing;
+            c->timing = false;
+            // the step's host-side bookkeeping (buffer swap) must be replayed
+            // consistently: capture two steps so the particle buffers return to
+            // their starting roles
+            HIPCK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int cur0 = c->cur;
+            rc = step_body(c, dt, a_old, a_new, four_pi_G, rel_tol, abs_tol, nyx_faithful, c->graph_cycles);
+            if (!rc) rc = step_body(c, dt, a_old, a_new, four_pi_G, rel_tol, abs_tol, nyx_faithful, c->graph_cycles);
+            hipGraph_t g = nullptr;
+            hipError_t ee = hipStreamEndCapture(c->stream, &g);
+            c->timing = was_timing;
+            if (rc) return rc;
+            if (ee != hipSuccess) FAIL(c, NYXPM_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ee));
+            c->cur = cur0;
+            c->graph = g;
+            HIPCK(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+            std::memcpy(c->g_key, key, sizeof(key));
+            c->g_prec = c->precision;
+        }
+        // one replay = two steps
+        if (s + 1 < nsteps) {
+            HIPCK(c, hipGraphLaunch(c->gexec, c->stream));
+            ++s;
+            // a box still active after the recorded cycles has not converged
+            if ((rc = solve_results(c, iters, nullptr, false))) return rc;
+        } else {
+            if ((rc = step_body(c, dt, a_old, a_new, four_pi_G, rel_tol, abs_tol, nyx_faithful, 0))) return rc;
+            if ((rc = solve_results(c, iters, nullptr, false))) return rc;
+        }
+    }
+    c->sorted_valid = false;
+    c->keys_valid = false;
+    c->density_valid = false;
+    return err_check(c);
+}
+
+int nyxpm_timing_enable(nyxpm_ctx* c, int on) {
+    if (!c) return NYXPM_ERR_ARG;
+    c->timing = on != 0;
+    return NYXPM_OK;
+}
+
+int nyxpm_timing_reset(nyxpm_ctx* c) {
+    if (!c) return NYXPM_ERR_ARG;
+    int rc = collect_timing(c);
+    c->timed.clear();
+    return rc;
+}
+
+int nyxpm_timing_get(nyxpm_ctx* c, const char* name, int64_t* count, double* mean_ms, double* total_ms) {
+    if (!c || !name) return NYXPM_ERR_ARG;
+    int rc = collect_timing(c);
+    if (rc) return rc;
+    auto it = c->timed.find(name);
+    long long n = it == c->timed.end() ? 0 : it->second.first;
+    double t = it == c->timed.end() ? 0.0 : it->second.second;
+    if (count) *count = n;
+    if (total_ms) *total_ms = t;
+    if (mean_ms) *mean_ms = n ? t / n : 0.0;
+    return NYXPM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,594 @@
+// particle_kernels.hip -- particle side of the PM step on gfx950:
+//   * tile binning (counting sort by deposit tile, wave-aggregated atomics),
+//   * deterministic CIC deposit (AssignDensity: NyxParticleContainer.H:90-97 ->
+//     amrex_deposit_cic + SumBoundary + 1/vol),
+//   * fold of tile halos + RHS preparation (Gravity.cpp:1207-1244, 283, 290),
+//   * fused central-difference gradient + CIC gather + kick(-drift)
+//     (get_new_grav_vector Gravity.cpp:817-850 + update_dm_particle_single,
+//     DarkMatterParticleContainer.cpp:249-306), with the periodic shift of
+//     Redistribute fused into the kick.
+//
+// Deposit determinism.  Each tile's particles are accumulated in LDS as exact
+// fixed-point integers (two unsigned 64-bit words per cell: units 2^(E-31) and
+// 2^(E-63) with 2^E <= max particle mass), so the sum does not depend on the order
+// in which particles arrive; the per-tile sums are then converted to FP64 and the
+// tile halos folded in a fixed order.  Runs are bitwise reproducible although the
+// binning itself uses atomics.
+#include "nyxpm_internal.h"
+
+namespace nyxpm {
+
+// ---------------------------------------------------------------------------
+// shared helpers
+// ---------------------------------------------------------------------------
+
+// Cell index i = floor((x - plo)*dxi + 0.5) of a coordinate, as both the deposit
+// and the interpolation compute it, plus the fractional weight.
+__device__ __forceinline__ int cic_index(double x, double plo, double dxi, double* frac) {
+    const double l = (x - plo) * dxi + 0.5;
+    const int i = (int)floor(l);
+    *frac = l - i;
+    return i;
+}
+
+// Wrap an index known to lie within one period of [0, n); flags anything else.
+__device__ __forceinline__ int wrap_checked(int i, int n, int* err) {
+    if (i < -n || i >= 2 * n) { if (err) atomicOr(err, 1); }
+    return wrap1(i, n);
+}
+
+// Wave-aggregated atomicAdd: lanes with equal keys share one atomic.  Returns, for
+// each valid lane, the counter's previous value plus the lane's rank among the
+// lanes with its key.  Must be called by every lane of the wave.
+__device__ __forceinline__ int agg_atomic_add(int* counters, int key, bool valid) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const unsigned long long below = (lane == 0) ? 0ull : ((~0ull) >> (kWave - lane));
+    int result = 0;
+    bool todo = valid;
+    for (;;) {
+        const unsigned long long act = __ballot(todo);
+        if (act == 0ull) break;
+        const int leader = __ffsll((unsigned long long)act) - 1;
+        const int lkey = __shfl(key, leader);
+        const bool mine = todo && key == lkey;
+        const unsigned long long mm = __ballot(mine);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&counters[lkey], (int)__popcll(mm));
+        base = __shfl(base, leader);
+        if (mine) {
+            result = base + (int)__popcll(mm & below);
+            todo = false;
+        }
+    }
+    return result;
+}
+
+__device__ __forceinline__ int tile_of(const BoxDev& B, double x, double y, double z, int* err) {
+    double f;
+    const int i = wrap_checked(cic_index(x, B.plo[0], B.dxi[0], &f), B.n[0], err);
+    const int j = wrap_checked(cic_index(y, B.plo[1], B.dxi[1], &f), B.n[1], err);
+    const int k = wrap_checked(cic_index(z, B.plo[2], B.dxi[2], &f), B.n[2], err);
+    const int tx = i / B.tile[0], ty = j / B.tile[1], tz = k / B.tile[2];
+    return (int)(B.tile_off + ((long long)tz * B.ntile[1] + ty) * B.ntile[0] + tx);
+}
+
+// ---------------------------------------------------------------------------
+// binning: keys + counts, exclusive scan, scatter
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(kBlock) k_keys(const BoxSeg* segs, int nseg, const BoxDev* boxes,
+                                                 const double* __restrict__ x, const double* __restrict__ y,
+                                                 const double* __restrict__ z, int* __restrict__ key,
+                                                 int* counts, int* err) {
+    const BoxSeg sg = segs[find_seg(segs, nseg, blockIdx.x)];
+    const BoxDev& B = boxes[sg.box];
+    const long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x;
+    const bool valid = q < B.np;
+    int t = 0;
+    if (valid) {
+        const long long p = B.part_off + q;
+        t = tile_of(B, x[p], y[p], z[p], err);
+        key[p] = t;
+    }
+    agg_atomic_add(counts, t, valid);
+}
+
+constexpr int kScanItems = 16;  // entries per thread in the scan passes
+
+__global__ void __launch_bounds__(kBlock) k_scan_reduce(const int* __restrict__ counts, long long n,
+                                                        long long* __restrict__ bsum) {
+    const long long base = (long long)blockIdx.x * kBlock * kScanItems;
+    long long s = 0;
+    for (int it = 0; it < kScanItems; ++it) {
+        const long long q = base + (long long)it * kBlock + threadIdx.x;
+        if (q < n) s += counts[q];
+    }
+    for (int o = kWave / 2; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    __shared__ long long ws[kBlock / kWave];
+    if ((threadIdx.x & (kWave - 1)) == 0) ws[threadIdx.x / kWave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long t = 0;
+        for (int w = 0; w < kBlock / kWave; ++w) t += ws[w];
+        bsum[blockIdx.x] = t;
+    }
+}
+
+// Exclusive scan of the block sums in place (one block), total in bsum[nb].
+__global__ void __launch_bounds__(1024) k_scan_top(long long* bsum, int nb) {
+    __shared__ long long sh[1024];
+    long long carry = 0;
+    for (int base = 0; base < nb; base += 1024) {
+        const int q = base + threadIdx.x;
+        const long long v = q < nb ? bsum[q] : 0;
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            long long t = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+            __syncthreads();
+            sh[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (q < nb) bsum[q] = carry + sh[threadIdx.x] - v;
+        const long long tot = sh[1023];
+        __syncthreads();
+        carry += tot;
+    }
+    if (threadIdx.x == 0) bsum[nb] = carry;
+}
+
+// start[q] = exclusive prefix of counts; cursor[q] = 0; start[n] = total.
+__global__ void __launch_bounds__(kBlock) k_scan_down(const int* __restrict__ counts, long long n,
+                                                      const long long* __restrict__ bsum,
+                                                      long long* __restrict__ start, int* __restrict__ cursor) {
+    __shared__ long long sh[kBlock];
+    const long long base = (long long)blockIdx.x * kBlock * kScanItems;
+    // each thread owns kScanItems consecutive entries
+    long long vals[kScanItems];
+    long long s = 0;
+    for (int it = 0; it < kScanItems; ++it) {
+        const long long q = base + (long long)threadIdx.x * kScanItems + it;
+        vals[it] = q < n ? counts[q] : 0;
+        s += vals[it];
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 1; o < kBlock; o <<= 1) {
+        long long t = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += t;
+        __syncthreads();
+    }
+    long long run = bsum[blockIdx.x] + sh[threadIdx.x] - s;
+    for (int it = 0; it < kScanItems; ++it) {
+        const long long q = base + (long long)threadIdx.x * kScanItems + it;
+        if (q < n) {
+            start[q] = run;
+            cursor[q] = 0;
+        }
+        run += vals[it];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) start[n] = bsum[gridDim.x];
+}
+
+struct PartPtrs {
+    double *x, *y, *z, *vx, *vy, *vz, *m;
+    int* id;
+};
+
+// Move every particle to its tile's slot (order inside a tile is arbitrary).
+__global__ void __launch_bounds__(kBlock) k_scatter(const BoxSeg* segs, int nseg, const BoxDev* boxes,
+                                                    const int* __restrict__ key,
+                                                    const long long* __restrict__ start, int* cursor,
+                                                    PartPtrs src, PartPtrs dst) {
+    const BoxSeg sg = segs[find_seg(segs, nseg, blockIdx.x)];
+    const BoxDev& B = boxes[sg.box];
+    const long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x;
+    const bool valid = q < B.np;
+    const long long p = B.part_off + q;
+    const int t = valid ? key[p] : 0;
+    const int r = agg_atomic_add(cursor, t, valid);
+    if (valid) {
+        const long long d = start[t] + r;
+        dst.x[d] = src.x[p];
+        dst.y[d] = src.y[p];
+        dst.z[d] = src.z[p];
+        dst.vx[d] = src.vx[p];
+        dst.vy[d] = src.vy[p];
+        dst.vz[d] = src.vz[p];
+        if (src.m) dst.m[d] = src.m[p];
+        dst.id[d] = src.id[p];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// deposit
+// ---------------------------------------------------------------------------
+
+// Index of a low-halo cell (lx==0 || ly==0 || lz==0) of a tile whose LDS block is
+// (bx+1) x (by+1) x (bz+1): the x face first, then the y face without its x=0
+// column, then the z face without x=0 and y=0.
+__device__ __forceinline__ int halo_index(int lx, int ly, int lz, int bx, int by, int bz) {
+    if (lx == 0) return ly * (bz + 1) + lz;
+    const int fx = (by + 1) * (bz + 1);
+    if (ly == 0) return fx + (lx - 1) * (bz + 1) + lz;
+    const int fy = bx * (bz + 1);
+    return fx + fy + (lx - 1) * by + (ly - 1);
+}
+
+// One block per tile.  Contributions m*sx[ii]*sy[jj]*sz[kk] (amrex_deposit_cic
+// weights, product left to right) of the tile's particles go to LDS as fixed
+// point; interior cells are written to `own`, the low halo layer to `halo`.
+__global__ void __launch_bounds__(kBlock) k_deposit(const BoxSeg* segs, int nseg, const BoxDev* boxes,
+                                                    const long long* __restrict__ start,
+                                                    const double* __restrict__ x, const double* __restrict__ y,
+                                                    const double* __restrict__ z, const double* __restrict__ m,
+                                                    double* __restrict__ own, double* __restrict__ halo, int* err) {
+    extern __shared__ unsigned long long acc[];
+    const BoxSeg sg = segs[find_seg(segs, nseg, blockIdx.x)];
+    const BoxDev& B = boxes[sg.box];
+    const int lt = blockIdx.x - sg.blk0;
+    const int tx = lt % B.ntile[0], ty = (lt / B.ntile[0]) % B.ntile[1], tz = lt / (B.ntile[0] * B.ntile[1]);
+    const int o0 = tx * B.tile[0], o1 = ty * B.tile[1], o2 = tz * B.tile[2];
+    const int bx = min(B.tile[0], B.n[0] - o0), by = min(B.tile[1], B.n[1] - o1), bz = min(B.tile[2], B.n[2] - o2);
+    const int Lx = bx + 1, Ly = by + 1, Lz = bz + 1;
+    const int LC = Lx * Ly * Lz;
+    for (int c = threadIdx.x; c < 2 * LC; c += kBlock) acc[c] = 0ull;
+    __syncthreads();
+
+    const long long gt = B.tile_off + lt;
+    const long long p0 = start[gt], p1 = start[gt + 1];
+    const int E = B.fx_exp;
+    for (long long p = p0 + threadIdx.x; p < p1; p += kBlock) {
+        double xint, yint, zint;
+        const int i = cic_index(x[p], B.plo[0], B.dxi[0], &xint);
+        const int j = cic_index(y[p], B.plo[1], B.dxi[1], &yint);
+        const int k = cic_index(z[p], B.plo[2], B.dxi[2], &zint);
+        const int iw = wrap_checked(i, B.n[0], err) - o0;
+        const int jw = wrap_checked(j, B.n[1], err) - o1;
+        const int kw = wrap_checked(k, B.n[2], err) - o2;
+        if (iw < 0 || iw >= bx || jw < 0 || jw >= by || kw < 0 || kw >= bz) {
+            atomicOr(err, 2);
+            continue;
+        }
+        const double sx[2] = {1.0 - xint, xint};
+        const double sy[2] = {1.0 - yint, yint};
+        const double sz[2] = {1.0 - zint, zint};
+        const double mp = B.has_mass ? m[p] : B.m_const;
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+            for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+                for (int ii = 0; ii < 2; ++ii) {
+                    const double w = sx[ii] * sy[jj] * sz[kk] * mp;
+                    const double s = ldexp(w, 31 - E);
+                    const double hf = floor(s);
+                    const unsigned long long hi = (unsigned long long)hf;
+                    const unsigned long long lo = (unsigned long long)rint(ldexp(s - hf, 32));
+                    const int idx = ((kw + kk) * Ly + (jw + jj)) * Lx + (iw + ii);
+                    atomicAdd(&acc[idx], hi);
+                    atomicAdd(&acc[LC + idx], lo);
+                }
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < LC; c += kBlock) {
+        const int lx = c % Lx, ly = (c / Lx) % Ly, lz = c / (Lx * Ly);
+        const double v = ldexp((double)acc[c], E - 31) + ldexp((double)acc[LC + c], E - 63);
+        if (lx > 0 && ly > 0 && lz > 0)
+            own[B.cell_off + cidx(o0 + lx - 1, o1 + ly - 1, o2 + lz - 1, B.n[0], B.n[1])] = v;
+        else
+            halo[gt * B.halo_stride + halo_index(lx, ly, lz, bx, by, bz)] = v;
+    }
+}
+
+// Sum of a block's values in a fixed order (wave butterfly, then waves in order).
+__device__ __forceinline__ double block_sum_fixed(double v) {
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    __shared__ double ws[kBlock / kWave];
+    __syncthreads();
+    if ((threadIdx.x & (kWave - 1)) == 0) ws[threadIdx.x / kWave] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int w = 0; w < kBlock / kWave; ++w) t += ws[w];
+    return t;
+}
+
+// rho = (own + halos of the up-to-7 upper neighbour tiles, fixed order) * 1/vol;
+// f = rho + (-mass_offset); per-block sum of f for the solvability correction.
+__global__ void __launch_bounds__(kBlock) k_fold(const CellSeg* segs, int nseg, const BoxDev* boxes,
+                                                 const double* __restrict__ own, const double* __restrict__ halo,
+                                                 double* __restrict__ rho, double* __restrict__ f,
+                                                 double* __restrict__ partial) {
+    const CellSeg& sg = segs[find_seg(segs, nseg, blockIdx.x)];
+    const BoxDev& B = boxes[sg.box];
+    const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
+    const long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x;
+    double fv = 0.0;
+    if (q < (long long)nx * ny * nz) {
+        const int i = (int)(q % nx);
+        const long long t = q / nx;
+        const int j = (int)(t % ny), k = (int)(t / ny);
+        const int T0 = B.tile[0], T1 = B.tile[1], T2 = B.tile[2];
+        const int tx = i / T0, ty = j / T1, tz = k / T2;
+        const int li = i - tx * T0, lj = j - ty * T1, lk = k - tz * T2;
+        const int bx = min(T0, nx - tx * T0), by = min(T1, ny - ty * T1), bz = min(T2, nz - tz * T2);
+        const bool topx = li == bx - 1, topy = lj == by - 1, topz = lk == bz - 1;
+        double v = own[B.cell_off + q];
+        // subsets of {x,y,z}: x, y, z, xy, xz, yz, xyz
+        const int sel[7] = {1, 2, 4, 3, 5, 6, 7};
+#pragma unroll
+        for (int s = 0; s < 7; ++s) {
+            const int S = sel[s];
+            const bool sx = S & 1, sy = S & 2, sz = S & 4;
+            if ((sx && !topx) || (sy && !topy) || (sz && !topz)) continue;
+            int ntx = tx + (sx ? 1 : 0), nty = ty + (sy ? 1 : 0), ntz = tz + (sz ? 1 : 0);
+            if (ntx == B.ntile[0]) ntx = 0;
+            if (nty == B.ntile[1]) nty = 0;
+            if (ntz == B.ntile[2]) ntz = 0;
+            const int nbx = min(T0, nx - ntx * T0), nby = min(T1, ny - nty * T1), nbz = min(T2, nz - ntz * T2);
+            const int hx = sx ? 0 : li + 1, hy = sy ? 0 : lj + 1, hz = sz ? 0 : lk + 1;
+            const long long nt = B.tile_off + ((long long)ntz * B.ntile[1] + nty) * B.ntile[0] + ntx;
+            v += halo[nt * B.halo_stride + halo_index(hx, hy, hz, nbx, nby, nbz)];
+        }
+        const double rv = v * B.inv_vol;
+        rho[B.cell_off + q] = rv;
+        fv = rv + (-B.mass_offset);
+        f[B.cell_off + q] = fv;
+    }
+    const double bs = block_sum_fixed(fv);
+    if (threadIdx.x == 0) partial[blockIdx.x] = bs;
+}
+
+// Per box: S = sum of its block partials (fixed order); solvability offset
+// sum = (S*dV)/ProbSize, as vol_weight_sum computes it (sum_utils.cpp:85-93).
+__global__ void __launch_bounds__(kBlock) k_rhs_sum(const CellSeg* segs, const double* __restrict__ partial,
+                                                    const double* __restrict__ dv_size, double* __restrict__ solv) {
+    const CellSeg& sg = segs[blockIdx.x];
+    double s = 0.0;
+    for (int b = threadIdx.x; b < sg.nblk; b += kBlock) s += partial[sg.blk0 + b];
+    const double S = block_sum_fixed(s);
+    if (threadIdx.x == 0) {
+        double sum = S * dv_size[2 * sg.box];
+        sum = sum / dv_size[2 * sg.box + 1];
+        solv[sg.box] = sum;
+    }
+}
+
+// f = ((f + (-sum)) * Ggravity) * (1/a), Ggravity = -4 pi G.
+__global__ void __launch_bounds__(kBlock) k_rhs_final(const CellSeg* segs, int nseg, const BoxDev* boxes,
+                                                      const double* __restrict__ solv, double* __restrict__ f,
+                                                      double Ggravity, double inv_a) {
+    const CellSeg& sg = segs[find_seg(segs, nseg, blockIdx.x)];
+    const BoxDev& B = boxes[sg.box];
+    const long long n = (long long)B.n[0] * B.n[1] * B.n[2];
+    const long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x;
+    if (q >= n) return;
+    const double ns = -solv[sg.box];
+    double v = f[B.cell_off + q] + ns;
+    f[B.cell_off + q] = (v * Ggravity) * inv_a;
+}
+
+// ---------------------------------------------------------------------------
+// gradient + CIC gather + kick (+ drift)
+// ---------------------------------------------------------------------------
+
+// Periodic position shift (Redistribute): above rhi -> subtract the length until
+// not (clamp to lo on undershoot); below lo -> add until not (clamp to rhi).
+// A coordinate more than two periods away (or not finite) is clamped and flagged
+// instead of looping.
+__device__ __forceinline__ double periodic_shift(double v, double lo, double rhi, double len, int* err) {
+    if (!(v >= lo - 2.0 * len && v <= rhi + 2.0 * len)) {
+        if (err) atomicOr(err, 8);
+        return (v < lo) ? lo : rhi;  // NaN compares false: goes to rhi
+    }
+    if (v > rhi) {
+        while (v > rhi) v = v - len;
+        if (v < lo) v = lo;
+    } else if (v < lo) {
+        while (v < lo) v = v + len;
+        if (v > rhi) v = rhi;
+    }
+    return v;
+}
+
+// val[d] = sum_{kk,jj,ii} sx[1-ii]*sy[1-jj]*sz[1-kk] * g_d(i-ii, j-jj, k-kk) with
+// g_d(c) = 0.5*(dxi*(phi(c) - phi(c-e_d)) + dxi*(phi(c+e_d) - phi(c))).
+__device__ __forceinline__ void gather_grad(const BoxDev& B, const double* __restrict__ P, double x, double y,
+                                            double z, double val[3], int* err) {
+    const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
+    double xint, yint, zint;
+    const int i = cic_index(x, B.plo[0], B.dxi[0], &xint);
+    const int j = cic_index(y, B.plo[1], B.dxi[1], &yint);
+    const int k = cic_index(z, B.plo[2], B.dxi[2], &zint);
+    if (i < -nx + 2 || i >= 2 * nx - 1 || j < -ny + 2 || j >= 2 * ny - 1 || k < -nz + 2 || k >= 2 * nz - 1)
+        atomicOr(err, 4);
+    int ix[4], iy[4], iz[4];
+#pragma unroll
+    for (int mm = 0; mm < 4; ++mm) {
+        ix[mm] = wrap1(i - 2 + mm, nx);
+        iy[mm] = wrap1(j - 2 + mm, ny);
+        iz[mm] = wrap1(k - 2 + mm, nz);
+    }
+    const double sx[2] = {1.0 - xint, xint};
+    const double sy[2] = {1.0 - yint, yint};
+    const double sz[2] = {1.0 - zint, zint};
+    auto ld = [&](int a, int b, int c) -> double { return P[cidx(ix[a], iy[b], iz[c], nx, ny)]; };
+    const double dxi0 = B.dxi[0], dxi1 = B.dxi[1], dxi2 = B.dxi[2];
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+#pragma unroll
+    for (int kk = 0; kk <= 1; ++kk)
+#pragma unroll
+        for (int jj = 0; jj <= 1; ++jj)
+#pragma unroll
+            for (int ii = 0; ii <= 1; ++ii) {
+                const int a = 2 - ii, b = 2 - jj, c = 2 - kk;
+                const double pc = ld(a, b, c);
+                const double gx = 0.5 * (dxi0 * (pc - ld(a - 1, b, c)) + dxi0 * (ld(a + 1, b, c) - pc));
+                const double gy = 0.5 * (dxi1 * (pc - ld(a, b - 1, c)) + dxi1 * (ld(a, b + 1, c) - pc));
+                const double gz = 0.5 * (dxi2 * (pc - ld(a, b, c - 1)) + dxi2 * (ld(a, b, c + 1) - pc));
+                const double w = sx[1 - ii] * sy[1 - jj] * sz[1 - kk];
+                v0 += w * gx;
+                v1 += w * gy;
+                v2 += w * gz;
+            }
+    val[0] = v0;
+    val[1] = v1;
+    val[2] = v2;
+}
+
+// moveKickDrift (DO_MOVE) or moveKick (!DO_MOVE) of update_dm_particle_single:
+//   v = (a_prev*v + (0.5*dt)*val) * (1/a_cur);  DO_MOVE: x += (dt*(1/a_cur))*v
+// DO_MOVE also bins the drifted particle (key + count for the next deposit);
+// !DO_MOVE applies the periodic shift of Redistribute.
+template <bool DO_MOVE>
+__global__ void __launch_bounds__(kBlock) k_kick(const BoxSeg* segs, int nseg, const BoxDev* boxes,
+                                                 const double* __restrict__ phi, double* __restrict__ x,
+                                                 double* __restrict__ y, double* __restrict__ z,
+                                                 double* __restrict__ vx, double* __restrict__ vy,
+                                                 double* __restrict__ vz, double dt, double a_prev, double a_cur,
+                                                 int* __restrict__ key, int* counts, int* err) {
+    const BoxSeg sg = segs[find_seg(segs, nseg, blockIdx.x)];
+    const BoxDev& B = boxes[sg.box];
+    const long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x;
+    const bool valid = q < B.np;
+    int t = 0;
+    if (valid) {
+        const long long p = B.part_off + q;
+        const double half_dt = 0.5 * dt;
+        const double a_cur_inv = 1.0 / a_cur;
+        const double dt_a_cur_inv = dt * a_cur_inv;
+        double px = x[p], py = y[p], pz = z[p];
+        double val[3];
+        gather_grad(B, phi + B.cell_off, px, py, pz, val, err);
+        double ux = a_prev * vx[p] + half_dt * val[0];
+        ux = ux * a_cur_inv;
+        double uy = a_prev * vy[p] + half_dt * val[1];
+        uy = uy * a_cur_inv;
+        double uz = a_prev * vz[p] + half_dt * val[2];
+        uz = uz * a_cur_inv;
+        vx[p] = ux;
+        vy[p] = uy;
+        vz[p] = uz;
+        if (DO_MOVE) {
+            px = px + dt_a_cur_inv * ux;
+            py = py + dt_a_cur_inv * uy;
+            pz = pz + dt_a_cur_inv * uz;
+            x[p] = px;
+            y[p] = py;
+            z[p] = pz;
+            t = tile_of(B, px, py, pz, err);
+            key[p] = t;
+        } else {
+            const double sxp = periodic_shift(px, B.plo[0], B.rhi[0], B.len[0], err);
+            const double syp = periodic_shift(py, B.plo[1], B.rhi[1], B.len[1], err);
+            const double szp = periodic_shift(pz, B.plo[2], B.rhi[2], B.len[2], err);
+            if (sxp != px) x[p] = sxp;
+            if (syp != py) y[p] = syp;
+            if (szp != pz) z[p] = szp;
+        }
+    }
+    if (DO_MOVE) agg_atomic_add(counts, t, valid);
+}
+
+// Periodic shift alone (nyxpm_redistribute).
+__global__ void __launch_bounds__(kBlock) k_wrap(const BoxSeg* segs, int nseg, const BoxDev* boxes,
+                                                 double* __restrict__ x, double* __restrict__ y,
+                                                 double* __restrict__ z, int* err) {
+    const BoxSeg sg = segs[find_seg(segs, nseg, blockIdx.x)];
+    const BoxDev& B = boxes[sg.box];
+    const long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x;
+    if (q >= B.np) return;
+    const long long p = B.part_off + q;
+    x[p] = periodic_shift(x[p], B.plo[0], B.rhi[0], B.len[0], err);
+    y[p] = periodic_shift(y[p], B.plo[1], B.rhi[1], B.len[1], err);
+    z[p] = periodic_shift(z[p], B.plo[2], B.rhi[2], B.len[2], err);
+}
+
+// Cell-centred gravity field of one box (get_new_grav_vector), 3 components.
+__global__ void __launch_bounds__(kBlock) k_grav_vector(const BoxDev* boxes, int box, const double* __restrict__ phi,
+                                                        double* __restrict__ g) {
+    const BoxDev& B = boxes[box];
+    const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
+    const long long nc = (long long)nx * ny * nz;
+    const long long q = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (q >= nc) return;
+    const int i = (int)(q % nx);
+    const long long t = q / nx;
+    const int j = (int)(t % ny), k = (int)(t / ny);
+    const double* P = phi + B.cell_off;
+    const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
+    const int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
+    const int km = k == 0 ? nz - 1 : k - 1, kp = k == nz - 1 ? 0 : k + 1;
+    const double pc = P[q];
+    const double fxl = B.dxi[0] * (pc - P[cidx(im, j, k, nx, ny)]);
+    const double fxh = B.dxi[0] * (P[cidx(ip, j, k, nx, ny)] - pc);
+    const double fyl = B.dxi[1] * (pc - P[cidx(i, jm, k, nx, ny)]);
+    const double fyh = B.dxi[1] * (P[cidx(i, jp, k, nx, ny)] - pc);
+    const double fzl = B.dxi[2] * (pc - P[cidx(i, j, km, nx, ny)]);
+    const double fzh = B.dxi[2] * (P[cidx(i, j, kp, nx, ny)] - pc);
+    g[q] = 0.5 * (fxl + fxh);
+    g[nc + q] = 0.5 * (fyl + fyh);
+    g[2 * nc + q] = 0.5 * (fzl + fzh);
+}
+
+// out[id] = in[p] for one box (undo the tile order for download).
+__global__ void __launch_bounds__(kBlock) k_unsort(long long p0, long long np, const int* __restrict__ id,
+                                                   const double* __restrict__ in, double* __restrict__ out) {
+    const long long q = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (q >= np) return;
+    out[id[p0 + q]] = in[p0 + q];
+}
+
+__global__ void __launch_bounds__(kBlock) k_iota(long long p0, long long np, int* __restrict__ id) {
+    const long long q = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (q < np) id[p0 + q] = (int)q;
+}
+
+__global__ void __launch_bounds__(kBlock) k_fill_int(int* a, long long n, int v) {
+    const long long q = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (q < n) a[q] = v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_fill_dbl(double* a, long long n, double v) {
+    const long long q = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (q < n) a[q] = v;
+}
+
+// f = rho + (-mass_offset) with per-block sums (used when the density was set
+// from outside rather than by the fold).
+__global__ void __launch_bounds__(kBlock) k_rhs1(const CellSeg* segs, int nseg, const BoxDev* boxes,
+                                                 const double* __restrict__ rho, double* __restrict__ f,
+                                                 double* __restrict__ partial) {
+    const CellSeg& sg = segs[find_seg(segs, nseg, blockIdx.x)];
+    const BoxDev& B = boxes[sg.box];
+    const long long n = (long long)B.n[0] * B.n[1] * B.n[2];
+    const long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x;
+    double fv = 0.0;
+    if (q < n) {
+        fv = rho[B.cell_off + q] + (-B.mass_offset);
+        f[B.cell_off + q] = fv;
+    }
+    const double bs = block_sum_fixed(fv);
+    if (threadIdx.x == 0) partial[blockIdx.x] = bs;
+}
+
+// Accelerations at the particles of one box (the val of update_dm_particle_single)
+// without touching the particles; written in upload (id) order, component-major.
+__global__ void __launch_bounds__(kBlock) k_gather(const BoxDev* boxes, int box, const double* __restrict__ phi,
+                                                   const double* __restrict__ x, const double* __restrict__ y,
+                                                   const double* __restrict__ z, const int* __restrict__ id,
+                                                   double* __restrict__ out, int* err) {
+    const BoxDev& B = boxes[box];
+    const long long q = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (q >= B.np) return;
+    const long long p = B.part_off + q;
+    double val[3];
+    gather_grad(B, phi + B.cell_off, x[p], y[p], z[p], val, err);
+    const long long o = id[p];
+    out[o] = val[0];
+    out[B.np + o] = val[1];
+    out[2 * B.np + o] = val[2];
+}
+
+}  // namespace nyxpm

@@ -1,0 +1,130 @@
+"""Seeded synthetic initial conditions for the PM workloads (numpy, FP64).
+
+Every generator draws from numpy's counter-based Philox bit generator keyed by
+the seed, so the CPU oracle and the GPU library are fed bit-identical inputs.
+Box: periodic unit cube [0,1)^3 unless stated; particle masses are equal and
+normalised to mean density 1 (m = 1/N_p).
+
+* ``lattice_gaussian``  -- cell-centred lattice + i.i.d. Gaussian displacements,
+  i.i.d. Gaussian velocities (64^3 CPU workload).
+* ``grf_lattice``       -- lattice + curl-free Gaussian-random-field displacement
+  psi_k = i k Phi_k with displacement power P(k) ~ k^index for 0 < |k| < k_Nyq
+  (modes at or beyond the Nyquist frequency, and k = 0, are zero), scaled so that
+  the rms of |psi| is ``rms_cells`` cells; velocity = vel_factor * psi.
+* ``plummer_clustered`` -- a uniform fraction plus Plummer spheres with uniform
+  centres and log-uniform scale radii, radii drawn from the inverse Plummer CDF
+  truncated at 16 scale radii, isotropic directions, wrapped periodically.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+SEED_BASE = 0x4E5958
+
+
+def rng(seed):
+    return np.random.Generator(np.random.Philox(key=int(seed)))
+
+
+def _wrap(a):
+    a = np.mod(a, 1.0)
+    a[a >= 1.0] = 0.0
+    return a
+
+
+def lattice_positions(n):
+    g = (np.arange(n) + 0.5) / n
+    z, y, x = np.meshgrid(g, g, g, indexing="ij")
+    return x.ravel(), y.ravel(), z.ravel()
+
+
+def lattice_gaussian(n, seed, sigma_cells=0.25, vel_sigma=0.01):
+    r = rng(seed)
+    x, y, z = lattice_positions(n)
+    d = r.standard_normal((3, n ** 3)) * (sigma_cells / n)
+    pos = [_wrap(x + d[0]), _wrap(y + d[1]), _wrap(z + d[2])]
+    vel = list(r.standard_normal((3, n ** 3)) * vel_sigma)
+    return pos, vel, 1.0 / n ** 3
+
+
+def grf_displacement(n, seed, index=-2.0, rms_cells=0.5):
+    """Curl-free displacement field (3, n, n, n) in box units."""
+    r = rng(seed)
+    w = r.standard_normal((n, n, n))
+    wk = np.fft.rfftn(w)
+    kz = np.fft.fftfreq(n) * n
+    ky = np.fft.fftfreq(n) * n
+    kx = np.fft.rfftfreq(n) * n
+    KZ, KY, KX = np.meshgrid(kz, ky, kx, indexing="ij")
+    k2 = KX ** 2 + KY ** 2 + KZ ** 2
+    kmag = np.sqrt(k2)
+    amp = np.zeros_like(kmag)
+    keep = (kmag > 0) & (kmag < n / 2) & (np.abs(KX) < n / 2) & (np.abs(KY) < n / 2) & (np.abs(KZ) < n / 2)
+    # |psi_k|^2 = k^2 |Phi_k|^2 ~ k^index  ->  |Phi_k| ~ k^((index-2)/2)
+    amp[keep] = kmag[keep] ** ((index - 2.0) / 2.0)
+    phik = wk * amp
+    psi = np.empty((3, n, n, n))
+    for d, K in enumerate((KX, KY, KZ)):
+        psi[d] = np.fft.irfftn(1j * K * phik, s=(n, n, n))
+    rms = np.sqrt(np.mean(np.sum(psi ** 2, axis=0)))
+    psi *= (rms_cells / n) / rms
+    return psi
+
+
+def grf_lattice(n, seed, index=-2.0, rms_cells=0.5, vel_factor=0.1):
+    psi = grf_displacement(n, seed, index, rms_cells)
+    x, y, z = lattice_positions(n)
+    pos = [_wrap(x + psi[0].ravel()), _wrap(y + psi[1].ravel()), _wrap(z + psi[2].ravel())]
+    vel = [vel_factor * psi[d].ravel() for d in range(3)]
+    return pos, vel, 1.0 / n ** 3
+
+
+def plummer_clustered(n, seed, n_spheres=2000, frac_uniform=0.2, rs_cells=(1.0, 8.0), trunc=16.0,
+                      vel_sigma=0.01, n_part=None, chunk=1 << 24):
+    """n^3 particles (or n_part): frac_uniform uniform, the rest in Plummer spheres."""
+    N = n ** 3 if n_part is None else int(n_part)
+    r = rng(seed)
+    n_uni = int(round(frac_uniform * N))
+    n_clu = N - n_uni
+    centres = r.random((n_spheres, 3))
+    rs = np.exp(r.uniform(np.log(rs_cells[0]), np.log(rs_cells[1]), n_spheres)) / n
+    # equal shares, the remainder to the first spheres
+    per = np.full(n_spheres, n_clu // n_spheres, dtype=np.int64)
+    per[: n_clu - per.sum()] += 1
+    owner = np.repeat(np.arange(n_spheres, dtype=np.int32), per)
+    pos = [np.empty(N), np.empty(N), np.empty(N)]
+    u = r.random((3, n_uni))
+    for d in range(3):
+        pos[d][:n_uni] = u[d]
+    umax = trunc ** 3 / (trunc ** 2 + 1.0) ** 1.5  # enclosed mass fraction at trunc*r_s
+    for s in range(0, n_clu, chunk):
+        e = min(n_clu, s + chunk)
+        m = e - s
+        o = owner[s:e]
+        uu = r.random(m) * umax
+        uu = np.maximum(uu, 1e-300)
+        rad = rs[o] / np.sqrt(uu ** (-2.0 / 3.0) - 1.0)
+        cth = r.uniform(-1.0, 1.0, m)
+        ph = r.uniform(0.0, 2.0 * np.pi, m)
+        sth = np.sqrt(1.0 - cth ** 2)
+        off = (rad * sth * np.cos(ph), rad * sth * np.sin(ph), rad * cth)
+        for d in range(3):
+            pos[d][n_uni + s:n_uni + e] = _wrap(centres[o, d] + off[d])
+    vel = list(r.standard_normal((3, N)) * vel_sigma)
+    return pos, vel, 1.0 / N
+
+
+def config(idx, n=None, seed=None):
+    """Inputs of workload idx (0: 64^3 lattice+Gaussian, 1: 256^3 GRF k^-2,
+    2: 512^3 Plummer-clustered, 4: 1024^3 GRF k^-1.5) at size n (default the
+    workload's own)."""
+    seed = SEED_BASE + idx if seed is None else seed
+    if idx == 0:
+        return lattice_gaussian(n or 64, seed)
+    if idx == 1:
+        return grf_lattice(n or 256, seed, -2.0, 0.5, 0.1)
+    if idx == 2:
+        return plummer_clustered(n or 512, seed)
+    if idx == 4:
+        return grf_lattice(n or 1024, seed, -1.5, 1.0, 0.1)
+    raise ValueError(idx)

@@ -1,0 +1,305 @@
+"""GPU parity: the HIP library (through the C ABI) against the CPU oracle and the
+exact golden vectors, on identical seeded inputs.
+
+Tolerances (from the path's contract): density 1e-13 relative (max-norm over the
+grid), accelerations and one-step positions 1e-12 relative, potential to the
+multigrid stopping rule; component kernels fed identical inputs agree bit for bit
+with the oracle where stated.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def ctx1(nyx, n, plo=(0, 0, 0), phi=(1, 1, 1), **kw):
+    return nyx.PMContext(nyx.BoxGeom(tuple(n), tuple(plo), tuple(phi)), **kw)
+
+
+def relmax(a, b):
+    return np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300)
+
+
+# ---------------------------------------------------------------------------
+# deposit
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("name", ["twoline", "mass8"])
+def test_deposit_reference_files_exact(nyx, golden, name):
+    with ctx1(nyx, golden["box32_n"], golden["box32_plo"], golden["box32_phi"]) as c:
+        pos, vel, m = golden[f"{name}_pos"], golden[f"{name}_vel"], golden[f"{name}_mass"]
+        c.upload(0, pos, vel, m=m)
+        rho = c.assign_density()
+        assert relmax(rho, golden[f"{name}_rho"]) <= 1e-14
+        assert c.sum_particle_mass() == pytest.approx(m.sum(), rel=1e-15)
+
+
+def test_deposit_rect_exact_and_oracle(nyx, orc, golden):
+    n, plo, phi = golden["rect_n"], golden["rect_plo"], golden["rect_phi"]
+    with ctx1(nyx, n, plo, phi, tile=(4, 4, 4)) as c:
+        c.upload(0, golden["rect_pos"], golden["rect_vel"], m=golden["rect_mass"])
+        rho = c.assign_density()
+    assert relmax(rho, golden["rect_rho"]) <= 1e-14
+    ref = orc.deposit(n, plo, phi, *golden["rect_pos"], m=golden["rect_mass"])
+    assert relmax(rho, ref) <= 1e-14
+
+
+def test_deposit_64_vs_oracle_and_deterministic(nyx, orc):
+    pos, vel, m = nyx.ics.lattice_gaussian(64, nyx.ics.SEED_BASE)
+    with ctx1(nyx, (64, 64, 64)) as c:
+        c.upload(0, pos, vel, m_const=m)
+        a = c.assign_density()
+        b = c.assign_density()
+    ref = orc.deposit((64, 64, 64), (0, 0, 0), (1, 1, 1), *pos, m_const=m)
+    assert relmax(a, ref) <= 1e-13
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("tile", [(8, 8, 8), (16, 8, 4), (4, 4, 4)])
+def test_deposit_clustered_tiles(nyx, orc, tile):
+    pos, vel, m = nyx.ics.plummer_clustered(32, 77, n_spheres=20)
+    with ctx1(nyx, (32, 32, 32), tile=tile) as c:
+        c.upload(0, pos, vel, m_const=m)
+        a = c.assign_density()
+        c.redistribute()  # forces a fresh binning (different in-tile order)
+        b = c.assign_density()
+    ref = orc.deposit((32, 32, 32), (0, 0, 0), (1, 1, 1), *pos, m_const=m)
+    assert relmax(a, ref) <= 1e-13
+    assert np.array_equal(a, b)  # order-independent fixed-point tile sums
+
+
+@pytest.mark.parametrize("n", [(20, 12, 6), (10, 10, 10), (2, 2, 2)])
+def test_deposit_partial_tiles_and_edges(nyx, orc, n):
+    r = np.random.default_rng(5)
+    NP = 2000
+    pos = [r.random(NP) for _ in range(3)]
+    # cell corners, the box corner, the upper edge image, a repeated position
+    pos[0][:4] = [0.0, 0.5, np.nextafter(1.0, 0), 0.3]
+    pos[1][:4] = [0.0, 0.5, 0.0, 0.3]
+    pos[2][:4] = [0.0, 0.5, 0.999, 0.3]
+    pos[0][4:40] = 0.123
+    pos[1][4:40] = 0.456
+    pos[2][4:40] = 0.789
+    vel = [np.zeros(NP)] * 3
+    m = r.uniform(0.5, 1.5, NP)
+    with ctx1(nyx, n) as c:
+        c.upload(0, pos, vel, m=m)
+        rho = c.assign_density()
+    ref = orc.deposit(n, (0, 0, 0), (1, 1, 1), *pos, m=m)
+    assert relmax(rho, ref) <= 1e-13
+
+
+def test_deposit_empty(nyx):
+    with ctx1(nyx, (8, 8, 8)) as c:
+        e = np.array([])
+        c.upload(0, [e, e, e], [e, e, e], m_const=1.0)
+        rho = c.assign_density()
+    assert not rho.any()
+
+
+# ---------------------------------------------------------------------------
+# multigrid, gravity field
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("n,L", [((32, 32, 32), (1, 1, 1)), ((48, 16, 24), (3, 1, 1.5))])
+def test_mg_iterates_bitwise(nyx, orc, n, L):
+    dx = [L[d] / n[d] for d in range(3)]
+    r = np.random.default_rng(3)
+    f = r.standard_normal((n[2], n[1], n[0]))
+    f -= f.mean()
+    with ctx1(nyx, n, (0, 0, 0), L) as c:
+        phi, it, norms = c.solve_poisson(f, fixed_iters=3)
+    ref, it2, st, _ = orc.mg_solve(n, dx, f, fixed_iters=3)
+    assert it == it2 == 3
+    assert np.array_equal(phi, ref)
+
+
+def test_mg_converged_meets_rule(nyx, orc):
+    n = (64, 64, 64)
+    dx = [1 / 64] * 3
+    r = np.random.default_rng(4)
+    f = r.standard_normal((64, 64, 64))
+    f -= f.mean()
+    with ctx1(nyx, n) as c:
+        phi, it, norms = c.solve_poisson(f, rel_tol=1e-10)
+    ref, it2, st, rc = orc.mg_solve(n, dx, f, rel_tol=1e-10)
+    assert it == it2
+    res = f - orc.apply_laplacian(n, dx, phi)
+    assert np.abs(res).max() <= 1e-10 * max(np.abs(f).max(), norms[0]) * (1 + 1e-9)
+    ex = orc.fft_solve(n, dx, f)
+    assert relmax(phi - phi.mean(), ex) < 1e-8
+
+
+def test_grav_vector_bitwise(nyx, orc, golden):
+    n, plo, phi_hi = golden["rect_n"], golden["rect_plo"], golden["rect_phi"]
+    with ctx1(nyx, n, plo, phi_hi) as c:
+        c.set_phi(golden["rect_pot"])
+        g = c.grav_vector()
+    ref = orc.grav_vector(n, plo, phi_hi, golden["rect_pot"])
+    assert np.array_equal(g, ref)
+    assert relmax(g, golden["rect_g"]) <= 1e-14
+
+
+# ---------------------------------------------------------------------------
+# particle update
+# ---------------------------------------------------------------------------
+
+def test_kick_drift_golden_and_bitwise(nyx, orc, golden):
+    n, plo, phi_hi = golden["rect_n"], golden["rect_plo"], golden["rect_phi"]
+    dt, a_prev, a_cur = golden["rect_kd_params"]
+    with ctx1(nyx, n, plo, phi_hi) as c:
+        c.upload(0, golden["rect_pos"], golden["rect_vel"], m=golden["rect_mass"])
+        c.set_phi(golden["rect_pot"])
+        c.move_kick_drift(dt, a_prev, a_cur)
+        pos, vel = c.download()
+    for d in range(3):
+        assert relmax(vel[d], golden["rect_kd_v"][d]) <= 1e-13
+        assert relmax(pos[d], golden["rect_kd_x"][d]) <= 1e-15
+    p2 = [a.copy() for a in golden["rect_pos"]]
+    v2 = [a.copy() for a in golden["rect_vel"]]
+    g = orc.grav_vector(n, plo, phi_hi, golden["rect_pot"])
+    orc.update_particles(n, plo, phi_hi, p2, v2, g, dt, a_prev, a_cur, 1)
+    for d in range(3):
+        assert np.array_equal(vel[d], v2[d])
+        assert np.array_equal(pos[d], p2[d])
+
+
+def test_kick_wraps_like_redistribute(nyx, orc, golden):
+    n, plo, phi_hi = golden["rect_n"], golden["rect_plo"], golden["rect_phi"]
+    with ctx1(nyx, n, plo, phi_hi) as c:
+        c.upload(0, golden["rect_pos"], golden["rect_vel"], m=golden["rect_mass"])
+        c.set_phi(golden["rect_pot"])
+        c.move_kick(0.01, 1.0, 1.0)
+        pos, vel = c.download()
+    p2 = [a.copy() for a in golden["rect_pos"]]
+    v2 = [a.copy() for a in golden["rect_vel"]]
+    g = orc.grav_vector(n, plo, phi_hi, golden["rect_pot"])
+    orc.update_particles(n, plo, phi_hi, p2, v2, g, 0.01, 1.0, 1.0, 0)
+    orc.enforce_periodic(n, plo, phi_hi, p2)
+    for d in range(3):
+        assert np.array_equal(pos[d], p2[d])
+        assert np.array_equal(vel[d], v2[d])
+
+
+# ---------------------------------------------------------------------------
+# whole step
+# ---------------------------------------------------------------------------
+
+def _oracle_steps(orc, n, pos, vel, m, nsteps, dt, rel_tol):
+    pos = [a.copy() for a in pos]
+    vel = [a.copy() for a in vel]
+    phi, _, _ = orc.solve_for_phi(n, (0, 0, 0), (1, 1, 1), pos, m_const=m, rel_tol=rel_tol)
+    g = None
+    out = []
+    for s in range(nsteps):
+        g, st = orc.pm_step(n, (0, 0, 0), (1, 1, 1), pos, vel, phi, m_const=m, dt=dt, rel_tol=rel_tol,
+                            want_g=True)
+        out.append(([a.copy() for a in pos], [a.copy() for a in vel], g.copy(), phi.copy(), int(st[0])))
+    return out
+
+
+def test_full_step_64_vs_oracle(nyx, orc):
+    # 64^3 cells, 64^3 particles, lattice + Gaussian (sigma 0.25 cell), one KDK step
+    pos, vel, m = nyx.ics.lattice_gaussian(64, nyx.ics.SEED_BASE)
+    n = (64, 64, 64)
+    ref = _oracle_steps(orc, n, pos, vel, m, 1, 1e-3, 1e-10)
+    with ctx1(nyx, n) as c:
+        c.upload(0, pos, vel, m_const=m)
+        c.pm_init(1.0, 1.0, 1e-10)
+        it = c.pm_steps(1, 1e-3, rel_tol=1e-10)
+        p, v = c.download()
+        g = c.grav_vector()
+        phi = c.get_phi()
+    rp, rv, rg, rphi, rit = ref[0]
+    assert it[0] == rit
+    assert relmax(g, rg) <= 1e-12
+    for d in range(3):
+        assert relmax(p[d], rp[d]) <= 1e-12
+        assert relmax(v[d], rv[d]) <= 1e-12
+
+
+def test_grf_recipe_ten_steps_vs_oracle(nyx, orc):
+    # the 256^3 drop-in recipe (GRF P(k)~k^-2, rms 0.5 cell, v = 0.1 psi) at 64^3
+    pos, vel, m = nyx.ics.grf_lattice(64, nyx.ics.SEED_BASE + 1, -2.0, 0.5, 0.1)
+    n = (64, 64, 64)
+    ref = _oracle_steps(orc, n, pos, vel, m, 10, 1e-3, 1e-10)
+    with ctx1(nyx, n) as c:
+        c.upload(0, pos, vel, m_const=m)
+        c.pm_init(1.0, 1.0, 1e-10)
+        for s in (1, 10):
+            c.pm_steps(1 if s == 1 else 9, 1e-3, rel_tol=1e-10)
+            p, v = c.download()
+            g = c.grav_vector()
+            rp, rv, rg, _, _ = ref[s - 1]
+            assert relmax(g, rg) <= 1e-12
+            for d in range(3):
+                assert relmax(p[d], rp[d]) <= 1e-12
+                assert relmax(v[d], rv[d]) <= 1e-12
+
+
+def test_nyx_faithful_mode(nyx, orc):
+    pos, vel, m = nyx.ics.lattice_gaussian(32, 5, sigma_cells=1.0)
+    n = (32, 32, 32)
+    with ctx1(nyx, n) as c:
+        c.upload(0, pos, vel, m_const=m)
+        c.pm_init(1.0, 1.0, 1e-11)
+        c.pm_steps(2, 1e-3, rel_tol=1e-11)
+        pa, va = c.download()
+    with ctx1(nyx, n) as c:
+        c.upload(0, pos, vel, m_const=m)
+        c.pm_steps(2, 1e-3, rel_tol=1e-11, nyx_faithful=True)
+        pb, vb = c.download()
+    for d in range(3):
+        assert relmax(vb[d], va[d]) < 1e-8
+
+
+def test_mirror_interface_step(nyx, orc):
+    pos, vel, m = nyx.ics.lattice_gaussian(32, 6)
+    n = (32, 32, 32)
+    with ctx1(nyx, n) as c:
+        pc = nyx.DarkMatterParticleContainer(c)
+        grav = nyx.Gravity(c, four_pi_G=1.0, sl_tol=1e-10, ml_tol=1e-10)
+        pc.InitFromArrays(pos, vel, mass_const=m)
+        grav.solve_for_old_phi(pc)
+        nyx.advance_particles_only(pc, grav, 1e-3)
+        p, v = pc.GetParticles()
+        assert pc.sumParticleMass() == pytest.approx(1.0, rel=1e-12)
+    ref = _oracle_steps(orc, n, pos, vel, m, 1, 1e-3, 1e-10)
+    for d in range(3):
+        assert relmax(p[d], ref[0][0][d]) <= 1e-12
+        assert relmax(v[d], ref[0][1][d]) <= 1e-12
+
+
+# ---------------------------------------------------------------------------
+# error paths (forced from the host; every kernel exits normally)
+# ---------------------------------------------------------------------------
+
+def test_not_converged_is_reported(nyx):
+    r = np.random.default_rng(9)
+    f = r.standard_normal((16, 16, 16))
+    f -= f.mean()
+    with ctx1(nyx, (16, 16, 16), max_iter=1) as c:
+        with pytest.raises(nyx.NyxPMError) as ei:
+            c.solve_poisson(f, rel_tol=1e-14)
+    assert ei.value.code == 4
+
+
+def test_unsupported_geometry_and_bad_args(nyx):
+    with pytest.raises(nyx.NyxPMError) as ei:
+        ctx1(nyx, (66, 66, 66))
+    assert ei.value.code == 5
+    with pytest.raises(nyx.NyxPMError):
+        ctx1(nyx, (1, 8, 8))
+    with ctx1(nyx, (8, 8, 8)) as c:
+        with pytest.raises(nyx.NyxPMError) as ei:
+            c.pm_steps(1, 1e-3)
+        assert ei.value.code == 6  # no particles
+
+
+def test_far_particle_is_an_error(nyx):
+    with ctx1(nyx, (8, 8, 8)) as c:
+        x = np.array([0.5, 5.0])
+        c.upload(0, [x, x.copy(), x.copy()], [np.zeros(2)] * 3, m_const=1.0)
+        with pytest.raises(nyx.NyxPMError) as ei:
+            c.assign_density()
+        assert ei.value.code == 6
