@@ -42,6 +42,8 @@ def kernel_bytes(name, ncell, npart, halo_cells, mixed_sz):
     N, P = ncell, npart
     table = {
         "smooth.d0": (sz + sz / 2 + sz / 2) * N,          # read e (all), r (colour), write e (colour)
+        "sweep.d0": 3 * sz * N,                            # fused red+black: read e, r; write e
+        "sweep0.d0": 2 * sz * N,                           # first sweep from e = 0: read r; write e
         "resid_restrict.d0": (2 * sz + sz / 8) * N,        # read e, r; write coarse r
         "prolong.d0": (2 * sz + sz / 8) * N,               # read/write e, read coarse e
         "fill.d0": sz * N,
@@ -152,9 +154,10 @@ def main():
     del g64, gmx
     log(f"accel rel-L2 mixed vs fp64 = {rel_l2:.3e}  (mixed cold-start cycles {int(it_m[0])})")
 
-    names = ["smooth.d0", "resid_restrict.d0", "prolong.d0", "fill.d0", "residual", "update", "deposit", "fold",
-             "rhs", "kick_drift", "kick", "scatter", "scan", "keys", "bottom"] + \
-            [f"{k}.d{d}" for k in ("smooth", "resid_restrict", "prolong", "fill") for d in range(1, 12)]
+    names = ["residual", "norm", "update", "deposit", "fold", "rhs", "kick_drift", "kick", "scatter", "scan",
+             "items", "keys", "bottom"] + \
+            [f"{k}.d{d}" for k in ("sweep0", "sweep", "smooth", "resid_restrict", "prolong", "fill")
+             for d in range(0, 12)]
     tile = 8
     halo_cells = (ncell // tile ** 3) * ((tile + 1) ** 3 - tile ** 3)
 
@@ -177,6 +180,16 @@ def main():
     roof = {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 1) if achieved else None,
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
             "traffic": None, "bytes_per_launch": int(b) if b else None, "mean_launch_ms": d["mean_ms"]}
+
+    def smoother_roof(st, sz):
+        e = st.get("sweep.d0")
+        if not e:
+            return None
+        b = kernel_bytes("sweep.d0", ncell, npart, halo_cells, sz)
+        ach = b / (e["mean_ms"] * 1e-3) / 1e9
+        return {"kernel": "sweep.d0 (fused red+black sweep, finest level)", "achieved": round(ach, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                "bytes_per_launch": int(b), "mean_launch_ms": e["mean_ms"]}
 
     cpu = None
     if not a.no_cpu:
@@ -204,6 +217,7 @@ def main():
         "vcycles_mixed": it_mix,
         "accel_rel_l2": rel_l2,
         "roofline": roof,
+        "smoother_roofline": {"mixed": smoother_roof(st_mix, 4), "fp64": smoother_roof(st_64, 8)},
         "stages_mixed": st_mix,
         "stages_fp64": st_64,
         "cpu_baseline": cpu,

@@ -78,6 +78,10 @@ typedef struct nyxpm_opts {
     int max_iter;      /* V-cycle limit (MLMG default 200); 0 = default */
     int tile[3];       /* deposit/gather tile shape in cells; 0 = default (8,8,8) */
     int verbose;       /* 0 = silent */
+    int smoother;      /* 0: fused red-black sweeps where the level allows (default);
+                          1: separate red and black half sweeps everywhere */
+    int kick;          /* 0: tile-staged gather + kick (default); 1: one thread per
+                          particle reading phi from global memory */
 } nyxpm_opts;
 
 /* ---- lifecycle ---------------------------------------------------------- */

@@ -74,6 +74,149 @@ __global__ void __launch_bounds__(kBlock) k_gsrb_half(const CellSeg* segs, int n
     E[c] = ec + res * inv_gamma;
 }
 
+// One full red-black Gauss-Seidel sweep (red half, then black half) in a single
+// pass over the level: the block owns a TX x TY column of cells and a chunk of
+// z-planes, and marches up in z keeping a ring of 4 planes of e (halo 2 in x, y)
+// and 2 planes of r (halo 1) in LDS.  Per plane kr: red cells of plane kr are
+// updated in LDS (including a one-cell ring around the column, recomputed
+// redundantly), then the black cells of plane kr-1 are updated from the new red
+// values and the plane's final values are written to eout.  Reads ein, writes
+// eout (a different buffer), so neighbouring blocks never see half-updated
+// values; the arithmetic per cell is exactly that of k_gsrb_half, so the result
+// equals two half sweeps bit for bit.  ZERO_IN: ein is identically zero (first
+// sweep after the correction is reset), so it is not read.
+// Needs nx % TX == 0, ny % TY == 0 and even nx, ny, nz.  HBM bytes per cell:
+// e read + r read + e write (3*sizeof(T)), halo re-reads served by L2.
+template <typename T, int TX, int TY, bool ZERO_IN>
+__global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, int nseg, const int* active,
+                                                          const T* __restrict__ ein, T* __restrict__ eout,
+                                                          const T* __restrict__ r) {
+    constexpr int NT = TX * TY / 2;
+    constexpr int EX = TX + 4, EY = TY + 4;   // e ring planes: halo 2
+    constexpr int RX = TX + 2, RY = TY + 2;   // r planes and red-update region: halo 1
+    constexpr int NE = (EX * EY + NT - 1) / NT, NR = (RX * RY + NT - 1) / NT;
+    __shared__ T se[4][EY][EX];
+    __shared__ T sr[2][RY][RX];
+    const int s = find_seg(segs, nseg, blockIdx.x);
+    const CellSeg& sg = segs[s];
+    if (!active[sg.box]) return;
+    const int nx = sg.n[0], ny = sg.n[1], nz = sg.n[2];
+    const int ntx = nx / TX, nty = ny / TY;
+    const int zc = sg.zchunk;
+    const int b = blockIdx.x - sg.blk0;
+    const int tx = b % ntx, ty = (b / ntx) % nty, tz = b / (ntx * nty);
+    const int x0 = tx * TX, y0 = ty * TY, z0 = tz * zc;
+    const int z1 = min(nz, z0 + zc);
+    const T dhx = T(sg.dh[0]), dhy = T(sg.dh[1]), dhz = T(sg.dh[2]);
+    const T gamma = T(sg.gamma), inv_gamma = T(sg.inv_gamma);
+    const T* Ein = ein + sg.off;
+    T* Eout = eout + sg.off;
+    const T* R = r + sg.off;
+    const int tid = threadIdx.x;
+
+    // Planes are fetched into registers one iteration ahead of their use, so the
+    // global-memory latency overlaps the previous plane's compute.
+    T pe[NE], pr[NR];
+    auto wrapz = [&](int k) { return k < 0 ? k + nz : (k >= nz ? k - nz : k); };
+    auto fetch_e = [&](int kz) {
+        const int kw = wrapz(kz);
+#pragma unroll
+        for (int u = 0; u < NE; ++u) {
+            const int idx = tid + u * NT;
+            if (idx < EX * EY) {
+                const int ly = idx / EX, lx = idx - ly * EX;
+                int gx = x0 + lx - 2, gy = y0 + ly - 2;
+                gx = gx < 0 ? gx + nx : (gx >= nx ? gx - nx : gx);
+                gy = gy < 0 ? gy + ny : (gy >= ny ? gy - ny : gy);
+                pe[u] = ZERO_IN ? T(0) : Ein[cidx(gx, gy, kw, nx, ny)];
+            }
+        }
+    };
+    auto put_e = [&](int kz) {
+        T* dst = &se[kz & 3][0][0];
+#pragma unroll
+        for (int u = 0; u < NE; ++u) {
+            const int idx = tid + u * NT;
+            if (idx < EX * EY) dst[idx] = pe[u];
+        }
+    };
+    auto fetch_r = [&](int kz) {
+        const int kw = wrapz(kz);
+#pragma unroll
+        for (int u = 0; u < NR; ++u) {
+            const int idx = tid + u * NT;
+            if (idx < RX * RY) {
+                const int ly = idx / RX, lx = idx - ly * RX;
+                int gx = x0 + lx - 1, gy = y0 + ly - 1;
+                gx = gx < 0 ? gx + nx : (gx >= nx ? gx - nx : gx);
+                gy = gy < 0 ? gy + ny : (gy >= ny ? gy - ny : gy);
+                pr[u] = R[cidx(gx, gy, kw, nx, ny)];
+            }
+        }
+    };
+    auto put_r = [&](int kz) {
+        T* dst = &sr[kz & 1][0][0];
+#pragma unroll
+        for (int u = 0; u < NR; ++u) {
+            const int idx = tid + u * NT;
+            if (idx < RX * RY) dst[idx] = pr[u];
+        }
+    };
+
+    fetch_e(z0 - 2);
+    put_e(z0 - 2);
+    fetch_e(z0 - 1);
+    put_e(z0 - 1);
+    fetch_e(z0);
+    fetch_r(z0 - 1);
+    for (int kr = z0 - 1; kr <= z1; ++kr) {
+        put_e(kr + 1);
+        put_r(kr);
+        if (kr < z1) {
+            fetch_e(kr + 2);
+            fetch_r(kr + 1);
+        }
+        __syncthreads();
+        {   // red cells of plane kr on the halo-1 region
+            const int sm = (kr - 1) & 3, sc = kr & 3, sp = (kr + 1) & 3, rs = kr & 1;
+            for (int idx = tid; idx < RX * RY; idx += NT) {
+                const int ly = idx / RX, lx = idx - ly * RX;
+                const int gx = x0 + lx - 1, gy = y0 + ly - 1;
+                if (((gx + gy + kr) & 1) != 0) continue;  // red: (i+j+k) even (periodic-consistent, n even)
+                const int ex = lx + 1, ey = ly + 1;
+                const T ec = se[sc][ey][ex];
+                const T res = sr[rs][ly][lx] - gamma * ec
+                            - dhx * (se[sc][ey][ex - 1] + se[sc][ey][ex + 1])
+                            - dhy * (se[sc][ey - 1][ex] + se[sc][ey + 1][ex])
+                            - dhz * (se[sm][ey][ex] + se[sp][ey][ex]);
+                se[sc][ey][ex] = ec + res * inv_gamma;
+            }
+        }
+        __syncthreads();
+        const int kb = kr - 1;
+        if (kb >= z0) {   // black cells of plane kb on the interior, write plane kb
+            const int sm = (kb - 1) & 3, sc = kb & 3, sp = (kb + 1) & 3, rs = kb & 1;
+            const int ly = tid / (TX / 2), m = tid - ly * (TX / 2);
+            const int lx = 2 * m;
+            const int gy = y0 + ly;
+            const int par = (x0 + lx + gy + kb) & 1;  // 0: cell lx is red, lx+1 black
+            const int bxl = lx + (1 - par);            // local x of the black cell
+            const int ex = bxl + 2, ey = ly + 2;
+            const T ec = se[sc][ey][ex];
+            const T res = sr[rs][ly + 1][bxl + 1] - gamma * ec
+                        - dhx * (se[sc][ey][ex - 1] + se[sc][ey][ex + 1])
+                        - dhy * (se[sc][ey - 1][ex] + se[sc][ey + 1][ex])
+                        - dhz * (se[sm][ey][ex] + se[sp][ey][ex]);
+            const T eb = ec + res * inv_gamma;
+            const T er = se[sc][ey][lx + par + 2];     // the red cell, updated this sweep
+            T* o = Eout + cidx(x0 + lx, gy, wrapz(kb), nx, ny);
+            if (par == 0) { o[0] = er; o[1] = eb; }
+            else          { o[0] = eb; o[1] = er; }
+        }
+        __syncthreads();
+    }
+}
+
 // Residual of the correction, restricted: r_c(I) = 0.125 * sum_children (r - L e),
 // children summed k, then j, then i innermost.  One thread per coarse cell.
 template <typename T>
@@ -103,9 +246,11 @@ __global__ void __launch_bounds__(kBlock) k_resid_restrict(const CellSeg* segs, 
     r[sg.coff + q] = T(0.125) * acc;
 }
 
-// e_fine += e_coarse(parent): piecewise-constant prolongation.
+// e_fine += e_coarse(parent): piecewise-constant prolongation (ec: the coarse
+// level's buffer, addressed at sg.coff).
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_prolong(const CellSeg* segs, int nseg, const int* active, T* e) {
+__global__ void __launch_bounds__(kBlock) k_prolong(const CellSeg* segs, int nseg, const int* active, T* e,
+                                                    const T* ec0, const T* ec1) {
     const int s = find_seg(segs, nseg, blockIdx.x);
     const CellSeg& sg = segs[s];
     if (!active[sg.box]) return;
@@ -115,7 +260,8 @@ __global__ void __launch_bounds__(kBlock) k_prolong(const CellSeg* segs, int nse
     const int i = (int)(q % nx);
     const long long t = q / nx;
     const int j = (int)(t % ny), k = (int)(t / ny);
-    const T ec = e[sg.coff + cidx(i >> 1, j >> 1, k >> 1, nx >> 1, ny >> 1)];
+    const T* ecrs = sg.ebuf ? ec1 : ec0;
+    const T ec = ecrs[sg.coff + cidx(i >> 1, j >> 1, k >> 1, nx >> 1, ny >> 1)];
     e[sg.off + q] = e[sg.off + q] + ec;
 }
 
@@ -153,9 +299,11 @@ __global__ void __launch_bounds__(kBlock) k_bottom(const CellSeg* segs, int nseg
     }
 }
 
-// Level-0 residual r = f - L phi (FP64), stored as T, with max-norms of r (and of
-// f when nf != nullptr) reduced per box.  sg.off addresses the MG pool, the
-// fields phi/f live in the level-0 pools at foff (passed in coff).
+// Level-0 residual r = f - L phi (FP64), stored as T.  A box is covered by
+// sg.nblk blocks striding over its cells; each block reduces max |r| (and max |f|
+// when nf != nullptr) and folds it into the box's norm with one atomicMax on the
+// bit pattern (exact and order independent).  sg.off addresses the MG pool; phi
+// and f live in the level-0 pools at sg.coff.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_residual0(const CellSeg* segs, int nseg, const int* active,
                                                       const double* __restrict__ f,
@@ -165,36 +313,49 @@ __global__ void __launch_bounds__(kBlock) k_residual0(const CellSeg* segs, int n
     const CellSeg& sg = segs[s];
     if (!active[sg.box]) return;
     const int nx = sg.n[0], ny = sg.n[1], nz = sg.n[2];
-    const long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x;
+    const long long nc = (long long)nx * ny * nz;
+    const double* P = phi + sg.coff;
     double ar = 0.0, af = 0.0;
-    if (q < (long long)nx * ny * nz) {
+    for (long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x; q < nc;
+         q += (long long)sg.nblk * kBlock) {
         const int i = (int)(q % nx);
         const long long t = q / nx;
         const int j = (int)(t % ny), k = (int)(t / ny);
-        const double* P = phi + sg.coff;
         const double fv = f[sg.coff + q];
         const double v = fv - lap7(P, i, j, k, nx, ny, nz, sg.dh[0], sg.dh[1], sg.dh[2]);
         r[sg.off + q] = T(v);
-        ar = fabs(v);
-        af = fabs(fv);
+        ar = fmax(ar, fabs(v));
+        af = fmax(af, fabs(fv));
     }
     for (int o = kWave / 2; o > 0; o >>= 1) {
         ar = fmax(ar, __shfl_xor(ar, o));
         af = fmax(af, __shfl_xor(af, o));
     }
+    __shared__ double wr[kBlock / kWave], wf[kBlock / kWave];
     if ((threadIdx.x & (kWave - 1)) == 0) {
-        atomicMax(&nr[sg.box], dbits(ar));
-        if (nf) atomicMax(&nf[sg.box], dbits(af));
+        wr[threadIdx.x / kWave] = ar;
+        wf[threadIdx.x / kWave] = af;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mr = wr[0], mf = wf[0];
+        for (int w = 1; w < kBlock / kWave; ++w) {
+            mr = fmax(mr, wr[w]);
+            mf = fmax(mf, wf[w]);
+        }
+        atomicMax(&nr[sg.box], dbits(mr));
+        if (nf) atomicMax(&nf[sg.box], dbits(mf));
     }
 }
 
 // phi += e (level 0).
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_update0(const CellSeg* segs, int nseg, const int* active,
-                                                    double* __restrict__ phi, const T* __restrict__ e) {
+                                                    double* __restrict__ phi, const T* e0, const T* e1) {
     const int s = find_seg(segs, nseg, blockIdx.x);
     const CellSeg& sg = segs[s];
     if (!active[sg.box]) return;
+    const T* __restrict__ e = sg.ebuf ? e1 : e0;
     const long long n = (long long)sg.n[0] * sg.n[1] * sg.n[2];
     const long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x;
     if (q < n) phi[sg.coff + q] = phi[sg.coff + q] + (double)e[sg.off + q];

@@ -124,6 +124,8 @@ struct nyxpm_ctx {
     int precision = NYXPM_FP64;
     int nu1 = 2, nu2 = 2, max_iter = 200;
     int graph_cycles = 24;
+    bool no_fused = false;  // force half-sweep smoothing (for comparisons)
+    bool untiled_kick = false;  // force the thread-per-particle kick (for comparisons)
     hipStream_t stream = nullptr;
     bool own_stream = false;
     std::vector<Box> boxes;
@@ -148,13 +150,17 @@ struct nyxpm_ctx {
     long long* tstart = nullptr;
     long long* bsum = nullptr;
     int scan_blocks = 0;
+    TileItem* items = nullptr;  // tile chunks of the last sort (tile-staged kernels)
+    int* nitems = nullptr;
+    long long items_cap = 0;
+    bool items_valid = false;
 
     // level-0 fields
     double *own = nullptr, *halo = nullptr, *rho = nullptr, *f = nullptr, *phi = nullptr, *partial = nullptr;
     double* gtmp = nullptr;  // 3 fields of the largest box (grav vector / downloads)
     // MG pools
-    double *r64 = nullptr, *e64 = nullptr, *G64 = nullptr;
-    float *r32 = nullptr, *e32 = nullptr, *G32 = nullptr;
+    double *r64 = nullptr, *e64 = nullptr, *e64b = nullptr, *G64 = nullptr;
+    float *r32 = nullptr, *e32 = nullptr, *e32b = nullptr, *G32 = nullptr;
     // per-box device state
     int *active = nullptr, *iters = nullptr, *failed = nullptr, *err_flag = nullptr;
     unsigned long long *nr = nullptr, *nf = nullptr;
@@ -170,8 +176,12 @@ struct nyxpm_ctx {
     struct Table {
         int first = 0, count = 0, blocks = 0;
     };
-    std::vector<Table> t_full, t_half, t_coarse;  // per depth (non-bottom boxes)
-    Table t_bottom, t_l0;
+    std::vector<Table> t_full, t_half, t_coarse, t_sweep;  // per depth (non-bottom boxes)
+    std::vector<int> sweep_tx;   // fused-sweep column width per depth (0: half sweeps)
+    std::vector<int> sweep_ty;   // fused-sweep column rows per depth (8 or 16)
+    std::vector<int> pre_buf;    // buffer holding e after pre-smoothing, per depth
+    std::vector<int> fin_buf;    // buffer holding e after post-smoothing, per depth
+    Table t_bottom, t_l0, t_res;  // t_res: level 0, at most kReduceBlocks blocks per box
     BoxSeg* pseg = nullptr;
     int pseg_n = 0, pseg_blocks = 0;
     BoxSeg* tseg = nullptr;
@@ -389,22 +399,76 @@ int build_tables(nyxpm_ctx* c) {
         t.blocks = blk;
         return t;
     };
+    // fused sweeps where every box at the depth has a column width dividing nx,
+    // rows of 8 and even dimensions; buffer schedule follows from nu1/nu2
     c->t_full.assign(c->dmax, {});
     c->t_half.assign(c->dmax, {});
     c->t_coarse.assign(c->dmax, {});
+    c->t_sweep.assign(c->dmax, {});
+    c->sweep_tx.assign(c->dmax, 0);
+    c->sweep_ty.assign(c->dmax, 8);
+    c->pre_buf.assign(c->dmax, 0);
+    c->fin_buf.assign(c->dmax, 0);
     for (int d = 0; d + 1 < c->dmax; ++d) {
-        std::vector<CellSeg> vf, vh, vc;
+        int tx = 64;
+        bool any = false;
+        for (int b = 0; b < nbox; ++b) {
+            const Box& B = c->boxes[b];
+            if (d + 1 >= B.nlev) continue;
+            any = true;
+            const Level& L = B.lev[d];
+            while (tx >= 16 && (L.n[0] % tx)) tx >>= 1;
+            if (L.n[1] % 8 || L.n[0] % 2 || L.n[2] % 2) tx = 0;
+        }
+        if (!any || tx < 16 || c->no_fused) tx = 0;
+        c->sweep_tx[d] = tx;
+        // 16-row columns where every box allows it and keeps >= 64 columns
+        int ty = 16;
+        for (int b = 0; b < nbox && tx; ++b) {
+            const Box& B = c->boxes[b];
+            if (d + 1 >= B.nlev) continue;
+            const Level& L = B.lev[d];
+            if (tx != 64 || L.n[1] % 16 || (long long)(L.n[0] / tx) * (L.n[1] / 16) < 64) ty = 8;
+        }
+        c->sweep_ty[d] = tx ? ty : 8;
+        int cur = 0;
+        if (tx) {
+            for (int q = 1; q < c->nu1; ++q) cur ^= 1;
+            c->pre_buf[d] = cur;
+            for (int q = 0; q < c->nu2; ++q) cur ^= 1;
+            c->fin_buf[d] = cur;
+        }
+    }
+    auto coarse_buf = [&](const Box& B, int d) {  // buffer of level d+1's result
+        return (d + 1 == B.nlev - 1) ? 0 : c->fin_buf[d + 1];
+    };
+    for (int d = 0; d + 1 < c->dmax; ++d) {
+        std::vector<CellSeg> vf, vh, vc, vs;
         for (int b = 0; b < nbox; ++b) {
             const Box& B = c->boxes[b];
             if (d + 1 >= B.nlev) continue;
             const Level& L = B.lev[d];
-            vf.push_back(mk(b, d, L.nc));
+            CellSeg f = mk(b, d, L.nc);
+            f.ebuf = coarse_buf(B, d);
+            vf.push_back(f);
             vh.push_back(mk(b, d, L.nc / 2));
             vc.push_back(mk(b, d, L.nc / 8));
+            if (c->sweep_tx[d]) {
+                const int tx = c->sweep_tx[d];
+                const long long cols = (long long)(L.n[0] / tx) * (L.n[1] / c->sweep_ty[d]);
+                long long nzc = 2048 / std::max(1LL, cols);
+                nzc = std::max(1LL, std::min(nzc, (long long)std::max(1, L.n[2] / 8)));
+                CellSeg w = mk(b, d, 1);
+                w.zchunk = (int)((L.n[2] + nzc - 1) / nzc);
+                nzc = (L.n[2] + w.zchunk - 1) / w.zchunk;
+                w.nblk = (int)(cols * nzc);
+                vs.push_back(w);
+            }
         }
         c->t_full[d] = add_table(vf);
         c->t_half[d] = add_table(vh);
         c->t_coarse[d] = add_table(vc);
+        c->t_sweep[d] = add_table(vs);
     }
     {
         std::vector<CellSeg> vb;
@@ -422,9 +486,18 @@ int build_tables(nyxpm_ctx* c) {
         for (int b = 0; b < nbox; ++b) {
             CellSeg s = mk(b, 0, c->boxes[b].lev[0].nc);
             s.coff = c->boxes[b].cell_off;  // level-0 field pools
+            s.ebuf = c->boxes[b].nlev == 1 ? 0 : c->fin_buf[0];
             v0.push_back(s);
         }
         c->t_l0 = add_table(v0);
+        std::vector<CellSeg> vr;
+        for (int b = 0; b < nbox; ++b) {
+            CellSeg s = mk(b, 0, c->boxes[b].lev[0].nc);
+            s.coff = c->boxes[b].cell_off;
+            s.nblk = std::min(s.nblk, kReduceBlocks);
+            vr.push_back(s);
+        }
+        c->t_res = add_table(vr);
     }
     int rc = dalloc(c, &c->seg_pool, S.size());
     if (rc) return rc;
@@ -517,12 +590,14 @@ int alloc_fields(nyxpm_ctx* c) {
     if ((rc = dalloc(c, &c->bsum, c->scan_blocks + 1))) return rc;
     if ((rc = dalloc(c, &c->r64, c->mg_total))) return rc;
     if ((rc = dalloc(c, &c->e64, c->mg_total))) return rc;
+    if ((rc = dalloc(c, &c->e64b, c->mg_total))) return rc;
     if ((rc = dalloc(c, &c->G64, c->G_total))) return rc;
     if ((rc = dalloc(c, &c->G32, c->G_total))) return rc;
     if ((rc = dalloc(c, &c->active, nbox))) return rc;
     if ((rc = dalloc(c, &c->iters, nbox))) return rc;
     if ((rc = dalloc(c, &c->failed, nbox))) return rc;
     if ((rc = dalloc(c, &c->err_flag, 1))) return rc;
+    if ((rc = dalloc(c, &c->nitems, 1))) return rc;
     if ((rc = dalloc(c, &c->nr, nbox))) return rc;
     if ((rc = dalloc(c, &c->nf, nbox))) return rc;
     if ((rc = dalloc(c, &c->state, 4 * nbox))) return rc;
@@ -553,6 +628,7 @@ int ensure_mixed_pools(nyxpm_ctx* c) {
     int rc;
     if (!c->r32 && (rc = dalloc(c, &c->r32, c->mg_total))) return rc;
     if (!c->e32 && (rc = dalloc(c, &c->e32, c->mg_total))) return rc;
+    if (!c->e32b && (rc = dalloc(c, &c->e32b, c->mg_total))) return rc;
     return NYXPM_OK;
 }
 
@@ -563,6 +639,7 @@ void free_particles(nyxpm_ctx* c) {
         dfree(c->P[s].m); dfree(c->P[s].id);
     }
     dfree(c->key);
+    dfree(c->items);
 }
 
 int alloc_particle_set(nyxpm_ctx* c, PartPtrs& P, long long n, bool mass) {
@@ -624,10 +701,17 @@ int phase_sort(nyxpm_ctx* c) {
         hipLaunchKernelGGL(k_scatter, dim3(c->pseg_blocks), dim3(kBlock), 0, c->stream, c->pseg, c->pseg_n,
                            c->dboxes, c->key, c->tstart, c->cursor, A, B);
     }
+    {
+        Timed t(c, "items");
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(kBlock), 0, c->stream, c->nitems, 1LL, 0);
+        hipLaunchKernelGGL(k_make_items, dim3(nblocks(c->ntiles_total)), dim3(kBlock), 0, c->stream, c->tstart,
+                           c->ntiles_total, c->items, c->nitems);
+    }
     LAUNCHCK(c);
     c->cur = 1 - c->cur;
     c->sorted_valid = true;
     c->keys_valid = false;
+    c->items_valid = true;
     return NYXPM_OK;
 }
 
@@ -646,8 +730,8 @@ int phase_deposit(nyxpm_ctx* c) {
     }
     {
         Timed t(c, "fold");
-        hipLaunchKernelGGL(k_fold, dim3(c->t_l0.blocks), dim3(kBlock), 0, c->stream, c->segs(c->t_l0),
-                           c->t_l0.count, c->dboxes, c->own, c->halo, c->rho, c->f, c->partial);
+        hipLaunchKernelGGL(k_fold, dim3(c->t_res.blocks), dim3(kBlock), 0, c->stream, c->segs(c->t_res),
+                           c->t_res.count, c->dboxes, c->own, c->halo, c->rho, c->f, c->partial);
     }
     LAUNCHCK(c);
     c->density_valid = true;
@@ -662,9 +746,9 @@ int phase_rhs(nyxpm_ctx* c, double a, double four_pi_G) {
     {
         Timed t(c, "rhs");
         if (!c->f1_valid)
-            hipLaunchKernelGGL(k_rhs1, dim3(c->t_l0.blocks), dim3(kBlock), 0, c->stream, c->segs(c->t_l0),
-                               c->t_l0.count, c->dboxes, c->rho, c->f, c->partial);
-        hipLaunchKernelGGL(k_rhs_sum, dim3(c->t_l0.count), dim3(kBlock), 0, c->stream, c->segs(c->t_l0),
+            hipLaunchKernelGGL(k_rhs1, dim3(c->t_res.blocks), dim3(kBlock), 0, c->stream, c->segs(c->t_res),
+                               c->t_res.count, c->dboxes, c->rho, c->f, c->partial);
+        hipLaunchKernelGGL(k_rhs_sum, dim3(c->t_res.count), dim3(kBlock), 0, c->stream, c->segs(c->t_res),
                            c->partial, c->dv_size, c->solv);
         hipLaunchKernelGGL(k_rhs_final, dim3(c->t_l0.blocks), dim3(kBlock), 0, c->stream, c->segs(c->t_l0),
                            c->t_l0.count, c->dboxes, c->solv, c->f, Ggravity, inv_a);
@@ -680,62 +764,110 @@ template <>
 struct Pools<double> {
     static double* r(nyxpm_ctx* c) { return c->r64; }
     static double* e(nyxpm_ctx* c) { return c->e64; }
+    static double* e2(nyxpm_ctx* c) { return c->e64b; }
     static double* G(nyxpm_ctx* c) { return c->G64; }
 };
 template <>
 struct Pools<float> {
     static float* r(nyxpm_ctx* c) { return c->r32; }
     static float* e(nyxpm_ctx* c) { return c->e32; }
+    static float* e2(nyxpm_ctx* c) { return c->e32b; }
     static float* G(nyxpm_ctx* c) { return c->G32; }
 };
+
+template <typename T, bool ZERO>
+void launch_sweep(nyxpm_ctx* c, int d, const T* ein, T* eout, const T* r) {
+    const auto& t = c->t_sweep[d];
+    if (c->sweep_ty[d] == 16) {
+        hipLaunchKernelGGL((k_rb_sweep<T, 64, 16, ZERO>), dim3(t.blocks), dim3(512), 0, c->stream, c->segs(t),
+                           t.count, c->active, ein, eout, r);
+        return;
+    }
+    switch (c->sweep_tx[d]) {
+        case 64:
+            hipLaunchKernelGGL((k_rb_sweep<T, 64, 8, ZERO>), dim3(t.blocks), dim3(256), 0, c->stream, c->segs(t),
+                               t.count, c->active, ein, eout, r);
+            break;
+        case 32:
+            hipLaunchKernelGGL((k_rb_sweep<T, 32, 8, ZERO>), dim3(t.blocks), dim3(128), 0, c->stream, c->segs(t),
+                               t.count, c->active, ein, eout, r);
+            break;
+        default:
+            hipLaunchKernelGGL((k_rb_sweep<T, 16, 8, ZERO>), dim3(t.blocks), dim3(64), 0, c->stream, c->segs(t),
+                               t.count, c->active, ein, eout, r);
+            break;
+    }
+}
 
 template <typename T>
 int vcycle(nyxpm_ctx* c) {
     T* r = Pools<T>::r(c);
-    T* e = Pools<T>::e(c);
+    T* eb[2] = {Pools<T>::e(c), Pools<T>::e2(c)};
     const int D = c->dmax;
     for (int d = 0; d + 1 < D; ++d) {
         auto& tf = c->t_full[d];
         auto& th = c->t_half[d];
         auto& tc = c->t_coarse[d];
         if (tf.count == 0) continue;
-        {
-            Timed t(c, lvl_name("fill", d));
-            hipLaunchKernelGGL(k_fill<T>, dim3(tf.blocks), dim3(kBlock), 0, c->stream, c->segs(tf), tf.count,
-                               c->active, e, T(0));
-        }
-        for (int s = 0; s < c->nu1; ++s)
-            for (int col = 0; col < 2; ++col) {
-                Timed t(c, lvl_name("smooth", d));
-                hipLaunchKernelGGL(k_gsrb_half<T>, dim3(th.blocks), dim3(kBlock), 0, c->stream, c->segs(th),
-                                   th.count, c->active, e, r, col);
+        int cur = 0;
+        if (c->sweep_tx[d]) {
+            {
+                Timed t(c, lvl_name("sweep0", d));
+                launch_sweep<T, true>(c, d, eb[0], eb[0], r);
             }
+            for (int q = 1; q < c->nu1; ++q) {
+                Timed t(c, lvl_name("sweep", d));
+                launch_sweep<T, false>(c, d, eb[cur], eb[1 - cur], r);
+                cur ^= 1;
+            }
+        } else {
+            {
+                Timed t(c, lvl_name("fill", d));
+                hipLaunchKernelGGL(k_fill<T>, dim3(tf.blocks), dim3(kBlock), 0, c->stream, c->segs(tf), tf.count,
+                                   c->active, eb[0], T(0));
+            }
+            for (int q = 0; q < c->nu1; ++q)
+                for (int col = 0; col < 2; ++col) {
+                    Timed t(c, lvl_name("smooth", d));
+                    hipLaunchKernelGGL(k_gsrb_half<T>, dim3(th.blocks), dim3(kBlock), 0, c->stream, c->segs(th),
+                                       th.count, c->active, eb[0], r, col);
+                }
+        }
         {
             Timed t(c, lvl_name("resid_restrict", d));
             hipLaunchKernelGGL(k_resid_restrict<T>, dim3(tc.blocks), dim3(kBlock), 0, c->stream, c->segs(tc),
-                               tc.count, c->active, e, r);
+                               tc.count, c->active, eb[cur], r);
         }
     }
     {
         Timed t(c, "bottom");
         hipLaunchKernelGGL(k_bottom<T>, dim3(c->t_bottom.count), dim3(kBlock), 0, c->stream, c->segs(c->t_bottom),
-                           c->t_bottom.count, c->active, e, r, Pools<T>::G(c));
+                           c->t_bottom.count, c->active, eb[0], r, Pools<T>::G(c));
     }
     for (int d = D - 2; d >= 0; --d) {
         auto& tf = c->t_full[d];
         auto& th = c->t_half[d];
         if (tf.count == 0) continue;
+        int cur = c->pre_buf[d];
         {
             Timed t(c, lvl_name("prolong", d));
             hipLaunchKernelGGL(k_prolong<T>, dim3(tf.blocks), dim3(kBlock), 0, c->stream, c->segs(tf), tf.count,
-                               c->active, e);
+                               c->active, eb[cur], eb[0], eb[1]);
         }
-        for (int s = 0; s < c->nu2; ++s)
-            for (int col = 0; col < 2; ++col) {
-                Timed t(c, lvl_name("smooth", d));
-                hipLaunchKernelGGL(k_gsrb_half<T>, dim3(th.blocks), dim3(kBlock), 0, c->stream, c->segs(th),
-                                   th.count, c->active, e, r, col);
+        if (c->sweep_tx[d]) {
+            for (int q = 0; q < c->nu2; ++q) {
+                Timed t(c, lvl_name("sweep", d));
+                launch_sweep<T, false>(c, d, eb[cur], eb[1 - cur], r);
+                cur ^= 1;
             }
+        } else {
+            for (int q = 0; q < c->nu2; ++q)
+                for (int col = 0; col < 2; ++col) {
+                    Timed t(c, lvl_name("smooth", d));
+                    hipLaunchKernelGGL(k_gsrb_half<T>, dim3(th.blocks), dim3(kBlock), 0, c->stream, c->segs(th),
+                                       th.count, c->active, eb[0], r, col);
+                }
+        }
     }
     LAUNCHCK(c);
     return NYXPM_OK;
@@ -744,8 +876,8 @@ int vcycle(nyxpm_ctx* c) {
 template <typename T>
 int residual(nyxpm_ctx* c, bool with_f) {
     Timed t(c, "residual");
-    hipLaunchKernelGGL(k_residual0<T>, dim3(c->t_l0.blocks), dim3(kBlock), 0, c->stream, c->segs(c->t_l0),
-                       c->t_l0.count, c->active, c->f, c->phi, Pools<T>::r(c), c->nr, with_f ? c->nf : nullptr);
+    hipLaunchKernelGGL(k_residual0<T>, dim3(c->t_res.blocks), dim3(kBlock), 0, c->stream, c->segs(c->t_res),
+                       c->t_res.count, c->active, c->f, c->phi, Pools<T>::r(c), c->nr, with_f ? c->nf : nullptr);
     return NYXPM_OK;
 }
 
@@ -753,7 +885,7 @@ template <typename T>
 int update(nyxpm_ctx* c) {
     Timed t(c, "update");
     hipLaunchKernelGGL(k_update0<T>, dim3(c->t_l0.blocks), dim3(kBlock), 0, c->stream, c->segs(c->t_l0),
-                       c->t_l0.count, c->active, c->phi, Pools<T>::e(c));
+                       c->t_l0.count, c->active, c->phi, Pools<T>::e(c), Pools<T>::e2(c));
     return NYXPM_OK;
 }
 
@@ -839,6 +971,21 @@ int solve_results(nyxpm_ctx* c, int* iters, double* final_res, bool allow_active
 template <bool MOVE>
 int phase_kick(nyxpm_ctx* c, double dt, double a_prev, double a_cur) {
     PartPtrs& A = c->P[c->cur];
+    if (c->items_valid && !c->untiled_kick) {
+        const size_t lds = sizeof(double) * (c->tile[0] + 3) * (c->tile[1] + 3) * (c->tile[2] + 3);
+        Timed t(c, MOVE ? "kick_drift" : "kick");
+        if (MOVE)
+            hipLaunchKernelGGL(k_fill_int, dim3(nblocks(c->ntiles_total)), dim3(kBlock), 0, c->stream, c->counts,
+                               c->ntiles_total, 0);
+        hipLaunchKernelGGL(k_kick_tiled<MOVE>, dim3((unsigned)c->items_cap), dim3(kBlock), lds, c->stream, c->tseg,
+                           c->tseg_n, c->dboxes, c->items, c->nitems, c->phi, A.x, A.y, A.z, A.vx, A.vy, A.vz, dt,
+                           a_prev, a_cur, c->key, c->counts, c->err_flag);
+        LAUNCHCK(c);
+        c->keys_valid = MOVE;
+        c->sorted_valid = false;
+        c->density_valid = false;
+        return NYXPM_OK;
+    }
     if (MOVE) {
         Timed t(c, "kick_drift");
         hipLaunchKernelGGL(k_fill_int, dim3(nblocks(c->ntiles_total)), dim3(kBlock), 0, c->stream, c->counts,
@@ -866,7 +1013,7 @@ int check_box(nyxpm_ctx* c, int box) {
 }
 
 int require_particles(nyxpm_ctx* c) {
-    if (c->npart_total <= 0 || !c->P[0].x) FAIL(c, NYXPM_ERR_STATE, "no particles uploaded");
+    if (!c->P[0].x) FAIL(c, NYXPM_ERR_STATE, "no particles uploaded");
     return NYXPM_OK;
 }
 
@@ -917,6 +1064,8 @@ int nyxpm_create_batch(nyxpm_ctx** out, int nbox, const nyxpm_geom* geoms, const
         for (int d = 0; d < 3; ++d)
             if (opts->tile[d] > 0) c->tile[d] = opts->tile[d];
         c->verbose = opts->verbose;
+        c->no_fused = opts->smoother == 1;
+        c->untiled_kick = opts->kick == 1;
     }
     *out = c;
     for (int d = 0; d < 3; ++d)
@@ -947,8 +1096,10 @@ int nyxpm_destroy(nyxpm_ctx* c) {
     free_particles(c);
     dfree(c->counts); dfree(c->cursor); dfree(c->tstart); dfree(c->bsum);
     dfree(c->own); dfree(c->halo); dfree(c->rho); dfree(c->f); dfree(c->phi); dfree(c->partial); dfree(c->gtmp);
-    dfree(c->r64); dfree(c->e64); dfree(c->G64); dfree(c->r32); dfree(c->e32); dfree(c->G32);
-    dfree(c->active); dfree(c->iters); dfree(c->failed); dfree(c->err_flag); dfree(c->nr); dfree(c->nf);
+
+    dfree(c->r64); dfree(c->e64); dfree(c->e64b); dfree(c->G64);
+    dfree(c->r32); dfree(c->e32); dfree(c->e32b); dfree(c->G32);
+    dfree(c->active); dfree(c->iters); dfree(c->failed); dfree(c->err_flag); dfree(c->nitems); dfree(c->nr); dfree(c->nf);
     dfree(c->state); dfree(c->solv); dfree(c->dv_size); dfree(c->dboxes);
     dfree(c->seg_pool); dfree(c->pseg); dfree(c->tseg);
     if (c->h_flags) hipHostFree(c->h_flags);
@@ -1069,10 +1220,17 @@ static int upload_impl(nyxpm_ctx* c, int box, int64_t np, const double* x, const
         LAUNCHCK(c);
     }
     HIPCK(c, hipStreamSynchronize(c->stream));
+    const long long icap = c->ntiles_total + tot / kChunk + 1;
+    if (icap > 0x7fffffffLL) FAIL(c, NYXPM_ERR_ARG, "too many tile chunks");
+    TileItem* nitm = nullptr;
+    if ((rc = dalloc(c, &nitm, icap))) return rc;
     free_particles(c);
     c->P[0] = N[0];
     c->P[1] = N[1];
     c->key = nkey;
+    c->items = nitm;
+    c->items_cap = icap;
+    c->items_valid = false;
     c->cur = 0;
     c->part_cap = cap;
     c->any_mass = need_mass;

@@ -20,6 +20,7 @@ constexpr int kWave = 64;          // CDNA wavefront width
 constexpr int kBlock = 256;        // threads per block of the grid kernels
 constexpr int kMaxLevels = 24;
 constexpr int kBottomMaxCells = 4096;
+constexpr int kReduceBlocks = 2048;  // blocks per box of the reducing level-0 kernels
 
 // One periodic box as the particle kernels see it.
 struct BoxDev {
@@ -51,6 +52,9 @@ struct CellSeg {
     long long off;         // offset of this level in its pool
     long long coff;        // offset of the next coarser level (restrict/prolong),
                            // or of the bottom Green's function (bottom solve)
+    int zchunk;            // z-planes per block of the fused sweep
+    int ebuf;              // which correction buffer (0: e, 1: e2) holds the coarse
+                           // level's result (prolong) / the level's result (update)
     double dh[3];          // 1/dx^2 per dimension on this level
     double gamma;          // -2 (dhx + dhy + dhz)
     double inv_gamma;      // 1/gamma

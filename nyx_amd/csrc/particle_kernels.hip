@@ -303,9 +303,9 @@ __global__ void __launch_bounds__(kBlock) k_fold(const CellSeg* segs, int nseg, 
     const CellSeg& sg = segs[find_seg(segs, nseg, blockIdx.x)];
     const BoxDev& B = boxes[sg.box];
     const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
-    const long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x;
-    double fv = 0.0;
-    if (q < (long long)nx * ny * nz) {
+    double acc = 0.0;  // this thread's cells, in increasing order
+    for (long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x; q < (long long)nx * ny * nz;
+         q += (long long)sg.nblk * kBlock) {
         const int i = (int)(q % nx);
         const long long t = q / nx;
         const int j = (int)(t % ny), k = (int)(t / ny);
@@ -333,10 +333,11 @@ __global__ void __launch_bounds__(kBlock) k_fold(const CellSeg* segs, int nseg, 
         }
         const double rv = v * B.inv_vol;
         rho[B.cell_off + q] = rv;
-        fv = rv + (-B.mass_offset);
+        const double fv = rv + (-B.mass_offset);
         f[B.cell_off + q] = fv;
+        acc += fv;
     }
-    const double bs = block_sum_fixed(fv);
+    const double bs = block_sum_fixed(acc);
     if (threadIdx.x == 0) partial[blockIdx.x] = bs;
 }
 
@@ -394,7 +395,7 @@ __device__ __forceinline__ double periodic_shift(double v, double lo, double rhi
 
 // val[d] = sum_{kk,jj,ii} sx[1-ii]*sy[1-jj]*sz[1-kk] * g_d(i-ii, j-jj, k-kk) with
 // g_d(c) = 0.5*(dxi*(phi(c) - phi(c-e_d)) + dxi*(phi(c+e_d) - phi(c))).
-__device__ __forceinline__ void gather_grad(const BoxDev& B, const double* __restrict__ P, double x, double y,
+__device__ __forceinline__ void gather_grad_inl(const BoxDev& B, const double* __restrict__ P, double x, double y,
                                             double z, double val[3], int* err) {
     const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
     double xint, yint, zint;
@@ -435,6 +436,11 @@ __device__ __forceinline__ void gather_grad(const BoxDev& B, const double* __res
     val[0] = v0;
     val[1] = v1;
     val[2] = v2;
+}
+
+__device__ __forceinline__ void gather_grad(const BoxDev& B, const double* __restrict__ P, double x, double y,
+                                            double z, double val[3], int* err) {
+    gather_grad_inl(B, P, x, y, z, val, err);
 }
 
 // moveKickDrift (DO_MOVE) or moveKick (!DO_MOVE) of update_dm_particle_single:
@@ -489,6 +495,179 @@ __global__ void __launch_bounds__(kBlock) k_kick(const BoxSeg* segs, int nseg, c
         }
     }
     if (DO_MOVE) agg_atomic_add(counts, t, valid);
+}
+
+// CIC-weighted central-difference gradient at a particle, evaluated in two
+// (kk, jj) loops so that only one x-line of the stencil is live at a time.
+// ld(a, b, c) returns phi at cell (i-2+a, j-2+b, k-2+c).  Summation order is that
+// of update_dm_particle_single: kk, then jj, then ii (ii = 0: cell i, ii = 1:
+// cell i-1), each term ((sx*sy)*sz)*g.
+template <class LD>
+__device__ __forceinline__ void cic_grad_lines(LD ld, double xint, double yint, double zint, double dxi0,
+                                               double dxi1, double dxi2, double val[3]) {
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+#pragma unroll 1
+    for (int kk = 0; kk <= 1; ++kk) {
+        const double wz = kk ? 1.0 - zint : zint;
+#pragma unroll 1
+        for (int jj = 0; jj <= 1; ++jj) {
+            const double wy = jj ? 1.0 - yint : yint;
+            const int b = 2 - jj, c = 2 - kk;
+            const double l0 = ld(0, b, c), l1 = ld(1, b, c), l2 = ld(2, b, c), l3 = ld(3, b, c);
+            {   // ii = 0: cell i (a = 2)
+                const double pc = l2;
+                const double gx = 0.5 * (dxi0 * (pc - l1) + dxi0 * (l3 - pc));
+                const double gy = 0.5 * (dxi1 * (pc - ld(2, b - 1, c)) + dxi1 * (ld(2, b + 1, c) - pc));
+                const double gz = 0.5 * (dxi2 * (pc - ld(2, b, c - 1)) + dxi2 * (ld(2, b, c + 1) - pc));
+                const double w = xint * wy * wz;
+                v0 += w * gx;
+                v1 += w * gy;
+                v2 += w * gz;
+            }
+            {   // ii = 1: cell i-1 (a = 1)
+                const double pc = l1;
+                const double gx = 0.5 * (dxi0 * (pc - l0) + dxi0 * (l2 - pc));
+                const double gy = 0.5 * (dxi1 * (pc - ld(1, b - 1, c)) + dxi1 * (ld(1, b + 1, c) - pc));
+                const double gz = 0.5 * (dxi2 * (pc - ld(1, b, c - 1)) + dxi2 * (ld(1, b, c + 1) - pc));
+                const double w = (1.0 - xint) * wy * wz;
+                v0 += w * gx;
+                v1 += w * gy;
+                v2 += w * gz;
+            }
+        }
+    }
+    val[0] = v0;
+    val[1] = v1;
+    val[2] = v2;
+}
+
+// Work items for the tile-staged particle kernels: each tile's particle range
+// cut into chunks of at most kChunk particles (empty tiles give none).  Slots are
+// reserved with one atomic per wave, so the item order varies between runs; every
+// item is processed independently, so results do not.
+constexpr int kChunk = 1024;
+
+struct TileItem {
+    long long p0;   // first particle
+    int np;         // particles in this chunk
+    int tile;       // global tile id
+};
+
+__global__ void __launch_bounds__(kBlock) k_make_items(const long long* __restrict__ start, long long ntiles,
+                                                       TileItem* __restrict__ items, int* nitems) {
+    const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+    int nc = 0;
+    long long p0 = 0, cnt = 0;
+    if (t < ntiles) {
+        p0 = start[t];
+        cnt = start[t + 1] - p0;
+        nc = (int)((cnt + kChunk - 1) / kChunk);
+    }
+    // wave-inclusive prefix of nc, one atomic per wave
+    const int lane = threadIdx.x & (kWave - 1);
+    int incl = nc;
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int v = __shfl_up(incl, o);
+        if (lane >= o) incl += v;
+    }
+    const int total = __shfl(incl, kWave - 1);
+    int base = 0;
+    if (lane == kWave - 1 && total > 0) base = atomicAdd(nitems, total);
+    base = __shfl(base, kWave - 1);
+    int slot = base + incl - nc;
+    for (int c = 0; c < nc; ++c) {
+        TileItem it;
+        it.p0 = p0 + (long long)c * kChunk;
+        it.np = (int)min((long long)kChunk, cnt - (long long)c * kChunk);
+        it.tile = (int)t;
+        items[slot + c] = it;
+    }
+}
+
+// Tile-staged gather + kick(-drift): phi over the tile plus a halo of 2 (the
+// gradient stencil of the CIC cells i-1, i of every particle binned here) is
+// staged in LDS once per item; particles whose cell index falls outside the tile
+// (binning older than a periodic shift) take the global-memory path.  Same
+// arithmetic as k_kick, bit for bit.
+template <bool DO_MOVE>
+__global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int ntseg, const BoxDev* boxes,
+                                                       const TileItem* __restrict__ items, const int* nitems,
+                                                       const double* __restrict__ phi, double* __restrict__ x,
+                                                       double* __restrict__ y, double* __restrict__ z,
+                                                       double* __restrict__ vx, double* __restrict__ vy,
+                                                       double* __restrict__ vz, double dt, double a_prev,
+                                                       double a_cur, int* __restrict__ key, int* counts, int* err) {
+    extern __shared__ double sphi[];
+    if ((int)blockIdx.x >= *nitems) return;
+    const TileItem it = items[blockIdx.x];
+    const int sgi = find_seg(tsegs, ntseg, it.tile);
+    const BoxDev& B = boxes[tsegs[sgi].box];
+    const int lt = it.tile - tsegs[sgi].blk0;
+    const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
+    const int tx = lt % B.ntile[0], ty = (lt / B.ntile[0]) % B.ntile[1], tz = lt / (B.ntile[0] * B.ntile[1]);
+    const int o0 = tx * B.tile[0], o1 = ty * B.tile[1], o2 = tz * B.tile[2];
+    const int bx = min(B.tile[0], nx - o0), by = min(B.tile[1], ny - o1), bz = min(B.tile[2], nz - o2);
+    const int SX = bx + 3, SY = by + 3, SZ = bz + 3;
+    const double* P = phi + B.cell_off;
+    for (int idx = threadIdx.x; idx < SX * SY * SZ; idx += kBlock) {
+        const int lx = idx % SX, ly = (idx / SX) % SY, lz = idx / (SX * SY);
+        const int gx = wrap1(o0 - 2 + lx, nx), gy = wrap1(o1 - 2 + ly, ny), gz = wrap1(o2 - 2 + lz, nz);
+        sphi[idx] = P[cidx(gx, gy, gz, nx, ny)];
+    }
+    __syncthreads();
+    const double half_dt = 0.5 * dt;
+    const double a_cur_inv = 1.0 / a_cur;
+    const double dt_a_cur_inv = dt * a_cur_inv;
+    const double dxi0 = B.dxi[0], dxi1 = B.dxi[1], dxi2 = B.dxi[2];
+    for (int q = threadIdx.x; q < it.np; q += kBlock) {
+        const long long p = it.p0 + q;
+        double px = x[p], py = y[p], pz = z[p];
+        double xint, yint, zint;
+        const int i = cic_index(px, B.plo[0], B.dxi[0], &xint);
+        const int j = cic_index(py, B.plo[1], B.dxi[1], &yint);
+        const int k = cic_index(pz, B.plo[2], B.dxi[2], &zint);
+        const int li = wrap1(i, nx) - o0, lj = wrap1(j, ny) - o1, lk = wrap1(k, nz) - o2;
+        double val[3];
+        if (li >= 0 && li < bx && lj >= 0 && lj < by && lk >= 0 && lk < bz) {
+            auto ld = [&](int a, int b, int c) -> double { return sphi[((lk + c) * SY + (lj + b)) * SX + (li + a)]; };
+            cic_grad_lines(ld, xint, yint, zint, dxi0, dxi1, dxi2, val);
+        } else {
+            // binned in another tile (cannot happen after a sort; kept for safety)
+            if (i < -nx + 2 || i >= 2 * nx - 1 || j < -ny + 2 || j >= 2 * ny - 1 || k < -nz + 2 || k >= 2 * nz - 1)
+                atomicOr(err, 4);
+            auto ld = [&](int a, int b, int c) -> double {
+                return P[cidx(wrap1(i - 2 + a, nx), wrap1(j - 2 + b, ny), wrap1(k - 2 + c, nz), nx, ny)];
+            };
+            cic_grad_lines(ld, xint, yint, zint, dxi0, dxi1, dxi2, val);
+        }
+        double ux = a_prev * vx[p] + half_dt * val[0];
+        ux = ux * a_cur_inv;
+        double uy = a_prev * vy[p] + half_dt * val[1];
+        uy = uy * a_cur_inv;
+        double uz = a_prev * vz[p] + half_dt * val[2];
+        uz = uz * a_cur_inv;
+        vx[p] = ux;
+        vy[p] = uy;
+        vz[p] = uz;
+        if (DO_MOVE) {
+            px = px + dt_a_cur_inv * ux;
+            py = py + dt_a_cur_inv * uy;
+            pz = pz + dt_a_cur_inv * uz;
+            x[p] = px;
+            y[p] = py;
+            z[p] = pz;
+            const int t = tile_of(B, px, py, pz, err);
+            key[p] = t;
+            agg_atomic_add(counts, t, true);
+        } else {
+            const double sxp = periodic_shift(px, B.plo[0], B.rhi[0], B.len[0], err);
+            const double syp = periodic_shift(py, B.plo[1], B.rhi[1], B.len[1], err);
+            const double szp = periodic_shift(pz, B.plo[2], B.rhi[2], B.len[2], err);
+            if (sxp != px) x[p] = sxp;
+            if (syp != py) y[p] = syp;
+            if (szp != pz) z[p] = szp;
+        }
+    }
 }
 
 // Periodic shift alone (nyxpm_redistribute).
@@ -563,13 +742,14 @@ __global__ void __launch_bounds__(kBlock) k_rhs1(const CellSeg* segs, int nseg, 
     const CellSeg& sg = segs[find_seg(segs, nseg, blockIdx.x)];
     const BoxDev& B = boxes[sg.box];
     const long long n = (long long)B.n[0] * B.n[1] * B.n[2];
-    const long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x;
-    double fv = 0.0;
-    if (q < n) {
-        fv = rho[B.cell_off + q] + (-B.mass_offset);
+    double acc = 0.0;
+    for (long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x; q < n;
+         q += (long long)sg.nblk * kBlock) {
+        const double fv = rho[B.cell_off + q] + (-B.mass_offset);
         f[B.cell_off + q] = fv;
+        acc += fv;
     }
-    const double bs = block_sum_fixed(fv);
+    const double bs = block_sum_fixed(acc);
     if (threadIdx.x == 0) partial[blockIdx.x] = bs;
 }
 

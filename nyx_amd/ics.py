@@ -65,7 +65,7 @@ def grf_displacement(n, seed, index=-2.0, rms_cells=0.5):
     phik = wk * amp
     psi = np.empty((3, n, n, n))
     for d, K in enumerate((KX, KY, KZ)):
-        psi[d] = np.fft.irfftn(1j * K * phik, s=(n, n, n))
+        psi[d] = np.fft.irfftn(1j * K * phik, s=(n, n, n), axes=(0, 1, 2))
     rms = np.sqrt(np.mean(np.sum(psi ** 2, axis=0)))
     psi *= (rms_cells / n) / rms
     return psi

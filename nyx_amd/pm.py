@@ -47,7 +47,8 @@ class BoxGeom:
 class PMContext:
     """One library context: a batch of independent periodic boxes on one GPU."""
 
-    def __init__(self, geoms, precision="fp64", tile=None, nu1=2, nu2=2, max_iter=200):
+    def __init__(self, geoms, precision="fp64", tile=None, nu1=2, nu2=2, max_iter=200, smoother="fused",
+                 kick="tiled"):
         if isinstance(geoms, BoxGeom):
             geoms = [geoms]
         self.geoms = list(geoms)
@@ -61,6 +62,8 @@ class PMContext:
         o = Opts()
         o.precision = NYXPM_MIXED if precision == "mixed" else NYXPM_FP64
         o.nu1, o.nu2, o.max_iter = int(nu1), int(nu2), int(max_iter)
+        o.smoother = 1 if smoother == "half" else 0
+        o.kick = 1 if kick == "global" else 0
         if tile is not None:
             for d in range(3):
                 o.tile[d] = int(tile[d])

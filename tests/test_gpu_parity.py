@@ -114,6 +114,22 @@ def test_mg_iterates_bitwise(nyx, orc, n, L):
     assert np.array_equal(phi, ref)
 
 
+@pytest.mark.parametrize("n", [(64, 64, 64), (96, 32, 48), (128, 16, 16)])
+@pytest.mark.parametrize("nu", [(2, 2), (1, 3)])
+def test_fused_sweeps_equal_half_sweeps(nyx, orc, n, nu):
+    r = np.random.default_rng(11)
+    f = r.standard_normal((n[2], n[1], n[0]))
+    f -= f.mean()
+    out = []
+    for sm in ("fused", "half"):
+        with ctx1(nyx, n, smoother=sm, nu1=nu[0], nu2=nu[1]) as c:
+            phi, it, _ = c.solve_poisson(f, fixed_iters=2)
+            out.append(phi)
+    assert np.array_equal(out[0], out[1])
+    ref, _, _, _ = orc.mg_solve(n, [1.0 / v for v in n], f, fixed_iters=2, nu1=nu[0], nu2=nu[1])
+    assert np.array_equal(out[0], ref)
+
+
 def test_mg_converged_meets_rule(nyx, orc):
     n = (64, 64, 64)
     dx = [1 / 64] * 3
@@ -179,6 +195,24 @@ def test_kick_wraps_like_redistribute(nyx, orc, golden):
     for d in range(3):
         assert np.array_equal(pos[d], p2[d])
         assert np.array_equal(vel[d], v2[d])
+
+
+@pytest.mark.parametrize("tile", [(8, 8, 8), (16, 8, 4)])
+def test_tiled_kick_equals_global_kick(nyx, tile):
+    # clustered particles (heavy tiles split into chunks), several steps so that
+    # the kick-drift runs on a binning one periodic shift old
+    pos, vel, m = nyx.ics.plummer_clustered(32, 78, n_spheres=10)
+    vel = [v * 20 for v in vel]
+    out = []
+    for kick in ("tiled", "global"):
+        with ctx1(nyx, (32, 32, 32), tile=tile, kick=kick) as c:
+            c.upload(0, pos, vel, m_const=m)
+            c.pm_init(1.0, 1.0, 1e-10)
+            c.pm_steps(3, 1e-2, rel_tol=1e-10)
+            out.append(c.download())
+    for d in range(3):
+        assert np.array_equal(out[0][0][d], out[1][0][d])
+        assert np.array_equal(out[0][1][d], out[1][1][d])
 
 
 # ---------------------------------------------------------------------------
