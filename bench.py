@@ -121,6 +121,8 @@ def main():
     ap.add_argument("--graph", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--json-out", default=None)
+    ap.add_argument("--modes", default="mixed,fp64", help="precisions to time (comma list)")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the mixed-vs-FP64 acceleration check")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md, "
@@ -146,13 +148,16 @@ def main():
     log("FP64 initial solve")
     ctx.set_precision("fp64")
     ctx.pm_init(1.0, 1.0, a.tol)
-    g64 = ctx.gather_accel()
-    ctx.set_precision("mixed")
-    it_m, _ = ctx.solve_for_phi(1.0, 1.0, a.tol, 0.0, use_guess=False)
-    gmx = ctx.gather_accel()
-    rel_l2 = float(np.sqrt(np.sum((gmx - g64) ** 2)) / np.sqrt(np.sum(g64 ** 2)))
-    del g64, gmx
-    log(f"accel rel-L2 mixed vs fp64 = {rel_l2:.3e}  (mixed cold-start cycles {int(it_m[0])})")
+    rel_l2 = None
+    if not a.no_accuracy:
+        g64 = ctx.gather_accel()
+        ctx.set_precision("mixed")
+        it_m, _ = ctx.solve_for_phi(1.0, 1.0, a.tol, 0.0, use_guess=False)
+        gmx = ctx.gather_accel()
+        rel_l2 = float(np.sqrt(np.sum((gmx - g64) ** 2)) / np.sqrt(np.sum(g64 ** 2)))
+        del g64, gmx
+        log(f"accel rel-L2 mixed vs fp64 = {rel_l2:.3e}  (mixed cold-start cycles {int(it_m[0])})")
+    modes = a.modes.split(",")
 
     names = ["residual", "norm", "update", "deposit", "fold", "rhs", "kick_drift", "kick", "scatter", "scan",
              "items", "keys", "bottom"] + \
@@ -161,15 +166,24 @@ def main():
     tile = 8
     halo_cells = (ncell // tile ** 3) * ((tile + 1) ** 3 - tile ** 3)
 
-    log(f"mixed: {a.warmup} warmup + {a.steps} timed steps")
-    t_mix, it_mix = run_mode(ctx, "mixed", a.steps, a.warmup, dt, a.tol, a.graph)
-    st_mix = stage_table(ctx, names, ncell, npart, halo_cells, 4, t_mix * 1e3)
-    log(f"  {t_mix / a.steps * 1e3:.2f} ms/step, {it_mix} V-cycles")
-    log(f"fp64: {a.warmup} warmup + {a.steps} timed steps")
-    t_64, it_64 = run_mode(ctx, "fp64", a.steps, a.warmup, dt, a.tol, a.graph)
-    st_64 = stage_table(ctx, names, ncell, npart, halo_cells, 8, t_64 * 1e3)
-    log(f"  {t_64 / a.steps * 1e3:.2f} ms/step, {it_64} V-cycles")
+    t_mix = t_64 = None
+    it_mix = it_64 = None
+    st_mix, st_64 = {}, {}
+    if "mixed" in modes:
+        log(f"mixed: {a.warmup} warmup + {a.steps} timed steps")
+        t_mix, it_mix = run_mode(ctx, "mixed", a.steps, a.warmup, dt, a.tol, a.graph)
+        st_mix = stage_table(ctx, names, ncell, npart, halo_cells, 4, t_mix * 1e3)
+        log(f"  {t_mix / a.steps * 1e3:.2f} ms/step, {it_mix} V-cycles")
+    if "fp64" in modes:
+        log(f"fp64: {a.warmup} warmup + {a.steps} timed steps")
+        t_64, it_64 = run_mode(ctx, "fp64", a.steps, a.warmup, dt, a.tol, a.graph)
+        st_64 = stage_table(ctx, names, ncell, npart, halo_cells, 8, t_64 * 1e3)
+        log(f"  {t_64 / a.steps * 1e3:.2f} ms/step, {it_64} V-cycles")
     ctx.close()
+    if t_mix is None:  # FP64 only: report it as the value
+        t_mix, it_mix, st_mix = t_64, it_64, st_64
+    if t_64 is None:
+        t_64, it_64, st_64 = t_mix, it_mix, st_mix
 
     v_mix = npart * a.steps / t_mix
     v_64 = npart * a.steps / t_64

@@ -82,6 +82,9 @@ typedef struct nyxpm_opts {
                           1: separate red and black half sweeps everywhere */
     int kick;          /* 0: tile-staged gather + kick (default); 1: one thread per
                           particle reading phi from global memory */
+    int sweep_tx, sweep_ty; /* fused-sweep column shape for every level that allows
+                          it (128x8, 64x16, 64x8, 32x8, 16x8); 0 = chosen per level */
+    int sweep_zchunk;  /* z-planes per fused-sweep block; 0 = chosen per level */
 } nyxpm_opts;
 
 /* ---- lifecycle ---------------------------------------------------------- */

@@ -97,13 +97,14 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
     constexpr int NE = (EX * EY + NT - 1) / NT, NR = (RX * RY + NT - 1) / NT;
     __shared__ T se[4][EY][EX];
     __shared__ T sr[2][RY][RX];
-    const int s = find_seg(segs, nseg, blockIdx.x);
+    const int bid = xcd_block(blockIdx.x, gridDim.x);
+    const int s = find_seg(segs, nseg, bid);
     const CellSeg& sg = segs[s];
     if (!active[sg.box]) return;
     const int nx = sg.n[0], ny = sg.n[1], nz = sg.n[2];
     const int ntx = nx / TX, nty = ny / TY;
     const int zc = sg.zchunk;
-    const int b = blockIdx.x - sg.blk0;
+    const int b = bid - sg.blk0;
     const int tx = b % ntx, ty = (b / ntx) % nty, tz = b / (ntx * nty);
     const int x0 = tx * TX, y0 = ty * TY, z0 = tz * zc;
     const int z1 = min(nz, z0 + zc);
@@ -177,12 +178,13 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
             fetch_r(kr + 1);
         }
         __syncthreads();
-        {   // red cells of plane kr on the halo-1 region
+        {   // red cells of plane kr on the halo-1 region: (i+j+k) even, which is
+            // consistent across the periodic seams because every n is even
             const int sm = (kr - 1) & 3, sc = kr & 3, sp = (kr + 1) & 3, rs = kr & 1;
-            for (int idx = tid; idx < RX * RY; idx += NT) {
-                const int ly = idx / RX, lx = idx - ly * RX;
-                const int gx = x0 + lx - 1, gy = y0 + ly - 1;
-                if (((gx + gy + kr) & 1) != 0) continue;  // red: (i+j+k) even (periodic-consistent, n even)
+            constexpr int HX = RX / 2;  // red cells per row of the region
+            for (int idx = tid; idx < HX * RY; idx += NT) {
+                const int ly = idx / HX, mm = idx - ly * HX;
+                const int lx = 2 * mm + ((x0 + y0 + ly + kr) & 1);  // gx + gy + kr even
                 const int ex = lx + 1, ey = ly + 1;
                 const T ec = se[sc][ey][ex];
                 const T res = sr[rs][ly][lx] - gamma * ec
@@ -309,14 +311,15 @@ __global__ void __launch_bounds__(kBlock) k_residual0(const CellSeg* segs, int n
                                                       const double* __restrict__ f,
                                                       const double* __restrict__ phi, T* __restrict__ r,
                                                       unsigned long long* nr, unsigned long long* nf) {
-    const int s = find_seg(segs, nseg, blockIdx.x);
+    const int bid = xcd_block(blockIdx.x, gridDim.x);
+    const int s = find_seg(segs, nseg, bid);
     const CellSeg& sg = segs[s];
     if (!active[sg.box]) return;
     const int nx = sg.n[0], ny = sg.n[1], nz = sg.n[2];
     const long long nc = (long long)nx * ny * nz;
     const double* P = phi + sg.coff;
     double ar = 0.0, af = 0.0;
-    for (long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x; q < nc;
+    for (long long q = (long long)(bid - sg.blk0) * kBlock + threadIdx.x; q < nc;
          q += (long long)sg.nblk * kBlock) {
         const int i = (int)(q % nx);
         const long long t = q / nx;

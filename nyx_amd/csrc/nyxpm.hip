@@ -126,6 +126,8 @@ struct nyxpm_ctx {
     int graph_cycles = 24;
     bool no_fused = false;  // force half-sweep smoothing (for comparisons)
     bool untiled_kick = false;  // force the thread-per-particle kick (for comparisons)
+    int sweep_zchunk = 0;       // fixed z-chunk of the fused sweep (0: chosen per level)
+    int force_tx[2] = {0, 0}, force_ty[2] = {8, 8};  // fixed sweep shape per precision (0: chosen)
     hipStream_t stream = nullptr;
     bool own_stream = false;
     std::vector<Box> boxes;
@@ -176,9 +178,14 @@ struct nyxpm_ctx {
     struct Table {
         int first = 0, count = 0, blocks = 0;
     };
-    std::vector<Table> t_full, t_half, t_coarse, t_sweep;  // per depth (non-bottom boxes)
-    std::vector<int> sweep_tx;   // fused-sweep column width per depth (0: half sweeps)
-    std::vector<int> sweep_ty;   // fused-sweep column rows per depth (8 or 16)
+    std::vector<Table> t_full, t_half, t_coarse;  // per depth (non-bottom boxes)
+    // fused-sweep column shape and launch table per precision (0: FP64, 1: FP32)
+    // and depth; tx == 0 means separate half sweeps at that depth
+    struct SweepCfg {
+        int tx = 0, ty = 8;
+        Table t;
+    };
+    std::vector<SweepCfg> sw[2];
     std::vector<int> pre_buf;    // buffer holding e after pre-smoothing, per depth
     std::vector<int> fin_buf;    // buffer holding e after post-smoothing, per depth
     Table t_bottom, t_l0, t_res;  // t_res: level 0, at most kReduceBlocks blocks per box
@@ -399,40 +406,57 @@ int build_tables(nyxpm_ctx* c) {
         t.blocks = blk;
         return t;
     };
-    // fused sweeps where every box at the depth has a column width dividing nx,
-    // rows of 8 and even dimensions; buffer schedule follows from nu1/nu2
+    // Fused sweeps where every box at the depth has a column shape dividing its
+    // level and even dimensions; the buffer schedule follows from nu1/nu2 (it does
+    // not depend on the column shape, so both precisions share it).
     c->t_full.assign(c->dmax, {});
     c->t_half.assign(c->dmax, {});
     c->t_coarse.assign(c->dmax, {});
-    c->t_sweep.assign(c->dmax, {});
-    c->sweep_tx.assign(c->dmax, 0);
-    c->sweep_ty.assign(c->dmax, 8);
     c->pre_buf.assign(c->dmax, 0);
     c->fin_buf.assign(c->dmax, 0);
+    static const int shapes64[][2] = {{64, 16}, {64, 8}, {32, 8}, {16, 8}};
+    static const int shapes32[][2] = {{128, 8}, {64, 16}, {64, 8}, {32, 8}, {16, 8}};
+    for (int pr = 0; pr < 2; ++pr) c->sw[pr].assign(c->dmax, {});
+    std::vector<bool> fusable(c->dmax, false);
     for (int d = 0; d + 1 < c->dmax; ++d) {
-        int tx = 64;
-        bool any = false;
+        bool any = false, even = true;
         for (int b = 0; b < nbox; ++b) {
             const Box& B = c->boxes[b];
             if (d + 1 >= B.nlev) continue;
             any = true;
             const Level& L = B.lev[d];
-            while (tx >= 16 && (L.n[0] % tx)) tx >>= 1;
-            if (L.n[1] % 8 || L.n[0] % 2 || L.n[2] % 2) tx = 0;
+            if (L.n[0] % 2 || L.n[1] % 2 || L.n[2] % 2) even = false;
         }
-        if (!any || tx < 16 || c->no_fused) tx = 0;
-        c->sweep_tx[d] = tx;
-        // 16-row columns where every box allows it and keeps >= 64 columns
-        int ty = 16;
-        for (int b = 0; b < nbox && tx; ++b) {
-            const Box& B = c->boxes[b];
-            if (d + 1 >= B.nlev) continue;
-            const Level& L = B.lev[d];
-            if (tx != 64 || L.n[1] % 16 || (long long)(L.n[0] / tx) * (L.n[1] / 16) < 64) ty = 8;
+        fusable[d] = any && even && !c->no_fused;
+        for (int pr = 0; pr < 2 && fusable[d]; ++pr) {
+            const int ns = pr == 0 ? 4 : 5;
+            for (int q = 0; q < ns; ++q) {
+                const int tx = pr == 0 ? shapes64[q][0] : shapes32[q][0];
+                const int ty = pr == 0 ? shapes64[q][1] : shapes32[q][1];
+                if (c->force_tx[pr] && (tx != c->force_tx[pr] || ty != c->force_ty[pr])) continue;
+                bool ok = true;
+                for (int b = 0; b < nbox && ok; ++b) {
+                    const Box& B = c->boxes[b];
+                    if (d + 1 >= B.nlev) continue;
+                    const Level& L = B.lev[d];
+                    if (L.n[0] % tx || L.n[1] % ty) ok = false;
+                    // the wide shapes only where they still give >= 64 columns
+                    if (!c->force_tx[pr] && (tx * ty > 512) && (long long)(L.n[0] / tx) * (L.n[1] / ty) < 64)
+                        ok = false;
+                }
+                if (ok) {
+                    c->sw[pr][d].tx = tx;
+                    c->sw[pr][d].ty = ty;
+                    break;
+                }
+            }
         }
-        c->sweep_ty[d] = tx ? ty : 8;
+        if (fusable[d] && (c->sw[0][d].tx == 0 || c->sw[1][d].tx == 0)) {
+            c->sw[0][d].tx = c->sw[1][d].tx = 0;
+            fusable[d] = false;
+        }
         int cur = 0;
-        if (tx) {
+        if (fusable[d]) {
             for (int q = 1; q < c->nu1; ++q) cur ^= 1;
             c->pre_buf[d] = cur;
             for (int q = 0; q < c->nu2; ++q) cur ^= 1;
@@ -443,7 +467,7 @@ int build_tables(nyxpm_ctx* c) {
         return (d + 1 == B.nlev - 1) ? 0 : c->fin_buf[d + 1];
     };
     for (int d = 0; d + 1 < c->dmax; ++d) {
-        std::vector<CellSeg> vf, vh, vc, vs;
+        std::vector<CellSeg> vf, vh, vc, vs[2];
         for (int b = 0; b < nbox; ++b) {
             const Box& B = c->boxes[b];
             if (d + 1 >= B.nlev) continue;
@@ -453,22 +477,30 @@ int build_tables(nyxpm_ctx* c) {
             vf.push_back(f);
             vh.push_back(mk(b, d, L.nc / 2));
             vc.push_back(mk(b, d, L.nc / 8));
-            if (c->sweep_tx[d]) {
-                const int tx = c->sweep_tx[d];
-                const long long cols = (long long)(L.n[0] / tx) * (L.n[1] / c->sweep_ty[d]);
-                long long nzc = 2048 / std::max(1LL, cols);
+            for (int pr = 0; pr < 2; ++pr) {
+                const int tx = c->sw[pr][d].tx, ty = c->sw[pr][d].ty;
+                if (!tx) continue;
+                const long long cols = (long long)(L.n[0] / tx) * (L.n[1] / ty);
+                // about four rounds of resident blocks (LDS- and thread-limited
+                // residency per CU), chunks of at least 8 planes
+                const int sz = pr == 0 ? 8 : 4;
+                const long long lds = (4LL * (ty + 4) * (tx + 4) + 2LL * (ty + 2) * (tx + 2)) * sz;
+                const long long per_cu = std::max(1LL, std::min(2048LL / (tx * ty / 2), (160LL * 1024) / lds));
+                const long long resident = 256LL * per_cu;
+                long long nzc = (4 * resident + cols - 1) / std::max(1LL, cols);
+                if (c->sweep_zchunk > 0) nzc = (L.n[2] + c->sweep_zchunk - 1) / c->sweep_zchunk;
                 nzc = std::max(1LL, std::min(nzc, (long long)std::max(1, L.n[2] / 8)));
                 CellSeg w = mk(b, d, 1);
                 w.zchunk = (int)((L.n[2] + nzc - 1) / nzc);
                 nzc = (L.n[2] + w.zchunk - 1) / w.zchunk;
                 w.nblk = (int)(cols * nzc);
-                vs.push_back(w);
+                vs[pr].push_back(w);
             }
         }
         c->t_full[d] = add_table(vf);
         c->t_half[d] = add_table(vh);
         c->t_coarse[d] = add_table(vc);
-        c->t_sweep[d] = add_table(vs);
+        for (int pr = 0; pr < 2; ++pr) c->sw[pr][d].t = add_table(vs[pr]);
     }
     {
         std::vector<CellSeg> vb;
@@ -777,26 +809,24 @@ struct Pools<float> {
 
 template <typename T, bool ZERO>
 void launch_sweep(nyxpm_ctx* c, int d, const T* ein, T* eout, const T* r) {
-    const auto& t = c->t_sweep[d];
-    if (c->sweep_ty[d] == 16) {
-        hipLaunchKernelGGL((k_rb_sweep<T, 64, 16, ZERO>), dim3(t.blocks), dim3(512), 0, c->stream, c->segs(t),
-                           t.count, c->active, ein, eout, r);
-        return;
-    }
-    switch (c->sweep_tx[d]) {
-        case 64:
-            hipLaunchKernelGGL((k_rb_sweep<T, 64, 8, ZERO>), dim3(t.blocks), dim3(256), 0, c->stream, c->segs(t),
-                               t.count, c->active, ein, eout, r);
-            break;
-        case 32:
-            hipLaunchKernelGGL((k_rb_sweep<T, 32, 8, ZERO>), dim3(t.blocks), dim3(128), 0, c->stream, c->segs(t),
-                               t.count, c->active, ein, eout, r);
-            break;
+    const auto& cfg = c->sw[sizeof(T) == 8 ? 0 : 1][d];
+    const auto& t = cfg.t;
+    const int key = cfg.tx * 100 + cfg.ty;
+#define NYXPM_SWEEP(TX, TY)                                                                                  \
+    case TX * 100 + TY:                                                                                      \
+        hipLaunchKernelGGL((k_rb_sweep<T, TX, TY, ZERO>), dim3(t.blocks), dim3(TX * TY / 2), 0, c->stream, \
+                           c->segs(t), t.count, c->active, ein, eout, r);                                   \
+        break;
+    switch (key) {
+        NYXPM_SWEEP(128, 8)
+        NYXPM_SWEEP(64, 16)
+        NYXPM_SWEEP(64, 8)
+        NYXPM_SWEEP(32, 8)
+        NYXPM_SWEEP(16, 8)
         default:
-            hipLaunchKernelGGL((k_rb_sweep<T, 16, 8, ZERO>), dim3(t.blocks), dim3(64), 0, c->stream, c->segs(t),
-                               t.count, c->active, ein, eout, r);
             break;
     }
+#undef NYXPM_SWEEP
 }
 
 template <typename T>
@@ -810,7 +840,7 @@ int vcycle(nyxpm_ctx* c) {
         auto& tc = c->t_coarse[d];
         if (tf.count == 0) continue;
         int cur = 0;
-        if (c->sweep_tx[d]) {
+        if (c->sw[sizeof(T) == 8 ? 0 : 1][d].tx) {
             {
                 Timed t(c, lvl_name("sweep0", d));
                 launch_sweep<T, true>(c, d, eb[0], eb[0], r);
@@ -854,7 +884,7 @@ int vcycle(nyxpm_ctx* c) {
             hipLaunchKernelGGL(k_prolong<T>, dim3(tf.blocks), dim3(kBlock), 0, c->stream, c->segs(tf), tf.count,
                                c->active, eb[cur], eb[0], eb[1]);
         }
-        if (c->sweep_tx[d]) {
+        if (c->sw[sizeof(T) == 8 ? 0 : 1][d].tx) {
             for (int q = 0; q < c->nu2; ++q) {
                 Timed t(c, lvl_name("sweep", d));
                 launch_sweep<T, false>(c, d, eb[cur], eb[1 - cur], r);
@@ -1066,6 +1096,11 @@ int nyxpm_create_batch(nyxpm_ctx** out, int nbox, const nyxpm_geom* geoms, const
         c->verbose = opts->verbose;
         c->no_fused = opts->smoother == 1;
         c->untiled_kick = opts->kick == 1;
+        if (opts->sweep_tx > 0) {
+            c->force_tx[0] = c->force_tx[1] = opts->sweep_tx;
+            c->force_ty[0] = c->force_ty[1] = opts->sweep_ty > 0 ? opts->sweep_ty : 8;
+        }
+        if (opts->sweep_zchunk > 0) c->sweep_zchunk = opts->sweep_zchunk;
     }
     *out = c;
     for (int d = 0; d < 3; ++d)

@@ -67,6 +67,15 @@ struct BoxSeg {
     int pad;
 };
 
+// XCD-aware block order.  Workgroups of a launch are dealt to the 8 XCDs round
+// robin (block b runs on XCD b % 8), and each XCD has its own L2.  This bijection
+// gives XCD x a contiguous range of logical blocks, so neighbouring tiles (which
+// share halo cache lines) run under the same L2.
+__device__ __forceinline__ int xcd_block(int b, int G) {
+    const int q = G >> 3, rm = G & 7, x = b & 7, j = b >> 3;
+    return (x < rm ? x * (q + 1) : rm * (q + 1) + (x - rm) * q) + j;
+}
+
 template <typename S>
 __device__ __forceinline__ int find_seg(const S* segs, int nseg, int blk) {
     int lo = 0, hi = nseg - 1;

@@ -48,7 +48,7 @@ class PMContext:
     """One library context: a batch of independent periodic boxes on one GPU."""
 
     def __init__(self, geoms, precision="fp64", tile=None, nu1=2, nu2=2, max_iter=200, smoother="fused",
-                 kick="tiled"):
+                 kick="tiled", sweep_shape=None, sweep_zchunk=0):
         if isinstance(geoms, BoxGeom):
             geoms = [geoms]
         self.geoms = list(geoms)
@@ -64,6 +64,9 @@ class PMContext:
         o.nu1, o.nu2, o.max_iter = int(nu1), int(nu2), int(max_iter)
         o.smoother = 1 if smoother == "half" else 0
         o.kick = 1 if kick == "global" else 0
+        if sweep_shape is not None:
+            o.sweep_tx, o.sweep_ty = int(sweep_shape[0]), int(sweep_shape[1])
+        o.sweep_zchunk = int(sweep_zchunk)
         if tile is not None:
             for d in range(3):
                 o.tile[d] = int(tile[d])

@@ -130,6 +130,22 @@ def test_fused_sweeps_equal_half_sweeps(nyx, orc, n, nu):
     assert np.array_equal(out[0], ref)
 
 
+@pytest.mark.parametrize("shape,zc", [((128, 8), 0), ((64, 16), 0), ((64, 8), 5), ((32, 8), 0), ((16, 8), 3)])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_sweep_shapes_equal_half_sweeps(nyx, shape, zc, precision):
+    # every fused-sweep column shape and uneven z-chunks, FP64 and FP32 V-cycles
+    n = (256, 64, 32)
+    r = np.random.default_rng(12)
+    f = r.standard_normal((n[2], n[1], n[0]))
+    f -= f.mean()
+    out = []
+    for sm in ("fused", "half"):
+        with ctx1(nyx, n, smoother=sm, precision=precision, sweep_shape=shape, sweep_zchunk=zc) as c:
+            phi, it, _ = c.solve_poisson(f, fixed_iters=2)
+            out.append(phi)
+    assert np.array_equal(out[0], out[1])
+
+
 def test_mg_converged_meets_rule(nyx, orc):
     n = (64, 64, 64)
     dx = [1 / 64] * 3
