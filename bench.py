@@ -44,6 +44,8 @@ def kernel_bytes(name, ncell, npart, halo_cells, mixed_sz):
         "smooth.d0": (sz + sz / 2 + sz / 2) * N,          # read e (all), r (colour), write e (colour)
         "sweep.d0": 3 * sz * N,                            # fused red+black: read e, r; write e
         "sweep0.d0": 2 * sz * N,                           # first sweep from e = 0: read r; write e
+        "sweep_p.d0": (3 * sz + sz / 8) * N,               # + coarse correction read (prolongation on load)
+        "sweep_u.d0": (2 * sz + 16) * N,                   # read e, r; phi += e (read/write FP64 phi)
         "resid_restrict.d0": (2 * sz + sz / 8) * N,        # read e, r; write coarse r
         "prolong.d0": (2 * sz + sz / 8) * N,               # read/write e, read coarse e
         "fill.d0": sz * N,
@@ -123,11 +125,21 @@ def main():
     ap.add_argument("--json-out", default=None)
     ap.add_argument("--modes", default="mixed,fp64", help="precisions to time (comma list)")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the mixed-vs-FP64 acceleration check")
+    ap.add_argument("--no-tune", action="store_true", help="skip the smoother-shape autotuner")
+    ap.add_argument("--workload", default="clustered512",
+                    choices=["clustered512", "batch48", "grf256"],
+                    help="clustered512: the headline line; batch48: 48 boxes (16 each of 64^3, 96^3, "
+                         "128^3) in one launch per phase vs one after another; grf256: FP64 drop-in, "
+                         "256^3 GRF, 10 steps")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md, "
               "'How the path would shard'); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if a.workload == "batch48":
+        return main_batch(a)
+    if a.workload == "grf256":
+        return main_grf256(a)
 
     import nyx_amd
     from nyx_amd import ics
@@ -143,6 +155,11 @@ def main():
     npart = n ** 3
     ncell = n ** 3
     log("device bytes", ctx.device_bytes())
+    tuned = None
+    if not a.no_tune:
+        t = time.perf_counter()
+        tuned = ctx.autotune()
+        log(f"autotuned smoother shapes in {time.perf_counter() - t:.1f} s: {tuned}")
 
     # accuracy: same particles, same density; FP64 solve vs mixed solve
     log("FP64 initial solve")
@@ -161,7 +178,8 @@ def main():
 
     names = ["residual", "norm", "update", "deposit", "fold", "rhs", "kick_drift", "kick", "scatter", "scan",
              "items", "keys", "bottom"] + \
-            [f"{k}.d{d}" for k in ("sweep0", "sweep", "smooth", "resid_restrict", "prolong", "fill")
+            [f"{k}.d{d}" for k in ("sweep0", "sweep", "sweep_p", "sweep_u", "smooth", "resid_restrict", "prolong",
+                                    "fill")
              for d in range(0, 12)]
     tile = 8
     halo_cells = (ncell // tile ** 3) * ((tile + 1) ** 3 - tile ** 3)
@@ -224,7 +242,7 @@ def main():
         "config": {"workload": f"{n}^3 cells, {n}^3 particles, 20% uniform + 80% in "
                                f"{max(1, int(2000 * (n / 512) ** 3))} Plummer spheres, mixed precision "
                                f"(FP32 V-cycles in FP64 refinement, FP64 particles), dt 1e-3, rel tol {a.tol:g}",
-                   "cells": ncell, "particles": npart, "graph": bool(a.graph)},
+                   "cells": ncell, "particles": npart, "graph": bool(a.graph), "autotuned": tuned},
         "fp64": {"value": round(v_64, 1), "ms_per_step": round(t_64 / a.steps * 1e3, 3), "vcycles": it_64,
                  "dtype": "f64"},
         "mixed_over_fp64": round(v_mix / v_64, 3),
@@ -241,6 +259,97 @@ def main():
         with open(a.json_out, "w") as fh:
             fh.write(line + "\n")
     print(line, flush=True)
+
+
+def emit(out, a):
+    line = json.dumps(out)
+    if a.json_out:
+        with open(a.json_out, "w") as fh:
+            fh.write(line + "\n")
+    print(line, flush=True)
+
+
+def main_batch(a):
+    """48 independent periodic boxes (16 each of 64^3, 96^3 and 128^3 cells,
+    particles = cells), lattice + GRF displacements (P(k) ~ k^-2, rms 0.5 cell,
+    v = 0.1 psi), each with its own seed; 20 hipGraph-captured KDK steps in one
+    launch per phase, FP64 then mixed, against the same boxes advanced one after
+    another (one context each, same step code)."""
+    import nyx_amd
+    from nyx_amd import ics
+    sizes = [64] * 16 + [96] * 16 + [128] * 16
+    steps = a.steps if a.steps != 5 else 20
+    log("generating 48 GRF boxes")
+    inputs = [ics.grf_lattice(n, ics.SEED_BASE + 3 + 1000 * b, -2.0, 0.5, 0.1) for b, n in enumerate(sizes)]
+    npart = sum(n ** 3 for n in sizes)
+    res = {}
+    for prec in ("fp64", "mixed"):
+        ctx = nyx_amd.PMContext([nyx_amd.BoxGeom((n, n, n)) for n in sizes], precision=prec)
+        tuned = ctx.autotune(fp64=prec == "fp64", fp32=prec == "mixed")
+        log(prec, "autotuned", tuned)
+        for b, (pos, vel, m) in enumerate(inputs):
+            ctx.upload(b, pos, vel, m_const=m)
+        ctx.pm_init(1.0, 1.0, a.tol)
+        ctx.pm_steps(2, 1e-3, rel_tol=a.tol, use_graph=True)  # capture + warm up
+        ctx.sync()
+        t0 = time.perf_counter()
+        it = ctx.pm_steps(steps, 1e-3, rel_tol=a.tol, use_graph=True)
+        ctx.sync()
+        tb = time.perf_counter() - t0
+        ctx.close()
+        # one after another
+        tseq = 0.0
+        for b, (pos, vel, m) in enumerate(inputs):
+            c1 = nyx_amd.PMContext(nyx_amd.BoxGeom((sizes[b],) * 3), precision=prec)
+            if b in (0, 16, 32):  # tune once per box size; the context is rebuilt per box
+                seq_tuned = c1.autotune(fp64=prec == "fp64", fp32=prec == "mixed")
+            c1.upload(0, pos, vel, m_const=m)
+            c1.pm_init(1.0, 1.0, a.tol)
+            c1.pm_steps(2, 1e-3, rel_tol=a.tol, use_graph=True)
+            c1.sync()
+            t0 = time.perf_counter()
+            c1.pm_steps(steps, 1e-3, rel_tol=a.tol, use_graph=True)
+            c1.sync()
+            tseq += time.perf_counter() - t0
+            c1.close()
+        res[prec] = {"batched": round(npart * steps / tb, 1), "sequential": round(npart * steps / tseq, 1),
+                     "speedup": round(tseq / tb, 3), "ms_per_step_batched": round(tb / steps * 1e3, 3),
+                     "vcycles_last_step": [int(v) for v in it[:1]] + [int(it[16]), int(it[32])]}
+        log(prec, res[prec])
+    out = {"metric": "batched PM KDK particle-steps/s, 48 boxes (16x64^3 + 16x96^3 + 16x128^3)",
+           "value": res["mixed"]["batched"], "unit": "particle-steps/s", "n_gpus": 1, "steps": steps,
+           "warmup": 2, "ms_per_step": res["mixed"]["ms_per_step_batched"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+           "config": {"workload": "48 periodic boxes, lattice + GRF (k^-2, rms 0.5 cell), hipGraph-captured steps",
+                      "particles": npart, "graph": True},
+           "fp64": res["fp64"], "mixed": res["mixed"]}
+    emit(out, a)
+
+
+def main_grf256(a):
+    """The FP64 drop-in configuration: 256^3 cells and particles, lattice + GRF
+    (P(k) ~ k^-2, rms 0.5 cell, v = 0.1 psi), 10 KDK steps, dt 1e-3, tol 1e-10."""
+    import nyx_amd
+    from nyx_amd import ics
+    n = 256
+    steps = 10
+    pos, vel, m = ics.grf_lattice(n, ics.SEED_BASE + 1, -2.0, 0.5, 0.1)
+    ctx = nyx_amd.PMContext(nyx_amd.BoxGeom((n, n, n)), precision="fp64")
+    ctx.upload(0, pos, vel, m_const=m)
+    ctx.pm_init(1.0, 1.0, a.tol)
+    ctx.sync()
+    t0 = time.perf_counter()
+    it = ctx.pm_steps(steps, 1e-3, rel_tol=a.tol, use_graph=a.graph)
+    ctx.sync()
+    t = time.perf_counter() - t0
+    ctx.close()
+    out = {"metric": "FP64 PM KDK particle-steps/s, 256^3 GRF drop-in", "value": round(n ** 3 * steps / t, 1),
+           "unit": "particle-steps/s", "n_gpus": 1, "steps": steps, "warmup": 0,
+           "ms_per_step": round(t / steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "256^3 cells/particles, GRF k^-2 rms 0.5 cell, 10 steps", "graph": bool(a.graph)},
+           "vcycles_last_step": int(it[0])}
+    emit(out, a)
 
 
 if __name__ == "__main__":

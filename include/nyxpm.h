@@ -172,6 +172,16 @@ int nyxpm_pm_steps(nyxpm_ctx *ctx, int nsteps, double dt, double a_old, double a
                    double four_pi_G, double rel_tol, double abs_tol, int nyx_faithful,
                    int use_graph, int *iters);
 
+/* ---- tuning ------------------------------------------------------------- */
+/* Autotune the fused smoother: for every multigrid depth (and precision in
+ * precision_mask: bit 0 FP64, bit 1 FP32) time each column shape the level
+ * allows (128x8, 64x16, 64x8, 32x8, 16x8) and z-chunk (8..256 planes) on this
+ * context's own level arrays, and keep the fastest.  Results never depend on the
+ * choice (every shape is bit-identical).  report (may be NULL) receives a JSON
+ * object {"fp64.d0": {"shape": [tx, ty], "zchunk": z, "us": t}, ...}.  No
+ * counterpart in the reference (AMReX's MLMG has fixed launch shapes). */
+int nyxpm_autotune(nyxpm_ctx *ctx, int precision_mask, char *report, int report_len);
+
 /* ---- measurement -------------------------------------------------------- */
 /* Kernel-class timing with HIP events on the context stream.  When enabled, each
  * launch of a timed class is bracketed by events; nyxpm_timing_get returns the

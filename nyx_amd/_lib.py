@@ -79,6 +79,7 @@ SIGNATURES = [
     ("nyxpm_redistribute", _i, [_vp]),
     ("nyxpm_pm_init", _i, [_vp, _d, _d, _d, _d]),
     ("nyxpm_pm_steps", _i, [_vp, _i, _d, _d, _d, _d, _d, _d, _i, _i, _vp]),
+    ("nyxpm_autotune", _i, [_vp, _i, _vp, _i]),
     ("nyxpm_timing_enable", _i, [_vp, _i]),
     ("nyxpm_timing_get", _i, [_vp, _s, _vp, _vp, _vp]),
     ("nyxpm_timing_reset", _i, [_vp]),

@@ -80,17 +80,29 @@ __global__ void __launch_bounds__(kBlock) k_gsrb_half(const CellSeg* segs, int n
 // and 2 planes of r (halo 1) in LDS.  Per plane kr: red cells of plane kr are
 // updated in LDS (including a one-cell ring around the column, recomputed
 // redundantly), then the black cells of plane kr-1 are updated from the new red
-// values and the plane's final values are written to eout.  Reads ein, writes
-// eout (a different buffer), so neighbouring blocks never see half-updated
-// values; the arithmetic per cell is exactly that of k_gsrb_half, so the result
-// equals two half sweeps bit for bit.  ZERO_IN: ein is identically zero (first
-// sweep after the correction is reset), so it is not read.
+// values and the plane's final values are written out.  Reads ein, writes eout (a
+// different buffer), so neighbouring blocks never see half-updated values; the
+// arithmetic per cell is exactly that of k_gsrb_half, so the result equals two
+// half sweeps bit for bit.
+// MODE bits:
+//   kSweepZero    ein is identically zero (first sweep after the correction is
+//                 reset), so it is not read;
+//   kSweepProlong the input is ein + (coarse correction of the parent cell), the
+//                 piecewise-constant prolongation of k_prolong done on load;
+//   kSweepUpdate  the result is not stored: phi += result (level 0, FP64 phi at
+//                 sg.foff), the update of k_update0 done on store.
 // Needs nx % TX == 0, ny % TY == 0 and even nx, ny, nz.  HBM bytes per cell:
-// e read + r read + e write (3*sizeof(T)), halo re-reads served by L2.
-template <typename T, int TX, int TY, bool ZERO_IN>
+// e read + r read + e write (3*sizeof(T)); halo re-reads are served by L2.
+constexpr int kSweepZero = 1, kSweepProlong = 2, kSweepUpdate = 4;
+
+template <typename T, int TX, int TY, int MODE>
 __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, int nseg, const int* active,
                                                           const T* __restrict__ ein, T* __restrict__ eout,
-                                                          const T* __restrict__ r) {
+                                                          const T* __restrict__ r, const T* ec0, const T* ec1,
+                                                          double* phi) {
+    constexpr bool ZERO_IN = (MODE & kSweepZero) != 0;
+    constexpr bool PRO = (MODE & kSweepProlong) != 0;
+    constexpr bool UPD = (MODE & kSweepUpdate) != 0;
     constexpr int NT = TX * TY / 2;
     constexpr int EX = TX + 4, EY = TY + 4;   // e ring planes: halo 2
     constexpr int RX = TX + 2, RY = TY + 2;   // r planes and red-update region: halo 1
@@ -113,6 +125,9 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
     const T* Ein = ein + sg.off;
     T* Eout = eout + sg.off;
     const T* R = r + sg.off;
+    const T* Ec = (sg.ebuf ? ec1 : ec0) + sg.coff;
+    const int cnx = nx >> 1, cny = ny >> 1;
+    double* Ph = phi + sg.foff;
     const int tid = threadIdx.x;
 
     // Planes are fetched into registers one iteration ahead of their use, so the
@@ -129,7 +144,13 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
                 int gx = x0 + lx - 2, gy = y0 + ly - 2;
                 gx = gx < 0 ? gx + nx : (gx >= nx ? gx - nx : gx);
                 gy = gy < 0 ? gy + ny : (gy >= ny ? gy - ny : gy);
-                pe[u] = ZERO_IN ? T(0) : Ein[cidx(gx, gy, kw, nx, ny)];
+                if (ZERO_IN) {
+                    pe[u] = T(0);
+                } else {
+                    T v = Ein[cidx(gx, gy, kw, nx, ny)];
+                    if (PRO) v = v + Ec[cidx(gx >> 1, gy >> 1, kw >> 1, cnx, cny)];
+                    pe[u] = v;
+                }
             }
         }
     };
@@ -211,9 +232,15 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
                         - dhz * (se[sm][ey][ex] + se[sp][ey][ex]);
             const T eb = ec + res * inv_gamma;
             const T er = se[sc][ey][lx + par + 2];     // the red cell, updated this sweep
-            T* o = Eout + cidx(x0 + lx, gy, wrapz(kb), nx, ny);
-            if (par == 0) { o[0] = er; o[1] = eb; }
-            else          { o[0] = eb; o[1] = er; }
+            const T e0 = par == 0 ? er : eb, e1 = par == 0 ? eb : er;
+            const size_t o = cidx(x0 + lx, gy, wrapz(kb), nx, ny);
+            if (UPD) {
+                Ph[o] = Ph[o] + (double)e0;
+                Ph[o + 1] = Ph[o + 1] + (double)e1;
+            } else {
+                Eout[o] = e0;
+                Eout[o + 1] = e1;
+            }
         }
         __syncthreads();
     }
@@ -301,40 +328,63 @@ __global__ void __launch_bounds__(kBlock) k_bottom(const CellSeg* segs, int nseg
     }
 }
 
-// Level-0 residual r = f - L phi (FP64), stored as T.  A box is covered by
-// sg.nblk blocks striding over its cells; each block reduces max |r| (and max |f|
-// when nf != nullptr) and folds it into the box's norm with one atomicMax on the
-// bit pattern (exact and order independent).  sg.off addresses the MG pool; phi
-// and f live in the level-0 pools at sg.coff.
+// Level-0 residual r = f - L phi (FP64), stored as T.  Threads own (i, j)
+// columns (64 x 4 per block) and march up a chunk of zc planes keeping
+// phi(k-1), phi(k), phi(k+1) in registers; in-plane neighbours come from cache.
+// The arithmetic is that of lap7.  Each block folds max |r| (and max |f| when nf
+// != nullptr) into one of kNormSlots per-box slots with atomicMax on the bit
+// pattern (exact and order independent); k_check reduces the slots.
+constexpr int kResTX = 64, kResTY = 4;
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_residual0(const CellSeg* segs, int nseg, const int* active,
-                                                      const double* __restrict__ f,
-                                                      const double* __restrict__ phi, T* __restrict__ r,
-                                                      unsigned long long* nr, unsigned long long* nf) {
+__global__ void __launch_bounds__(kResTX * kResTY) k_residual0(const CellSeg* segs, int nseg, const int* active,
+                                                              const double* __restrict__ f,
+                                                              const double* __restrict__ phi, T* __restrict__ r,
+                                                              unsigned long long* nr, unsigned long long* nf) {
     const int bid = xcd_block(blockIdx.x, gridDim.x);
     const int s = find_seg(segs, nseg, bid);
     const CellSeg& sg = segs[s];
     if (!active[sg.box]) return;
     const int nx = sg.n[0], ny = sg.n[1], nz = sg.n[2];
-    const long long nc = (long long)nx * ny * nz;
+    const int nbx = (nx + kResTX - 1) / kResTX, nby = (ny + kResTY - 1) / kResTY;
+    const int b = bid - sg.blk0;
+    const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+    const int i = bx * kResTX + (threadIdx.x & (kResTX - 1));
+    const int j = by * kResTY + (threadIdx.x / kResTX);
+    const int z0 = bz * sg.zchunk, z1 = min(nz, z0 + sg.zchunk);
     const double* P = phi + sg.coff;
+    const double* F = f + sg.coff;
+    T* Rr = r + sg.off;
+    const double dhx = sg.dh[0], dhy = sg.dh[1], dhz = sg.dh[2];
     double ar = 0.0, af = 0.0;
-    for (long long q = (long long)(bid - sg.blk0) * kBlock + threadIdx.x; q < nc;
-         q += (long long)sg.nblk * kBlock) {
-        const int i = (int)(q % nx);
-        const long long t = q / nx;
-        const int j = (int)(t % ny), k = (int)(t / ny);
-        const double fv = f[sg.coff + q];
-        const double v = fv - lap7(P, i, j, k, nx, ny, nz, sg.dh[0], sg.dh[1], sg.dh[2]);
-        r[sg.off + q] = T(v);
-        ar = fmax(ar, fabs(v));
-        af = fmax(af, fabs(fv));
+    if (i < nx && j < ny) {
+        const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
+        const int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
+        const size_t plane = (size_t)nx * ny;
+        const size_t col = (size_t)j * nx + i;
+        const size_t oxm = (size_t)j * nx + im, oxp = (size_t)j * nx + ip;
+        const size_t oym = (size_t)jm * nx + i, oyp = (size_t)jp * nx + i;
+        double pm = P[(size_t)(z0 == 0 ? nz - 1 : z0 - 1) * plane + col];
+        double pc = P[(size_t)z0 * plane + col];
+        for (int k = z0; k < z1; ++k) {
+            const size_t zo = (size_t)k * plane;
+            const double pp = P[(size_t)(k == nz - 1 ? 0 : k + 1) * plane + col];
+            const double lx = dhx * (P[zo + oxm] - 2.0 * pc + P[zo + oxp]);
+            const double ly = dhy * (P[zo + oym] - 2.0 * pc + P[zo + oyp]);
+            const double lz = dhz * (pm - 2.0 * pc + pp);
+            const double fv = F[zo + col];
+            const double v = fv - (lx + ly + lz);
+            Rr[zo + col] = T(v);
+            ar = fmax(ar, fabs(v));
+            af = fmax(af, fabs(fv));
+            pm = pc;
+            pc = pp;
+        }
     }
     for (int o = kWave / 2; o > 0; o >>= 1) {
         ar = fmax(ar, __shfl_xor(ar, o));
         af = fmax(af, __shfl_xor(af, o));
     }
-    __shared__ double wr[kBlock / kWave], wf[kBlock / kWave];
+    __shared__ double wr[kResTX * kResTY / kWave], wf[kResTX * kResTY / kWave];
     if ((threadIdx.x & (kWave - 1)) == 0) {
         wr[threadIdx.x / kWave] = ar;
         wf[threadIdx.x / kWave] = af;
@@ -342,12 +392,13 @@ __global__ void __launch_bounds__(kBlock) k_residual0(const CellSeg* segs, int n
     __syncthreads();
     if (threadIdx.x == 0) {
         double mr = wr[0], mf = wf[0];
-        for (int w = 1; w < kBlock / kWave; ++w) {
+        for (int w = 1; w < kResTX * kResTY / kWave; ++w) {
             mr = fmax(mr, wr[w]);
             mf = fmax(mf, wf[w]);
         }
-        atomicMax(&nr[sg.box], dbits(mr));
-        if (nf) atomicMax(&nf[sg.box], dbits(mf));
+        const int slot = sg.box * kNormSlots + (bid & (kNormSlots - 1));
+        atomicMax(&nr[slot], dbits(mr));
+        if (nf) atomicMax(&nf[slot], dbits(mf));
     }
 }
 
@@ -375,9 +426,16 @@ __global__ void k_check(int nbox, int init, int fixed_iters, int max_iter, doubl
                         unsigned long long* nr, unsigned long long* nf, double* state) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nbox) return;
+    unsigned long long br = 0ull, bf = 0ull;
+    for (int q = 0; q < kNormSlots; ++q) {
+        br = max(br, nr[b * kNormSlots + q]);
+        bf = max(bf, nf[b * kNormSlots + q]);
+        nr[b * kNormSlots + q] = 0ull;
+        nf[b * kNormSlots + q] = 0ull;
+    }
     if (init) {
-        const double r0 = __longlong_as_double((long long)nr[b]);
-        const double fn = __longlong_as_double((long long)nf[b]);
+        const double r0 = __longlong_as_double((long long)br);
+        const double fn = __longlong_as_double((long long)bf);
         const double mx = r0 > fn ? r0 : fn;
         const double rt = rel_tol > 1e-16 ? rel_tol : 1e-16;
         double target = rt * mx;
@@ -390,7 +448,7 @@ __global__ void k_check(int nbox, int init, int fixed_iters, int max_iter, doubl
         failed[b] = 0;
         active[b] = fixed_iters > 0 ? 1 : (r0 > target ? 1 : 0);
     } else if (active[b]) {
-        const double rn = __longlong_as_double((long long)nr[b]);
+        const double rn = __longlong_as_double((long long)br);
         const int it = iters[b] + 1;
         iters[b] = it;
         state[4 * b + 2] = rn;
@@ -402,8 +460,6 @@ __global__ void k_check(int nbox, int init, int fixed_iters, int max_iter, doubl
             failed[b] = (more && it >= max_iter) ? 1 : 0;
         }
     }
-    nr[b] = 0ull;
-    nf[b] = 0ull;
 }
 
 }  // namespace nyxpm

@@ -128,6 +128,7 @@ struct nyxpm_ctx {
     bool untiled_kick = false;  // force the thread-per-particle kick (for comparisons)
     int sweep_zchunk = 0;       // fixed z-chunk of the fused sweep (0: chosen per level)
     int force_tx[2] = {0, 0}, force_ty[2] = {8, 8};  // fixed sweep shape per precision (0: chosen)
+    bool fuse_update = true;    // phi += e in the last finest-level sweep
     hipStream_t stream = nullptr;
     bool own_stream = false;
     std::vector<Box> boxes;
@@ -183,12 +184,15 @@ struct nyxpm_ctx {
     // and depth; tx == 0 means separate half sweeps at that depth
     struct SweepCfg {
         int tx = 0, ty = 8;
+        int zc = 0;        // z-chunk chosen by the autotuner (0: occupancy heuristic)
+        double us = 0.0;   // autotuned finest-launch time (0: not tuned)
         Table t;
     };
     std::vector<SweepCfg> sw[2];
     std::vector<int> pre_buf;    // buffer holding e after pre-smoothing, per depth
     std::vector<int> fin_buf;    // buffer holding e after post-smoothing, per depth
     Table t_bottom, t_l0, t_res;  // t_res: level 0, at most kReduceBlocks blocks per box
+    Table t_resz;                 // level 0, z-marching residual columns
     BoxSeg* pseg = nullptr;
     int pseg_n = 0, pseg_blocks = 0;
     BoxSeg* tseg = nullptr;
@@ -373,6 +377,27 @@ int setup_boxes(nyxpm_ctx* c, int nbox, const nyxpm_geom* geoms) {
     return NYXPM_OK;
 }
 
+// Blocks and z-chunk of one box's fused-sweep piece for the current shape of
+// (precision pr, depth d): the autotuned z-chunk, else about four rounds of
+// resident blocks (LDS- and thread-limited residency per CU), chunks of at least
+// 8 planes.
+void sweep_seg_shape(nyxpm_ctx* c, int pr, int d, const Level& L, CellSeg& w) {
+    const int tx = c->sw[pr][d].tx, ty = c->sw[pr][d].ty;
+    const long long cols = (long long)(L.n[0] / tx) * (L.n[1] / ty);
+    const int sz = pr == 0 ? 8 : 4;
+    const long long lds = (4LL * (ty + 4) * (tx + 4) + 2LL * (ty + 2) * (tx + 2)) * sz;
+    const long long per_cu = std::max(1LL, std::min(2048LL / (tx * ty / 2), (160LL * 1024) / lds));
+    const long long resident = 256LL * per_cu;
+    long long nzc = (4 * resident + cols - 1) / std::max(1LL, cols);
+    int zfix = c->sw[pr][d].zc > 0 ? c->sw[pr][d].zc : c->sweep_zchunk;
+    if (zfix > 0) nzc = (L.n[2] + zfix - 1) / zfix;
+    else nzc = std::max(1LL, std::min(nzc, (long long)std::max(1, L.n[2] / 8)));
+    nzc = std::max(1LL, std::min(nzc, (long long)L.n[2]));
+    w.zchunk = (int)((L.n[2] + nzc - 1) / nzc);
+    nzc = (L.n[2] + w.zchunk - 1) / w.zchunk;
+    w.nblk = (int)(cols * nzc);
+}
+
 int build_tables(nyxpm_ctx* c) {
     std::vector<CellSeg>& S = c->seg_host;
     S.clear();
@@ -390,6 +415,7 @@ int build_tables(nyxpm_ctx* c) {
         s.inv_gamma = L.inv_gamma;
         s.off = L.off;
         s.coff = (l + 1 < B.nlev) ? B.lev[l + 1].off : 0;
+        s.foff = B.cell_off;
         s.nblk = nblocks(per);
         return s;
     };
@@ -463,6 +489,11 @@ int build_tables(nyxpm_ctx* c) {
             c->fin_buf[d] = cur;
         }
     }
+    // a box solved entirely by the bottom solve has no finest-level sweep to carry
+    // the phi update
+    c->fuse_update = true;
+    for (int b = 0; b < nbox; ++b)
+        if (c->boxes[b].nlev == 1) c->fuse_update = false;
     auto coarse_buf = [&](const Box& B, int d) {  // buffer of level d+1's result
         return (d + 1 == B.nlev - 1) ? 0 : c->fin_buf[d + 1];
     };
@@ -478,22 +509,10 @@ int build_tables(nyxpm_ctx* c) {
             vh.push_back(mk(b, d, L.nc / 2));
             vc.push_back(mk(b, d, L.nc / 8));
             for (int pr = 0; pr < 2; ++pr) {
-                const int tx = c->sw[pr][d].tx, ty = c->sw[pr][d].ty;
-                if (!tx) continue;
-                const long long cols = (long long)(L.n[0] / tx) * (L.n[1] / ty);
-                // about four rounds of resident blocks (LDS- and thread-limited
-                // residency per CU), chunks of at least 8 planes
-                const int sz = pr == 0 ? 8 : 4;
-                const long long lds = (4LL * (ty + 4) * (tx + 4) + 2LL * (ty + 2) * (tx + 2)) * sz;
-                const long long per_cu = std::max(1LL, std::min(2048LL / (tx * ty / 2), (160LL * 1024) / lds));
-                const long long resident = 256LL * per_cu;
-                long long nzc = (4 * resident + cols - 1) / std::max(1LL, cols);
-                if (c->sweep_zchunk > 0) nzc = (L.n[2] + c->sweep_zchunk - 1) / c->sweep_zchunk;
-                nzc = std::max(1LL, std::min(nzc, (long long)std::max(1, L.n[2] / 8)));
+                if (!c->sw[pr][d].tx) continue;
                 CellSeg w = mk(b, d, 1);
-                w.zchunk = (int)((L.n[2] + nzc - 1) / nzc);
-                nzc = (L.n[2] + w.zchunk - 1) / w.zchunk;
-                w.nblk = (int)(cols * nzc);
+                w.ebuf = coarse_buf(B, d);  // for the prolongation done on load
+                sweep_seg_shape(c, pr, d, L, w);
                 vs[pr].push_back(w);
             }
         }
@@ -522,14 +541,23 @@ int build_tables(nyxpm_ctx* c) {
             v0.push_back(s);
         }
         c->t_l0 = add_table(v0);
-        std::vector<CellSeg> vr;
+        std::vector<CellSeg> vr, vz;
         for (int b = 0; b < nbox; ++b) {
             CellSeg s = mk(b, 0, c->boxes[b].lev[0].nc);
             s.coff = c->boxes[b].cell_off;
             s.nblk = std::min(s.nblk, kReduceBlocks);
             vr.push_back(s);
+            // z-marching residual: 64 x 4 columns, chunks of up to 32 planes
+            const Level& L = c->boxes[b].lev[0];
+            CellSeg z = mk(b, 0, 1);
+            z.coff = c->boxes[b].cell_off;
+            const int nbx = (L.n[0] + kResTX - 1) / kResTX, nby = (L.n[1] + kResTY - 1) / kResTY;
+            z.zchunk = std::min(32, L.n[2]);
+            z.nblk = nbx * nby * ((L.n[2] + z.zchunk - 1) / z.zchunk);
+            vz.push_back(z);
         }
         c->t_res = add_table(vr);
+        c->t_resz = add_table(vz);
     }
     int rc = dalloc(c, &c->seg_pool, S.size());
     if (rc) return rc;
@@ -630,8 +658,8 @@ int alloc_fields(nyxpm_ctx* c) {
     if ((rc = dalloc(c, &c->failed, nbox))) return rc;
     if ((rc = dalloc(c, &c->err_flag, 1))) return rc;
     if ((rc = dalloc(c, &c->nitems, 1))) return rc;
-    if ((rc = dalloc(c, &c->nr, nbox))) return rc;
-    if ((rc = dalloc(c, &c->nf, nbox))) return rc;
+    if ((rc = dalloc(c, &c->nr, (size_t)nbox * kNormSlots))) return rc;
+    if ((rc = dalloc(c, &c->nf, (size_t)nbox * kNormSlots))) return rc;
     if ((rc = dalloc(c, &c->state, 4 * nbox))) return rc;
     if ((rc = dalloc(c, &c->solv, nbox))) return rc;
     if ((rc = dalloc(c, &c->dv_size, 2 * nbox))) return rc;
@@ -639,8 +667,8 @@ int alloc_fields(nyxpm_ctx* c) {
     HIPCK(c, hipHostMalloc((void**)&c->h_flags, (3 * nbox + 1) * sizeof(int), hipHostMallocDefault));
     HIPCK(c, hipHostMalloc((void**)&c->h_state, 4 * nbox * sizeof(double), hipHostMallocDefault));
     HIPCK(c, hipMemset(c->phi, 0, c->ncell_total * sizeof(double)));
-    HIPCK(c, hipMemset(c->nr, 0, nbox * sizeof(unsigned long long)));
-    HIPCK(c, hipMemset(c->nf, 0, nbox * sizeof(unsigned long long)));
+    HIPCK(c, hipMemset(c->nr, 0, (size_t)nbox * kNormSlots * sizeof(unsigned long long)));
+    HIPCK(c, hipMemset(c->nf, 0, (size_t)nbox * kNormSlots * sizeof(unsigned long long)));
     HIPCK(c, hipMemset(c->err_flag, 0, sizeof(int)));
     HIPCK(c, hipMemset(c->counts, 0, c->ntiles_total * sizeof(int)));
     // bottom Green's functions
@@ -807,15 +835,15 @@ struct Pools<float> {
     static float* G(nyxpm_ctx* c) { return c->G32; }
 };
 
-template <typename T, bool ZERO>
-void launch_sweep(nyxpm_ctx* c, int d, const T* ein, T* eout, const T* r) {
+template <typename T, int MODE>
+void launch_sweep(nyxpm_ctx* c, int d, const T* ein, T* eout, const T* r, const T* ec0, const T* ec1) {
     const auto& cfg = c->sw[sizeof(T) == 8 ? 0 : 1][d];
     const auto& t = cfg.t;
     const int key = cfg.tx * 100 + cfg.ty;
 #define NYXPM_SWEEP(TX, TY)                                                                                  \
     case TX * 100 + TY:                                                                                      \
-        hipLaunchKernelGGL((k_rb_sweep<T, TX, TY, ZERO>), dim3(t.blocks), dim3(TX * TY / 2), 0, c->stream, \
-                           c->segs(t), t.count, c->active, ein, eout, r);                                   \
+        hipLaunchKernelGGL((k_rb_sweep<T, TX, TY, MODE>), dim3(t.blocks), dim3(TX * TY / 2), 0, c->stream, \
+                           c->segs(t), t.count, c->active, ein, eout, r, ec0, ec1, c->phi);                 \
         break;
     switch (key) {
         NYXPM_SWEEP(128, 8)
@@ -829,25 +857,29 @@ void launch_sweep(nyxpm_ctx* c, int d, const T* ein, T* eout, const T* r) {
 #undef NYXPM_SWEEP
 }
 
+// One V-cycle on every active box.  Returns in *updated whether phi += e has
+// already been applied (fused into the last finest-level sweep).
 template <typename T>
-int vcycle(nyxpm_ctx* c) {
+int vcycle(nyxpm_ctx* c, bool* updated) {
     T* r = Pools<T>::r(c);
     T* eb[2] = {Pools<T>::e(c), Pools<T>::e2(c)};
+    const int pr = sizeof(T) == 8 ? 0 : 1;
     const int D = c->dmax;
+    *updated = false;
     for (int d = 0; d + 1 < D; ++d) {
         auto& tf = c->t_full[d];
         auto& th = c->t_half[d];
         auto& tc = c->t_coarse[d];
         if (tf.count == 0) continue;
         int cur = 0;
-        if (c->sw[sizeof(T) == 8 ? 0 : 1][d].tx) {
+        if (c->sw[pr][d].tx) {
             {
                 Timed t(c, lvl_name("sweep0", d));
-                launch_sweep<T, true>(c, d, eb[0], eb[0], r);
+                launch_sweep<T, kSweepZero>(c, d, eb[0], eb[0], r, eb[0], eb[1]);
             }
             for (int q = 1; q < c->nu1; ++q) {
                 Timed t(c, lvl_name("sweep", d));
-                launch_sweep<T, false>(c, d, eb[cur], eb[1 - cur], r);
+                launch_sweep<T, 0>(c, d, eb[cur], eb[1 - cur], r, eb[0], eb[1]);
                 cur ^= 1;
             }
         } else {
@@ -879,24 +911,43 @@ int vcycle(nyxpm_ctx* c) {
         auto& th = c->t_half[d];
         if (tf.count == 0) continue;
         int cur = c->pre_buf[d];
-        {
-            Timed t(c, lvl_name("prolong", d));
-            hipLaunchKernelGGL(k_prolong<T>, dim3(tf.blocks), dim3(kBlock), 0, c->stream, c->segs(tf), tf.count,
-                               c->active, eb[cur], eb[0], eb[1]);
-        }
-        if (c->sw[sizeof(T) == 8 ? 0 : 1][d].tx) {
+        if (c->sw[pr][d].tx && c->nu2 > 0) {
+            // prolongation fused into the first sweep; phi += e into the last one
+            // on the finest level
             for (int q = 0; q < c->nu2; ++q) {
-                Timed t(c, lvl_name("sweep", d));
-                launch_sweep<T, false>(c, d, eb[cur], eb[1 - cur], r);
+                const bool first = q == 0, last = q + 1 == c->nu2, upd = last && d == 0 && c->fuse_update;
+                Timed t(c, lvl_name(first ? "sweep_p" : (upd ? "sweep_u" : "sweep"), d));
+                if (first && upd)
+                    launch_sweep<T, kSweepProlong | kSweepUpdate>(c, d, eb[cur], eb[1 - cur], r, eb[0], eb[1]);
+                else if (first)
+                    launch_sweep<T, kSweepProlong>(c, d, eb[cur], eb[1 - cur], r, eb[0], eb[1]);
+                else if (upd)
+                    launch_sweep<T, kSweepUpdate>(c, d, eb[cur], eb[1 - cur], r, eb[0], eb[1]);
+                else
+                    launch_sweep<T, 0>(c, d, eb[cur], eb[1 - cur], r, eb[0], eb[1]);
                 cur ^= 1;
+                if (upd) *updated = true;
             }
         } else {
-            for (int q = 0; q < c->nu2; ++q)
-                for (int col = 0; col < 2; ++col) {
-                    Timed t(c, lvl_name("smooth", d));
-                    hipLaunchKernelGGL(k_gsrb_half<T>, dim3(th.blocks), dim3(kBlock), 0, c->stream, c->segs(th),
-                                       th.count, c->active, eb[0], r, col);
+            {
+                Timed t(c, lvl_name("prolong", d));
+                hipLaunchKernelGGL(k_prolong<T>, dim3(tf.blocks), dim3(kBlock), 0, c->stream, c->segs(tf),
+                                   tf.count, c->active, eb[cur], eb[0], eb[1]);
+            }
+            if (c->sw[pr][d].tx) {
+                for (int q = 0; q < c->nu2; ++q) {
+                    Timed t(c, lvl_name("sweep", d));
+                    launch_sweep<T, 0>(c, d, eb[cur], eb[1 - cur], r, eb[0], eb[1]);
+                    cur ^= 1;
                 }
+            } else {
+                for (int q = 0; q < c->nu2; ++q)
+                    for (int col = 0; col < 2; ++col) {
+                        Timed t(c, lvl_name("smooth", d));
+                        hipLaunchKernelGGL(k_gsrb_half<T>, dim3(th.blocks), dim3(kBlock), 0, c->stream,
+                                           c->segs(th), th.count, c->active, eb[0], r, col);
+                    }
+            }
         }
     }
     LAUNCHCK(c);
@@ -906,8 +957,9 @@ int vcycle(nyxpm_ctx* c) {
 template <typename T>
 int residual(nyxpm_ctx* c, bool with_f) {
     Timed t(c, "residual");
-    hipLaunchKernelGGL(k_residual0<T>, dim3(c->t_res.blocks), dim3(kBlock), 0, c->stream, c->segs(c->t_res),
-                       c->t_res.count, c->active, c->f, c->phi, Pools<T>::r(c), c->nr, with_f ? c->nf : nullptr);
+    hipLaunchKernelGGL(k_residual0<T>, dim3(c->t_resz.blocks), dim3(kResTX * kResTY), 0, c->stream,
+                       c->segs(c->t_resz), c->t_resz.count, c->active, c->f, c->phi, Pools<T>::r(c), c->nr,
+                       with_f ? c->nf : nullptr);
     return NYXPM_OK;
 }
 
@@ -949,8 +1001,9 @@ int solve_impl(nyxpm_ctx* c, double rel, double abs_tol, int fixed_iters, int gr
     if ((rc = check(c, true, fixed_iters, rel, abs_tol))) return rc;
     if (graph_cycles > 0) {
         for (int it = 0; it < graph_cycles; ++it) {
-            if ((rc = vcycle<T>(c))) return rc;
-            if ((rc = update<T>(c))) return rc;
+            bool upd;
+            if ((rc = vcycle<T>(c, &upd))) return rc;
+            if (!upd && (rc = update<T>(c))) return rc;
             if ((rc = residual<T>(c, false))) return rc;
             if ((rc = check(c, false, fixed_iters, rel, abs_tol))) return rc;
         }
@@ -961,8 +1014,9 @@ int solve_impl(nyxpm_ctx* c, double rel, double abs_tol, int fixed_iters, int gr
         bool any;
         if ((rc = read_flags(c, &any))) return rc;
         if (!any) break;
-        if ((rc = vcycle<T>(c))) return rc;
-        if ((rc = update<T>(c))) return rc;
+        bool upd;
+        if ((rc = vcycle<T>(c, &upd))) return rc;
+        if (!upd && (rc = update<T>(c))) return rc;
         if ((rc = residual<T>(c, false))) return rc;
         if ((rc = check(c, false, fixed_iters, rel, abs_tol))) return rc;
     }
@@ -1576,6 +1630,109 @@ int nyxpm_pm_steps(nyxpm_ctx* c, int nsteps, double dt, double a_old, double a_n
     c->keys_valid = false;
     c->density_valid = false;
     return err_check(c);
+}
+
+// Re-derive the fused-sweep pieces of (pr, d) in place (same boxes, new shape or
+// z-chunk) and upload them.
+static int rebuild_sweep_table(nyxpm_ctx* c, int pr, int d) {
+    auto& t = c->sw[pr][d].t;
+    int blk = 0;
+    for (int q = 0; q < t.count; ++q) {
+        CellSeg& w = c->seg_host[t.first + q];
+        const Level& L = c->boxes[w.box].lev[d];
+        sweep_seg_shape(c, pr, d, L, w);
+        w.blk0 = blk;
+        blk += w.nblk;
+    }
+    t.blocks = blk;
+    HIPCK(c, hipMemcpy(c->seg_pool + t.first, c->seg_host.data() + t.first, t.count * sizeof(CellSeg),
+                       hipMemcpyHostToDevice));
+    return NYXPM_OK;
+}
+
+int nyxpm_autotune(nyxpm_ctx* c, int precision_mask, char* report, int report_len) {
+    if (!c) return NYXPM_ERR_ARG;
+    int rc;
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    if ((precision_mask & 2) && (rc = ensure_mixed_pools(c))) return rc;
+    const int nbox = (int)c->boxes.size();
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(kBlock), 0, c->stream, c->active, (long long)nbox, 1);
+    static const int shapes[][2] = {{128, 8}, {64, 16}, {64, 8}, {32, 8}, {16, 8}};
+    static const int zcs[] = {8, 16, 32, 64, 128, 256};
+    hipEvent_t e0, e1;
+    HIPCK(c, hipEventCreate(&e0));
+    HIPCK(c, hipEventCreate(&e1));
+    std::string rep = "{";
+    bool first_entry = true;
+    for (int pr = 0; pr < 2; ++pr) {
+        if (!(precision_mask & (1 << pr))) continue;
+        for (int d = 0; d + 1 < c->dmax; ++d) {
+            auto& cfg = c->sw[pr][d];
+            if (!cfg.tx || cfg.t.count == 0) continue;
+            const int tx0 = cfg.tx, ty0 = cfg.ty, zc0 = cfg.zc;
+            double best = 1e30;
+            int btx = tx0, bty = ty0, bzc = zc0;
+            int maxnz = 0;
+            for (int q = 0; q < cfg.t.count; ++q)
+                maxnz = std::max(maxnz, c->boxes[c->seg_host[cfg.t.first + q].box].lev[d].n[2]);
+            for (auto& sh : shapes) {
+                bool ok = true;
+                for (int q = 0; q < cfg.t.count && ok; ++q) {
+                    const Level& L = c->boxes[c->seg_host[cfg.t.first + q].box].lev[d];
+                    if (L.n[0] % sh[0] || L.n[1] % sh[1]) ok = false;
+                }
+                if (!ok) continue;
+                for (int zc : zcs) {
+                    if (zc > maxnz) continue;
+                    cfg.tx = sh[0];
+                    cfg.ty = sh[1];
+                    cfg.zc = zc;
+                    if ((rc = rebuild_sweep_table(c, pr, d))) return rc;
+                    float ms = 0.f;
+                    for (int rep_i = 0; rep_i < 2; ++rep_i) {  // warm-up, then a timed batch of 4
+                        HIPCK(c, hipEventRecord(e0, c->stream));
+                        for (int k = 0; k < (rep_i ? 4 : 1); ++k) {
+                            if (pr == 0)
+                                launch_sweep<double, 0>(c, d, c->e64, c->e64b, c->r64, c->e64, c->e64b);
+                            else
+                                launch_sweep<float, 0>(c, d, c->e32, c->e32b, c->r32, c->e32, c->e32b);
+                        }
+                        HIPCK(c, hipEventRecord(e1, c->stream));
+                        HIPCK(c, hipEventSynchronize(e1));
+                        HIPCK(c, hipEventElapsedTime(&ms, e0, e1));
+                    }
+                    LAUNCHCK(c);
+                    const double us = ms * 1e3 / 4.0;
+                    if (us < best) {
+                        best = us;
+                        btx = sh[0];
+                        bty = sh[1];
+                        bzc = zc;
+                    }
+                }
+            }
+            cfg.tx = btx;
+            cfg.ty = bty;
+            cfg.zc = bzc;
+            cfg.us = best;
+            if ((rc = rebuild_sweep_table(c, pr, d))) return rc;
+            char buf[160];
+            snprintf(buf, sizeof(buf), "%s\"%s.d%d\": {\"shape\": [%d, %d], \"zchunk\": %d, \"us\": %.2f}",
+                     first_entry ? "" : ", ", pr == 0 ? "fp64" : "fp32", d, btx, bty, bzc, best);
+            rep += buf;
+            first_entry = false;
+        }
+    }
+    rep += "}";
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
+    if (report && report_len > 0) {
+        std::strncpy(report, rep.c_str(), (size_t)report_len - 1);
+        report[report_len - 1] = 0;
+    }
+    return NYXPM_OK;
 }
 
 int nyxpm_timing_enable(nyxpm_ctx* c, int on) {

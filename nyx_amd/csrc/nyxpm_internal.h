@@ -21,6 +21,7 @@ constexpr int kBlock = 256;        // threads per block of the grid kernels
 constexpr int kMaxLevels = 24;
 constexpr int kBottomMaxCells = 4096;
 constexpr int kReduceBlocks = 2048;  // blocks per box of the reducing level-0 kernels
+constexpr int kNormSlots = 64;       // atomicMax slots per box for residual norms
 
 // One periodic box as the particle kernels see it.
 struct BoxDev {
@@ -52,6 +53,7 @@ struct CellSeg {
     long long off;         // offset of this level in its pool
     long long coff;        // offset of the next coarser level (restrict/prolong),
                            // or of the bottom Green's function (bottom solve)
+    long long foff;        // offset of the box's level-0 fields (phi, f)
     int zchunk;            // z-planes per block of the fused sweep
     int ebuf;              // which correction buffer (0: e, 1: e2) holds the coarse
                            // level's result (prolong) / the level's result (update)

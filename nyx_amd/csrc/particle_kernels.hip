@@ -545,7 +545,7 @@ __device__ __forceinline__ void cic_grad_lines(LD ld, double xint, double yint, 
 // cut into chunks of at most kChunk particles (empty tiles give none).  Slots are
 // reserved with one atomic per wave, so the item order varies between runs; every
 // item is processed independently, so results do not.
-constexpr int kChunk = 1024;
+constexpr int kChunk = 4 * kBlock;  // particles per work item (fewer items: less phi staging)
 
 struct TileItem {
     long long p0;   // first particle
@@ -609,6 +609,15 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
     const int bx = min(B.tile[0], nx - o0), by = min(B.tile[1], ny - o1), bz = min(B.tile[2], nz - o2);
     const int SX = bx + 3, SY = by + 3, SZ = bz + 3;
     const double* P = phi + B.cell_off;
+    // the first particle of this thread is loaded before the staging of phi so the
+    // two latencies overlap; later ones are prefetched one iteration ahead
+    long long p = it.p0 + threadIdx.x;
+    bool valid = (int)threadIdx.x < it.np;
+    double px = 0.0, py = 0.0, pz = 0.0, qx = 0.0, qy = 0.0, qz = 0.0;
+    if (valid) {
+        px = x[p]; py = y[p]; pz = z[p];
+        qx = vx[p]; qy = vy[p]; qz = vz[p];
+    }
     for (int idx = threadIdx.x; idx < SX * SY * SZ; idx += kBlock) {
         const int lx = idx % SX, ly = (idx / SX) % SY, lz = idx / (SX * SY);
         const int gx = wrap1(o0 - 2 + lx, nx), gy = wrap1(o1 - 2 + ly, ny), gz = wrap1(o2 - 2 + lz, nz);
@@ -620,8 +629,14 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
     const double dt_a_cur_inv = dt * a_cur_inv;
     const double dxi0 = B.dxi[0], dxi1 = B.dxi[1], dxi2 = B.dxi[2];
     for (int q = threadIdx.x; q < it.np; q += kBlock) {
-        const long long p = it.p0 + q;
-        double px = x[p], py = y[p], pz = z[p];
+        // prefetch the next particle of this thread
+        const long long pn = p + kBlock;
+        const bool vn = q + kBlock < it.np;
+        double nxp = 0.0, nyp = 0.0, nzp = 0.0, nqx = 0.0, nqy = 0.0, nqz = 0.0;
+        if (vn) {
+            nxp = x[pn]; nyp = y[pn]; nzp = z[pn];
+            nqx = vx[pn]; nqy = vy[pn]; nqz = vz[pn];
+        }
         double xint, yint, zint;
         const int i = cic_index(px, B.plo[0], B.dxi[0], &xint);
         const int j = cic_index(py, B.plo[1], B.dxi[1], &yint);
@@ -640,11 +655,11 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
             };
             cic_grad_lines(ld, xint, yint, zint, dxi0, dxi1, dxi2, val);
         }
-        double ux = a_prev * vx[p] + half_dt * val[0];
+        double ux = a_prev * qx + half_dt * val[0];
         ux = ux * a_cur_inv;
-        double uy = a_prev * vy[p] + half_dt * val[1];
+        double uy = a_prev * qy + half_dt * val[1];
         uy = uy * a_cur_inv;
-        double uz = a_prev * vz[p] + half_dt * val[2];
+        double uz = a_prev * qz + half_dt * val[2];
         uz = uz * a_cur_inv;
         vx[p] = ux;
         vy[p] = uy;
@@ -667,7 +682,11 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
             if (syp != py) y[p] = syp;
             if (szp != pz) z[p] = szp;
         }
+        p = pn;
+        px = nxp; py = nyp; pz = nzp;
+        qx = nqx; qy = nqy; qz = nqz;
     }
+    (void)valid;
 }
 
 // Periodic shift alone (nyxpm_redistribute).

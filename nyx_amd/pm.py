@@ -217,6 +217,16 @@ class PMContext:
                                        int(bool(use_graph)), ptr(it)))
         return it
 
+    # -- tuning ---------------------------------------------------------------
+    def autotune(self, fp64=True, fp32=True):
+        """Time every fused-sweep column shape and z-chunk per multigrid depth on
+        this context's levels and keep the fastest; returns the choices."""
+        import json
+        buf = ctypes.create_string_buffer(8192)
+        mask = (1 if fp64 else 0) | (2 if fp32 else 0)
+        self._ck(self.L.nyxpm_autotune(self.h, mask, buf, len(buf)))
+        return json.loads(buf.value.decode())
+
     # -- measurement --------------------------------------------------------
     def timing_enable(self, on=True):
         self._ck(self.L.nyxpm_timing_enable(self.h, int(bool(on))))

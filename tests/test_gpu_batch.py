@@ -1,0 +1,114 @@
+"""GPU: batches of independent boxes in one launch per phase, and hipGraph-
+captured steps, against solo eager runs (bit for bit) and the oracle."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(16, 16, 16), (24, 24, 24), (32, 32, 32)]
+
+
+def _inputs(nyx, n, seed, mass=False):
+    pos, vel, m = nyx.ics.grf_lattice(n[0], seed, -2.0, 0.5, 0.1)
+    if mass:
+        r = np.random.default_rng(seed)
+        mm = m * r.uniform(0.5, 1.5, len(pos[0]))
+        return pos, vel, mm, None
+    return pos, vel, None, m
+
+
+def _upload(c, box, inp):
+    pos, vel, mm, m = inp
+    c.upload(box, pos, vel, m=mm, m_const=m if m is not None else 0.0)
+
+
+def _run(nyx, geoms, inputs, steps, precision="fp64", use_graph=False):
+    with nyx.PMContext([nyx.BoxGeom(g) for g in geoms], precision=precision) as c:
+        for b, inp in enumerate(inputs):
+            _upload(c, b, inp)
+        c.pm_init(1.0, 1.0, 1e-10)
+        its = c.pm_steps(steps, 1e-3, rel_tol=1e-10, use_graph=use_graph)
+        out = [c.download(b) for b in range(len(geoms))]
+        phis = [c.get_phi(b) for b in range(len(geoms))]
+    return out, phis, its
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_equals_solo(nyx, precision):
+    inputs = [_inputs(nyx, n, 100 + i, mass=(i == 1)) for i, n in enumerate(SIZES)]
+    batch, bphi, bits = _run(nyx, SIZES, inputs, 2, precision)
+    for b, n in enumerate(SIZES):
+        solo, sphi, sits = _run(nyx, [n], [inputs[b]], 2, precision)
+        assert bits[b] == sits[0]
+        assert np.array_equal(bphi[b], sphi[0])
+        for d in range(3):
+            assert np.array_equal(batch[b][0][d], solo[0][0][d])
+            assert np.array_equal(batch[b][1][d], solo[0][1][d])
+
+
+def test_batch_vs_oracle(nyx, orc):
+    inputs = [_inputs(nyx, n, 200 + i) for i, n in enumerate(SIZES)]
+    batch, _, _ = _run(nyx, SIZES, inputs, 1)
+    for b, n in enumerate(SIZES):
+        pos, vel, _, m = inputs[b]
+        p = [a.copy() for a in pos]
+        v = [a.copy() for a in vel]
+        phi, _, _ = orc.solve_for_phi(n, (0, 0, 0), (1, 1, 1), p, m_const=m, rel_tol=1e-10)
+        orc.pm_step(n, (0, 0, 0), (1, 1, 1), p, v, phi, m_const=m, dt=1e-3, rel_tol=1e-10)
+        for d in range(3):
+            assert np.abs(batch[b][0][d] - p[d]).max() <= 1e-12 * np.abs(p[d]).max()
+            assert np.abs(batch[b][1][d] - v[d]).max() <= 1e-12 * np.abs(v[d]).max()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("steps", [1, 4, 5])
+def test_graph_replay_equals_eager(nyx, precision, steps):
+    # 4 and 5 steps replay the captured two-step graph twice, with eager work and
+    # synchronisations (convergence checks, the odd last step) in between
+    inputs = [_inputs(nyx, n, 300 + i) for i, n in enumerate(SIZES)]
+    eager, ephi, eits = _run(nyx, SIZES, inputs, steps, precision, use_graph=False)
+    graph, gphi, gits = _run(nyx, SIZES, inputs, steps, precision, use_graph=True)
+    assert list(eits) == list(gits)
+    for b in range(len(SIZES)):
+        assert np.array_equal(ephi[b], gphi[b])
+        for d in range(3):
+            assert np.array_equal(eager[b][0][d], graph[b][0][d])
+            assert np.array_equal(eager[b][1][d], graph[b][1][d])
+
+
+def test_precision_switch_in_one_context(nyx):
+    pos, vel, _, m = _inputs(nyx, (32, 32, 32), 7)
+    with nyx.PMContext(nyx.BoxGeom((32, 32, 32))) as c:
+        c.upload(0, pos, vel, m_const=m)
+        c.pm_init(1.0, 1.0, 1e-10)
+        g64 = c.gather_accel()
+        c.set_precision("mixed")
+        it, _ = c.solve_for_phi(1.0, 1.0, 1e-10, 0.0, use_guess=False)
+        gmx = c.gather_accel()
+        c.set_precision("fp64")
+        c.solve_for_phi(1.0, 1.0, 1e-10, 0.0, use_guess=False)
+        g64b = c.gather_accel()
+    assert np.array_equal(g64, g64b)
+    rel = np.sqrt(np.sum((gmx - g64) ** 2) / np.sum(g64 ** 2))
+    assert rel < 1e-6
+
+
+def test_autotune_keeps_results(nyx):
+    # the autotuner only changes launch shapes; results stay bit for bit
+    sizes = [(32, 32, 32), (64, 64, 64), (96, 96, 96)]  # fused sweeps on the two finest depths
+    inputs = [_inputs(nyx, n, 400 + i) for i, n in enumerate(sizes)]
+    outs = []
+    for tune in (False, True):
+        with nyx.PMContext([nyx.BoxGeom(g) for g in sizes], precision="mixed") as c:
+            if tune:
+                rep = c.autotune()
+                assert "fp32.d0" in rep and "fp64.d0" in rep and all(v["us"] > 0 for v in rep.values())
+            for b, inp in enumerate(inputs):
+                _upload(c, b, inp)
+            c.pm_init(1.0, 1.0, 1e-10)
+            c.pm_steps(2, 1e-3, rel_tol=1e-10, use_graph=True)
+            outs.append([c.download(b) for b in range(len(sizes))])
+    for b in range(len(sizes)):
+        for d in range(3):
+            assert np.array_equal(outs[0][b][0][d], outs[1][b][0][d])
+            assert np.array_equal(outs[0][b][1][d], outs[1][b][1][d])
