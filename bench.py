@@ -126,6 +126,7 @@ def main():
     ap.add_argument("--modes", default="mixed,fp64", help="precisions to time (comma list)")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the mixed-vs-FP64 acceleration check")
     ap.add_argument("--no-tune", action="store_true", help="skip the smoother-shape autotuner")
+    ap.add_argument("--kick-chunk", type=int, default=0, help="particles per kick work item (0: default)")
     ap.add_argument("--workload", default="clustered512",
                     choices=["clustered512", "batch48", "grf256"],
                     help="clustered512: the headline line; batch48: 48 boxes (16 each of 64^3, 96^3, "
@@ -149,7 +150,7 @@ def main():
     t = time.perf_counter()
     pos, vel, m = ics.plummer_clustered(n, ics.SEED_BASE + 2, n_spheres=max(1, int(2000 * (n / 512) ** 3)))
     log(f"  {time.perf_counter() - t:.1f} s")
-    ctx = nyx_amd.PMContext(nyx_amd.BoxGeom((n, n, n)), precision="fp64")
+    ctx = nyx_amd.PMContext(nyx_amd.BoxGeom((n, n, n)), precision="fp64", kick_chunk=a.kick_chunk)
     ctx.upload(0, pos, vel, m_const=m)
     del pos, vel
     npart = n ** 3

@@ -85,6 +85,8 @@ typedef struct nyxpm_opts {
     int sweep_tx, sweep_ty; /* fused-sweep column shape for every level that allows
                           it (128x8, 64x16, 64x8, 32x8, 16x8); 0 = chosen per level */
     int sweep_zchunk;  /* z-planes per fused-sweep block; 0 = chosen per level */
+    int kick_chunk;    /* particles per work item of the tile-staged kick (>= 256);
+                          0 = default (4096) */
 } nyxpm_opts;
 
 /* ---- lifecycle ---------------------------------------------------------- */

@@ -129,6 +129,7 @@ struct nyxpm_ctx {
     int sweep_zchunk = 0;       // fixed z-chunk of the fused sweep (0: chosen per level)
     int force_tx[2] = {0, 0}, force_ty[2] = {8, 8};  // fixed sweep shape per precision (0: chosen)
     bool fuse_update = true;    // phi += e in the last finest-level sweep
+    int kick_chunk = kChunk;    // particles per work item of the tile-staged kicks
     hipStream_t stream = nullptr;
     bool own_stream = false;
     std::vector<Box> boxes;
@@ -765,7 +766,7 @@ int phase_sort(nyxpm_ctx* c) {
         Timed t(c, "items");
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(kBlock), 0, c->stream, c->nitems, 1LL, 0);
         hipLaunchKernelGGL(k_make_items, dim3(nblocks(c->ntiles_total)), dim3(kBlock), 0, c->stream, c->tstart,
-                           c->ntiles_total, c->items, c->nitems);
+                           c->ntiles_total, c->items, c->nitems, c->kick_chunk);
     }
     LAUNCHCK(c);
     c->cur = 1 - c->cur;
@@ -1055,8 +1056,11 @@ int solve_results(nyxpm_ctx* c, int* iters, double* final_res, bool allow_active
 template <bool MOVE>
 int phase_kick(nyxpm_ctx* c, double dt, double a_prev, double a_cur) {
     PartPtrs& A = c->P[c->cur];
-    if (c->items_valid && !c->untiled_kick) {
-        const size_t lds = sizeof(double) * (c->tile[0] + 3) * (c->tile[1] + 3) * (c->tile[2] + 3);
+    // staged phi tile with a halo of 2, plus the three components of g on the
+    // (tile+1)^3 cells its particles interpolate from
+    const size_t lds = sizeof(double) * ((c->tile[0] + 3) * (c->tile[1] + 3) * (c->tile[2] + 3) +
+                                         3 * (c->tile[0] + 1) * (c->tile[1] + 1) * (c->tile[2] + 1));
+    if (c->items_valid && !c->untiled_kick && lds <= 64 * 1024) {
         Timed t(c, MOVE ? "kick_drift" : "kick");
         if (MOVE)
             hipLaunchKernelGGL(k_fill_int, dim3(nblocks(c->ntiles_total)), dim3(kBlock), 0, c->stream, c->counts,
@@ -1155,6 +1159,7 @@ int nyxpm_create_batch(nyxpm_ctx** out, int nbox, const nyxpm_geom* geoms, const
             c->force_ty[0] = c->force_ty[1] = opts->sweep_ty > 0 ? opts->sweep_ty : 8;
         }
         if (opts->sweep_zchunk > 0) c->sweep_zchunk = opts->sweep_zchunk;
+        if (opts->kick_chunk > 0) c->kick_chunk = std::max(kBlock, opts->kick_chunk);
     }
     *out = c;
     for (int d = 0; d < 3; ++d)
@@ -1309,7 +1314,7 @@ static int upload_impl(nyxpm_ctx* c, int box, int64_t np, const double* x, const
         LAUNCHCK(c);
     }
     HIPCK(c, hipStreamSynchronize(c->stream));
-    const long long icap = c->ntiles_total + tot / kChunk + 1;
+    const long long icap = c->ntiles_total + tot / c->kick_chunk + 1;
     if (icap > 0x7fffffffLL) FAIL(c, NYXPM_ERR_ARG, "too many tile chunks");
     TileItem* nitm = nullptr;
     if ((rc = dalloc(c, &nitm, icap))) return rc;

@@ -545,7 +545,7 @@ __device__ __forceinline__ void cic_grad_lines(LD ld, double xint, double yint, 
 // cut into chunks of at most kChunk particles (empty tiles give none).  Slots are
 // reserved with one atomic per wave, so the item order varies between runs; every
 // item is processed independently, so results do not.
-constexpr int kChunk = 4 * kBlock;  // particles per work item (fewer items: less phi staging)
+constexpr int kChunk = 16 * kBlock;  // default particles per work item (fewer items: less phi staging)
 
 struct TileItem {
     long long p0;   // first particle
@@ -554,14 +554,14 @@ struct TileItem {
 };
 
 __global__ void __launch_bounds__(kBlock) k_make_items(const long long* __restrict__ start, long long ntiles,
-                                                       TileItem* __restrict__ items, int* nitems) {
+                                                       TileItem* __restrict__ items, int* nitems, int chunk) {
     const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
     int nc = 0;
     long long p0 = 0, cnt = 0;
     if (t < ntiles) {
         p0 = start[t];
         cnt = start[t + 1] - p0;
-        nc = (int)((cnt + kChunk - 1) / kChunk);
+        nc = (int)((cnt + chunk - 1) / chunk);
     }
     // wave-inclusive prefix of nc, one atomic per wave
     const int lane = threadIdx.x & (kWave - 1);
@@ -577,8 +577,8 @@ __global__ void __launch_bounds__(kBlock) k_make_items(const long long* __restri
     int slot = base + incl - nc;
     for (int c = 0; c < nc; ++c) {
         TileItem it;
-        it.p0 = p0 + (long long)c * kChunk;
-        it.np = (int)min((long long)kChunk, cnt - (long long)c * kChunk);
+        it.p0 = p0 + (long long)c * chunk;
+        it.np = (int)min((long long)chunk, cnt - (long long)c * chunk);
         it.tile = (int)t;
         items[slot + c] = it;
     }
@@ -624,10 +624,25 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
         sphi[idx] = P[cidx(gx, gy, gz, nx, ny)];
     }
     __syncthreads();
+    const double dxi0 = B.dxi[0], dxi1 = B.dxi[1], dxi2 = B.dxi[2];
+    // g at the cells the tile's particles interpolate from (local -1 .. b-1 per
+    // dimension), once per block: the average of face gradients of phi.
+    const int GX = bx + 1, GY = by + 1, GZ = bz + 1, GN = GX * GY * GZ;
+    double* sgx = sphi + SX * SY * SZ;
+    double* sgy = sgx + GN;
+    double* sgz = sgy + GN;
+    for (int idx = threadIdx.x; idx < GN; idx += kBlock) {
+        const int cx = idx % GX, cy = (idx / GX) % GY, cz = idx / (GX * GY);
+        const int o = ((cz + 1) * SY + (cy + 1)) * SX + (cx + 1);  // staged index of the cell
+        const double pc = sphi[o];
+        sgx[idx] = 0.5 * (dxi0 * (pc - sphi[o - 1]) + dxi0 * (sphi[o + 1] - pc));
+        sgy[idx] = 0.5 * (dxi1 * (pc - sphi[o - SX]) + dxi1 * (sphi[o + SX] - pc));
+        sgz[idx] = 0.5 * (dxi2 * (pc - sphi[o - SX * SY]) + dxi2 * (sphi[o + SX * SY] - pc));
+    }
+    __syncthreads();
     const double half_dt = 0.5 * dt;
     const double a_cur_inv = 1.0 / a_cur;
     const double dt_a_cur_inv = dt * a_cur_inv;
-    const double dxi0 = B.dxi[0], dxi1 = B.dxi[1], dxi2 = B.dxi[2];
     for (int q = threadIdx.x; q < it.np; q += kBlock) {
         // prefetch the next particle of this thread
         const long long pn = p + kBlock;
@@ -644,8 +659,33 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
         const int li = wrap1(i, nx) - o0, lj = wrap1(j, ny) - o1, lk = wrap1(k, nz) - o2;
         double val[3];
         if (li >= 0 && li < bx && lj >= 0 && lj < by && lk >= 0 && lk < bz) {
-            auto ld = [&](int a, int b, int c) -> double { return sphi[((lk + c) * SY + (lj + b)) * SX + (li + a)]; };
-            cic_grad_lines(ld, xint, yint, zint, dxi0, dxi1, dxi2, val);
+            // val_d = sum_{kk,jj,ii} sx[1-ii]*sy[1-jj]*sz[1-kk] * g_d(i-ii, j-jj, k-kk);
+            // cell i-ii has g index li+1-ii
+            double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+#pragma unroll
+            for (int kk = 0; kk <= 1; ++kk) {
+                const double wz = kk ? 1.0 - zint : zint;
+#pragma unroll
+                for (int jj = 0; jj <= 1; ++jj) {
+                    const double wy = jj ? 1.0 - yint : yint;
+                    const int row = ((lk + 1 - kk) * GY + (lj + 1 - jj)) * GX + li;
+                    {
+                        const double w = xint * wy * wz;  // ii = 0: cell i
+                        v0 += w * sgx[row + 1];
+                        v1 += w * sgy[row + 1];
+                        v2 += w * sgz[row + 1];
+                    }
+                    {
+                        const double w = (1.0 - xint) * wy * wz;  // ii = 1: cell i-1
+                        v0 += w * sgx[row];
+                        v1 += w * sgy[row];
+                        v2 += w * sgz[row];
+                    }
+                }
+            }
+            val[0] = v0;
+            val[1] = v1;
+            val[2] = v2;
         } else {
             // binned in another tile (cannot happen after a sort; kept for safety)
             if (i < -nx + 2 || i >= 2 * nx - 1 || j < -ny + 2 || j >= 2 * ny - 1 || k < -nz + 2 || k >= 2 * nz - 1)

@@ -48,7 +48,7 @@ class PMContext:
     """One library context: a batch of independent periodic boxes on one GPU."""
 
     def __init__(self, geoms, precision="fp64", tile=None, nu1=2, nu2=2, max_iter=200, smoother="fused",
-                 kick="tiled", sweep_shape=None, sweep_zchunk=0):
+                 kick="tiled", sweep_shape=None, sweep_zchunk=0, kick_chunk=0):
         if isinstance(geoms, BoxGeom):
             geoms = [geoms]
         self.geoms = list(geoms)
@@ -67,6 +67,7 @@ class PMContext:
         if sweep_shape is not None:
             o.sweep_tx, o.sweep_ty = int(sweep_shape[0]), int(sweep_shape[1])
         o.sweep_zchunk = int(sweep_zchunk)
+        o.kick_chunk = int(kick_chunk)
         if tile is not None:
             for d in range(3):
                 o.tile[d] = int(tile[d])
