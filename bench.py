@@ -55,6 +55,7 @@ def kernel_bytes(name, ncell, npart, halo_cells, mixed_sz):
         "fold": 8 * N + 8 * halo_cells + 16 * N,           # read own + halos; write rho, f
         "rhs": 16 * N,
         "kick_drift": 96 * P + 4 * P + 8 * N,              # read/write x,v; key; read phi
+        "kick_pair": 96 * P + 4 * P + 8 * N,               # kick + next kick-drift: same traffic
         "kick": 72 * P + 8 * N,                            # read x,v; write v; read phi
         "scatter": 4 * P + 2 * 56 * P,                     # key; move x,v,id
         "keys": 24 * P + 4 * P,
@@ -177,7 +178,7 @@ def main():
         log(f"accel rel-L2 mixed vs fp64 = {rel_l2:.3e}  (mixed cold-start cycles {int(it_m[0])})")
     modes = a.modes.split(",")
 
-    names = ["residual", "norm", "update", "deposit", "fold", "rhs", "kick_drift", "kick", "scatter", "scan",
+    names = ["residual", "norm", "update", "deposit", "fold", "rhs", "kick_drift", "kick_pair", "kick", "scatter", "scan",
              "items", "keys", "bottom"] + \
             [f"{k}.d{d}" for k in ("sweep0", "sweep", "sweep_p", "sweep_u", "smooth", "resid_restrict", "prolong",
                                     "fill")

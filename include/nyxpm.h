@@ -87,6 +87,9 @@ typedef struct nyxpm_opts {
     int sweep_zchunk;  /* z-planes per fused-sweep block; 0 = chosen per level */
     int kick_chunk;    /* particles per work item of the tile-staged kick (>= 256);
                           0 = default (4096) */
+    int separate_kicks; /* 0 = nyxpm_pm_steps runs the closing kick of a step and
+                           the next step's kick-drift as one particle pass
+                           (bit-identical); 1 = separate passes */
 } nyxpm_opts;
 
 /* ---- lifecycle ---------------------------------------------------------- */

@@ -42,7 +42,8 @@ class Opts(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("nu1", ctypes.c_int), ("nu2", ctypes.c_int),
                 ("max_iter", ctypes.c_int), ("tile", ctypes.c_int * 3), ("verbose", ctypes.c_int),
                 ("smoother", ctypes.c_int), ("kick", ctypes.c_int), ("sweep_tx", ctypes.c_int),
-                ("sweep_ty", ctypes.c_int), ("sweep_zchunk", ctypes.c_int), ("kick_chunk", ctypes.c_int)]
+                ("sweep_ty", ctypes.c_int), ("sweep_zchunk", ctypes.c_int), ("kick_chunk", ctypes.c_int),
+                ("separate_kicks", ctypes.c_int)]
 
 
 # (name, restype, argtypes) for every function in include/nyxpm.h

@@ -126,6 +126,7 @@ struct nyxpm_ctx {
     int graph_cycles = 24;
     bool no_fused = false;  // force half-sweep smoothing (for comparisons)
     bool untiled_kick = false;  // force the thread-per-particle kick (for comparisons)
+    bool separate_kicks = false;  // nyxpm_pm_steps without the kick / kick-drift pairing
     int sweep_zchunk = 0;       // fixed z-chunk of the fused sweep (0: chosen per level)
     int force_tx[2] = {0, 0}, force_ty[2] = {8, 8};  // fixed sweep shape per precision (0: chosen)
     bool fuse_update = true;    // phi += e in the last finest-level sweep
@@ -210,6 +211,7 @@ struct nyxpm_ctx {
     hipGraph_t graph = nullptr;
     double g_key[6] = {0, 0, 0, 0, 0, 0};
     int g_prec = -1;
+    int g_mode = -1;  // 0 separate kicks, 1 faithful, 2 pair kicks
 
     long long device_bytes = 0;
 
@@ -1053,26 +1055,33 @@ int solve_results(nyxpm_ctx* c, int* iters, double* final_res, bool allow_active
     return NYXPM_OK;
 }
 
-template <bool MOVE>
-int phase_kick(nyxpm_ctx* c, double dt, double a_prev, double a_cur) {
+// MODE is kKick (a_prev, a_cur), kKickDrift (a_prev, a_cur) or kKickPair: the
+// kick (a_prev, a_cur) of one step and the kick-drift (a_prev2, a_cur2) of the next.
+template <int MODE>
+int phase_kick(nyxpm_ctx* c, double dt, double a_prev, double a_cur, double a_prev2 = 1.0, double a_cur2 = 1.0) {
     PartPtrs& A = c->P[c->cur];
+    constexpr bool MOVE = MODE != kKick;
     // staged phi tile with a halo of 2, plus the three components of g on the
     // (tile+1)^3 cells its particles interpolate from
     const size_t lds = sizeof(double) * ((c->tile[0] + 3) * (c->tile[1] + 3) * (c->tile[2] + 3) +
                                          3 * (c->tile[0] + 1) * (c->tile[1] + 1) * (c->tile[2] + 1));
     if (c->items_valid && !c->untiled_kick && lds <= 64 * 1024) {
-        Timed t(c, MOVE ? "kick_drift" : "kick");
+        Timed t(c, MODE == kKickPair ? "kick_pair" : MOVE ? "kick_drift" : "kick");
         if (MOVE)
             hipLaunchKernelGGL(k_fill_int, dim3(nblocks(c->ntiles_total)), dim3(kBlock), 0, c->stream, c->counts,
                                c->ntiles_total, 0);
-        hipLaunchKernelGGL(k_kick_tiled<MOVE>, dim3((unsigned)c->items_cap), dim3(kBlock), lds, c->stream, c->tseg,
+        hipLaunchKernelGGL(k_kick_tiled<MODE>, dim3((unsigned)c->items_cap), dim3(kBlock), lds, c->stream, c->tseg,
                            c->tseg_n, c->dboxes, c->items, c->nitems, c->phi, A.x, A.y, A.z, A.vx, A.vy, A.vz, dt,
-                           a_prev, a_cur, c->key, c->counts, c->err_flag);
+                           a_prev, a_cur, a_prev2, a_cur2, c->key, c->counts, c->err_flag);
         LAUNCHCK(c);
         c->keys_valid = MOVE;
         c->sorted_valid = false;
         c->density_valid = false;
         return NYXPM_OK;
+    }
+    if (MODE == kKickPair) {
+        int rc = phase_kick<kKick>(c, dt, a_prev, a_cur);
+        return rc ? rc : phase_kick<kKickDrift>(c, dt, a_prev2, a_cur2);
     }
     if (MOVE) {
         Timed t(c, "kick_drift");
@@ -1122,13 +1131,28 @@ int step_body(nyxpm_ctx* c, double dt, double a_old, double a_new, double four_p
         if ((rc = solve(c, rel, abs_tol, 0, graph_cycles))) return rc;
         if (graph_cycles == 0 && (rc = solve_results(c, nullptr, nullptr, false))) return rc;
     }
-    if ((rc = phase_kick<true>(c, dt, a_old, a_half))) return rc;
+    if ((rc = phase_kick<kKickDrift>(c, dt, a_old, a_half))) return rc;
     if ((rc = phase_sort(c))) return rc;
     if ((rc = phase_deposit(c))) return rc;
     if ((rc = phase_rhs(c, a_new, four_pi_G))) return rc;
     if ((rc = solve(c, rel, abs_tol, 0, graph_cycles))) return rc;
-    if ((rc = phase_kick<false>(c, dt, a_half, a_new))) return rc;
+    if ((rc = phase_kick<kKick>(c, dt, a_half, a_new))) return rc;
     return NYXPM_OK;
+}
+
+// A single-solve KDK step whose opening kick-drift was already done (by the
+// previous step's pair kick or by the prologue of nyxpm_pm_steps).  With
+// pair_next the closing kick is fused with the next step's kick-drift.
+int step_tail(nyxpm_ctx* c, double dt, double a_old, double a_new, double four_pi_G, double rel, double abs_tol,
+              int graph_cycles, bool pair_next) {
+    int rc;
+    const double a_half = 0.5 * (a_old + a_new);
+    if ((rc = phase_sort(c))) return rc;
+    if ((rc = phase_deposit(c))) return rc;
+    if ((rc = phase_rhs(c, a_new, four_pi_G))) return rc;
+    if ((rc = solve(c, rel, abs_tol, 0, graph_cycles))) return rc;
+    if (pair_next) return phase_kick<kKickPair>(c, dt, a_half, a_new, a_old, a_half);
+    return phase_kick<kKick>(c, dt, a_half, a_new);
 }
 
 }  // namespace
@@ -1154,6 +1178,7 @@ int nyxpm_create_batch(nyxpm_ctx** out, int nbox, const nyxpm_geom* geoms, const
         c->verbose = opts->verbose;
         c->no_fused = opts->smoother == 1;
         c->untiled_kick = opts->kick == 1;
+        c->separate_kicks = opts->separate_kicks != 0;
         if (opts->sweep_tx > 0) {
             c->force_tx[0] = c->force_tx[1] = opts->sweep_tx;
             c->force_ty[0] = c->force_ty[1] = opts->sweep_ty > 0 ? opts->sweep_ty : 8;
@@ -1542,7 +1567,7 @@ int nyxpm_move_kick_drift(nyxpm_ctx* c, double dt, double a_old, double a_half) 
     int rc;
     if ((rc = require_particles(c))) return rc;
     if (!(a_half > 0.0)) FAIL(c, NYXPM_ERR_ARG, "a_half must be > 0");
-    if ((rc = phase_kick<true>(c, dt, a_old, a_half))) return rc;
+    if ((rc = phase_kick<kKickDrift>(c, dt, a_old, a_half))) return rc;
     return nyxpm_sync(c);
 }
 
@@ -1551,7 +1576,7 @@ int nyxpm_move_kick(nyxpm_ctx* c, double dt, double a_new, double a_half) {
     int rc;
     if ((rc = require_particles(c))) return rc;
     if (!(a_new > 0.0)) FAIL(c, NYXPM_ERR_ARG, "a_new must be > 0");
-    if ((rc = phase_kick<false>(c, dt, a_half, a_new))) return rc;
+    if ((rc = phase_kick<kKick>(c, dt, a_half, a_new))) return rc;
     return nyxpm_sync(c);
 }
 
@@ -1588,14 +1613,28 @@ int nyxpm_pm_steps(nyxpm_ctx* c, int nsteps, double dt, double a_old, double a_n
     if ((rc = require_particles(c))) return rc;
     if (!c->phi_valid && !nyx_faithful) FAIL(c, NYXPM_ERR_STATE, "call nyxpm_pm_init before stepping");
     if (!(a_old > 0.0) || !(a_new > 0.0)) FAIL(c, NYXPM_ERR_ARG, "scale factors must be > 0");
+    // Without the faithful re-solve, the closing kick of step s and the opening
+    // kick-drift of step s+1 read g from the same phi at the same positions and
+    // run as one particle pass (bit-identical to the separate kernels).
+    const bool fused = !nyx_faithful && !c->separate_kicks;
+    const double a_half = 0.5 * (a_old + a_new);
+    if (fused && nsteps > 0 && (rc = phase_kick<kKickDrift>(c, dt, a_old, a_half))) return rc;
+    auto one_step = [&](int s) -> int {
+        if (fused) return step_tail(c, dt, a_old, a_new, four_pi_G, rel_tol, abs_tol, 0, s + 1 < nsteps);
+        return step_body(c, dt, a_old, a_new, four_pi_G, rel_tol, abs_tol, nyx_faithful, 0);
+    };
     for (int s = 0; s < nsteps; ++s) {
-        if (!use_graph) {
-            if ((rc = step_body(c, dt, a_old, a_new, four_pi_G, rel_tol, abs_tol, nyx_faithful, 0))) return rc;
+        // a replay covers two steps; in the fused form both must end in a pair kick
+        const bool replay = use_graph && (fused ? s + 2 < nsteps : s + 1 < nsteps);
+        if (!replay) {
+            if ((rc = one_step(s))) return rc;
             if ((rc = solve_results(c, iters, nullptr, false))) return rc;
             continue;
         }
-        const double key[6] = {dt, a_old, a_new, four_pi_G, rel_tol, abs_tol + (nyx_faithful ? 1e300 : 0.0)};
-        bool same = c->gexec && c->g_prec == c->precision && std::memcmp(key, c->g_key, sizeof(key)) == 0;
+        const double key[6] = {dt, a_old, a_new, four_pi_G, rel_tol, abs_tol};
+        const int mode = fused ? 2 : nyx_faithful ? 1 : 0;
+        bool same = c->gexec && c->g_prec == c->precision && c->g_mode == mode &&
+                    std::memcmp(key, c->g_key, sizeof(key)) == 0;
         if (!same) {
             if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
             if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
@@ -1607,8 +1646,10 @@ int nyxpm_pm_steps(nyxpm_ctx* c, int nsteps, double dt, double a_old, double a_n
             // their starting roles
             HIPCK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             int cur0 = c->cur;
-            rc = step_body(c, dt, a_old, a_new, four_pi_G, rel_tol, abs_tol, nyx_faithful, c->graph_cycles);
-            if (!rc) rc = step_body(c, dt, a_old, a_new, four_pi_G, rel_tol, abs_tol, nyx_faithful, c->graph_cycles);
+            for (int r = 0; r < 2 && !rc; ++r)
+                rc = fused ? step_tail(c, dt, a_old, a_new, four_pi_G, rel_tol, abs_tol, c->graph_cycles, true)
+                           : step_body(c, dt, a_old, a_new, four_pi_G, rel_tol, abs_tol, nyx_faithful,
+                                       c->graph_cycles);
             hipGraph_t g = nullptr;
             hipError_t ee = hipStreamEndCapture(c->stream, &g);
             c->timing = was_timing;
@@ -1619,17 +1660,12 @@ int nyxpm_pm_steps(nyxpm_ctx* c, int nsteps, double dt, double a_old, double a_n
             HIPCK(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
             std::memcpy(c->g_key, key, sizeof(key));
             c->g_prec = c->precision;
+            c->g_mode = mode;
         }
-        // one replay = two steps
-        if (s + 1 < nsteps) {
-            HIPCK(c, hipGraphLaunch(c->gexec, c->stream));
-            ++s;
-            // a box still active after the recorded cycles has not converged
-            if ((rc = solve_results(c, iters, nullptr, false))) return rc;
-        } else {
-            if ((rc = step_body(c, dt, a_old, a_new, four_pi_G, rel_tol, abs_tol, nyx_faithful, 0))) return rc;
-            if ((rc = solve_results(c, iters, nullptr, false))) return rc;
-        }
+        HIPCK(c, hipGraphLaunch(c->gexec, c->stream));
+        ++s;
+        // a box still active after the recorded cycles has not converged
+        if ((rc = solve_results(c, iters, nullptr, false))) return rc;
     }
     c->sorted_valid = false;
     c->keys_valid = false;

@@ -584,19 +584,32 @@ __global__ void __launch_bounds__(kBlock) k_make_items(const long long* __restri
     }
 }
 
-// Tile-staged gather + kick(-drift): phi over the tile plus a halo of 2 (the
-// gradient stencil of the CIC cells i-1, i of every particle binned here) is
-// staged in LDS once per item; particles whose cell index falls outside the tile
-// (binning older than a periodic shift) take the global-memory path.  Same
-// arithmetic as k_kick, bit for bit.
-template <bool DO_MOVE>
+// Kick variants of the tile-staged particle kernel.
+constexpr int kKick = 0;       // moveKick: kick, then the periodic shift of Redistribute
+constexpr int kKickDrift = 1;  // moveKickDrift: kick, drift, new tile key
+constexpr int kKickPair = 2;   // moveKick of one step followed by moveKickDrift of the next
+
+// Tile-staged gather + kick(-drift): phi over the tile plus a halo of 2 is
+// staged in LDS once per item and g (the average of the face gradients) is
+// formed there on the (tile+1)^3 cells that the CIC stencils of the tile's
+// particles touch; particles whose cell index falls outside the tile (binning
+// older than a periodic shift) take the global-memory path.  Same arithmetic as
+// k_kick, bit for bit.
+//
+// kKickPair runs the closing kick of step n (a_prev, a_cur) and the opening
+// kick-drift of step n+1 (a_prev2, a_cur2) in one pass: both use g from the same
+// phi at the same position, so the interpolated value is reused unless the
+// periodic shift moved the particle, in which case it is interpolated again at
+// the shifted position (as the separate kernels would).
+template <int MODE>
 __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int ntseg, const BoxDev* boxes,
                                                        const TileItem* __restrict__ items, const int* nitems,
                                                        const double* __restrict__ phi, double* __restrict__ x,
                                                        double* __restrict__ y, double* __restrict__ z,
                                                        double* __restrict__ vx, double* __restrict__ vy,
                                                        double* __restrict__ vz, double dt, double a_prev,
-                                                       double a_cur, int* __restrict__ key, int* counts, int* err) {
+                                                       double a_cur, double a_prev2, double a_cur2,
+                                                       int* __restrict__ key, int* counts, int* err) {
     extern __shared__ double sphi[];
     if ((int)blockIdx.x >= *nitems) return;
     const TileItem it = items[blockIdx.x];
@@ -612,9 +625,8 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
     // the first particle of this thread is loaded before the staging of phi so the
     // two latencies overlap; later ones are prefetched one iteration ahead
     long long p = it.p0 + threadIdx.x;
-    bool valid = (int)threadIdx.x < it.np;
     double px = 0.0, py = 0.0, pz = 0.0, qx = 0.0, qy = 0.0, qz = 0.0;
-    if (valid) {
+    if ((int)threadIdx.x < it.np) {
         px = x[p]; py = y[p]; pz = z[p];
         qx = vx[p]; qy = vy[p]; qz = vz[p];
     }
@@ -626,7 +638,7 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
     __syncthreads();
     const double dxi0 = B.dxi[0], dxi1 = B.dxi[1], dxi2 = B.dxi[2];
     // g at the cells the tile's particles interpolate from (local -1 .. b-1 per
-    // dimension), once per block: the average of face gradients of phi.
+    // dimension), once per block
     const int GX = bx + 1, GY = by + 1, GZ = bz + 1, GN = GX * GY * GZ;
     double* sgx = sphi + SX * SY * SZ;
     double* sgy = sgx + GN;
@@ -640,26 +652,14 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
         sgz[idx] = 0.5 * (dxi2 * (pc - sphi[o - SX * SY]) + dxi2 * (sphi[o + SX * SY] - pc));
     }
     __syncthreads();
-    const double half_dt = 0.5 * dt;
-    const double a_cur_inv = 1.0 / a_cur;
-    const double dt_a_cur_inv = dt * a_cur_inv;
-    for (int q = threadIdx.x; q < it.np; q += kBlock) {
-        // prefetch the next particle of this thread
-        const long long pn = p + kBlock;
-        const bool vn = q + kBlock < it.np;
-        double nxp = 0.0, nyp = 0.0, nzp = 0.0, nqx = 0.0, nqy = 0.0, nqz = 0.0;
-        if (vn) {
-            nxp = x[pn]; nyp = y[pn]; nzp = z[pn];
-            nqx = vx[pn]; nqy = vy[pn]; nqz = vz[pn];
-        }
+    // val_d = sum_{kk,jj,ii} sx[1-ii]*sy[1-jj]*sz[1-kk] * g_d(i-ii, j-jj, k-kk)
+    auto interp = [&](double ax, double ay, double az, double val[3]) {
         double xint, yint, zint;
-        const int i = cic_index(px, B.plo[0], B.dxi[0], &xint);
-        const int j = cic_index(py, B.plo[1], B.dxi[1], &yint);
-        const int k = cic_index(pz, B.plo[2], B.dxi[2], &zint);
+        const int i = cic_index(ax, B.plo[0], dxi0, &xint);
+        const int j = cic_index(ay, B.plo[1], dxi1, &yint);
+        const int k = cic_index(az, B.plo[2], dxi2, &zint);
         const int li = wrap1(i, nx) - o0, lj = wrap1(j, ny) - o1, lk = wrap1(k, nz) - o2;
-        double val[3];
         if (li >= 0 && li < bx && lj >= 0 && lj < by && lk >= 0 && lk < bz) {
-            // val_d = sum_{kk,jj,ii} sx[1-ii]*sy[1-jj]*sz[1-kk] * g_d(i-ii, j-jj, k-kk);
             // cell i-ii has g index li+1-ii
             double v0 = 0.0, v1 = 0.0, v2 = 0.0;
 #pragma unroll
@@ -687,7 +687,7 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
             val[1] = v1;
             val[2] = v2;
         } else {
-            // binned in another tile (cannot happen after a sort; kept for safety)
+            // binned in another tile
             if (i < -nx + 2 || i >= 2 * nx - 1 || j < -ny + 2 || j >= 2 * ny - 1 || k < -nz + 2 || k >= 2 * nz - 1)
                 atomicOr(err, 4);
             auto ld = [&](int a, int b, int c) -> double {
@@ -695,38 +695,73 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
             };
             cic_grad_lines(ld, xint, yint, zint, dxi0, dxi1, dxi2, val);
         }
+    };
+    const double half_dt = 0.5 * dt;
+    const double a_cur_inv = 1.0 / a_cur;
+    const double dt_a_cur_inv = dt * a_cur_inv;
+    const double a_cur2_inv = 1.0 / a_cur2;
+    const double dt_a_cur2_inv = dt * a_cur2_inv;
+    for (int q = threadIdx.x; q < it.np; q += kBlock) {
+        // prefetch the next particle of this thread
+        const long long pn = p + kBlock;
+        const bool vn = q + kBlock < it.np;
+        double nxp = 0.0, nyp = 0.0, nzp = 0.0, nqx = 0.0, nqy = 0.0, nqz = 0.0;
+        if (vn) {
+            nxp = x[pn]; nyp = y[pn]; nzp = z[pn];
+            nqx = vx[pn]; nqy = vy[pn]; nqz = vz[pn];
+        }
+        double val[3];
+        interp(px, py, pz, val);
         double ux = a_prev * qx + half_dt * val[0];
         ux = ux * a_cur_inv;
         double uy = a_prev * qy + half_dt * val[1];
         uy = uy * a_cur_inv;
         double uz = a_prev * qz + half_dt * val[2];
         uz = uz * a_cur_inv;
-        vx[p] = ux;
-        vy[p] = uy;
-        vz[p] = uz;
-        if (DO_MOVE) {
+        if (MODE == kKickDrift) {
             px = px + dt_a_cur_inv * ux;
             py = py + dt_a_cur_inv * uy;
             pz = pz + dt_a_cur_inv * uz;
+        } else {
+            const double sxp = periodic_shift(px, B.plo[0], B.rhi[0], B.len[0], err);
+            const double syp = periodic_shift(py, B.plo[1], B.rhi[1], B.len[1], err);
+            const double szp = periodic_shift(pz, B.plo[2], B.rhi[2], B.len[2], err);
+            const bool moved = sxp != px || syp != py || szp != pz;
+            if (MODE == kKick) {
+                if (sxp != px) x[p] = sxp;
+                if (syp != py) y[p] = syp;
+                if (szp != pz) z[p] = szp;
+            } else {
+                px = sxp;
+                py = syp;
+                pz = szp;
+                if (moved) interp(px, py, pz, val);
+                double wx = a_prev2 * ux + half_dt * val[0];
+                ux = wx * a_cur2_inv;
+                double wy = a_prev2 * uy + half_dt * val[1];
+                uy = wy * a_cur2_inv;
+                double wz = a_prev2 * uz + half_dt * val[2];
+                uz = wz * a_cur2_inv;
+                px = px + dt_a_cur2_inv * ux;
+                py = py + dt_a_cur2_inv * uy;
+                pz = pz + dt_a_cur2_inv * uz;
+            }
+        }
+        vx[p] = ux;
+        vy[p] = uy;
+        vz[p] = uz;
+        if (MODE != kKick) {
             x[p] = px;
             y[p] = py;
             z[p] = pz;
             const int t = tile_of(B, px, py, pz, err);
             key[p] = t;
             agg_atomic_add(counts, t, true);
-        } else {
-            const double sxp = periodic_shift(px, B.plo[0], B.rhi[0], B.len[0], err);
-            const double syp = periodic_shift(py, B.plo[1], B.rhi[1], B.len[1], err);
-            const double szp = periodic_shift(pz, B.plo[2], B.rhi[2], B.len[2], err);
-            if (sxp != px) x[p] = sxp;
-            if (syp != py) y[p] = syp;
-            if (szp != pz) z[p] = szp;
         }
         p = pn;
         px = nxp; py = nyp; pz = nzp;
         qx = nqx; qy = nqy; qz = nqz;
     }
-    (void)valid;
 }
 
 // Periodic shift alone (nyxpm_redistribute).

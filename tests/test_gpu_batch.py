@@ -112,3 +112,35 @@ def test_autotune_keeps_results(nyx):
         for d in range(3):
             assert np.array_equal(outs[0][b][0][d], outs[1][b][0][d])
             assert np.array_equal(outs[0][b][1][d], outs[1][b][1][d])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("steps,use_graph", [(1, False), (3, False), (6, True)])
+def test_pair_kicks_equal_separate(nyx, precision, steps, use_graph):
+    # the closing kick of a step fused with the next step's kick-drift gives the
+    # same bits as the two passes; fast particles cross the periodic boundary, so
+    # the re-interpolation after the shift is exercised
+    # cubic cells in both boxes (the second is 0.75 x 1.25 x 0.5)
+    geoms = [nyx.BoxGeom((32, 32, 32)), nyx.BoxGeom((24, 40, 16), (0.0, 0.0, 0.0), (0.75, 1.25, 0.5))]
+    inputs = []
+    for i, g in enumerate(geoms):
+        r = np.random.default_rng(500 + i)
+        npart = g.ncell
+        pos = [r.uniform(g.prob_lo[d], g.prob_hi[d], npart) for d in range(3)]
+        vel = [r.normal(0.0, 8.0, npart) for _ in range(3)]
+        inputs.append((pos, vel, None, 1.0 / npart))
+    outs = []
+    for pair in (False, True):
+        with nyx.PMContext(geoms, precision=precision, pair_kicks=pair) as c:
+            for b, inp in enumerate(inputs):
+                _upload(c, b, inp)
+            c.pm_init(1.0, 1.0, 1e-10)
+            its = c.pm_steps(steps, 1e-3, rel_tol=1e-10, use_graph=use_graph)
+            outs.append(([c.download(b) for b in range(2)], [c.get_phi(b) for b in range(2)], its))
+    (sep, sphi, sits), (fus, fphi, fits) = outs
+    assert list(sits) == list(fits)
+    for b in range(2):
+        assert np.array_equal(sphi[b], fphi[b])
+        for d in range(3):
+            assert np.array_equal(sep[b][0][d], fus[b][0][d])
+            assert np.array_equal(sep[b][1][d], fus[b][1][d])

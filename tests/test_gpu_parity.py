@@ -235,7 +235,8 @@ def test_tiled_kick_equals_global_kick(nyx, tile):
 # whole step
 # ---------------------------------------------------------------------------
 
-def _oracle_steps(orc, n, pos, vel, m, nsteps, dt, rel_tol):
+def _oracle_steps(orc, n, pos, vel, m, nsteps, dt, rel_tol, keep=None):
+    # state after each step (or only after the steps listed in keep; None elsewhere)
     pos = [a.copy() for a in pos]
     vel = [a.copy() for a in vel]
     phi, _, _ = orc.solve_for_phi(n, (0, 0, 0), (1, 1, 1), pos, m_const=m, rel_tol=rel_tol)
@@ -244,7 +245,10 @@ def _oracle_steps(orc, n, pos, vel, m, nsteps, dt, rel_tol):
     for s in range(nsteps):
         g, st = orc.pm_step(n, (0, 0, 0), (1, 1, 1), pos, vel, phi, m_const=m, dt=dt, rel_tol=rel_tol,
                             want_g=True)
-        out.append(([a.copy() for a in pos], [a.copy() for a in vel], g.copy(), phi.copy(), int(st[0])))
+        if keep is None or s + 1 in keep:
+            out.append(([a.copy() for a in pos], [a.copy() for a in vel], g.copy(), phi.copy(), int(st[0])))
+        else:
+            out.append(None)
     return out
 
 
@@ -281,6 +285,29 @@ def test_grf_recipe_ten_steps_vs_oracle(nyx, orc):
             p, v = c.download()
             g = c.grav_vector()
             rp, rv, rg, _, _ = ref[s - 1]
+            assert relmax(g, rg) <= 1e-12
+            for d in range(3):
+                assert relmax(p[d], rp[d]) <= 1e-12
+                assert relmax(v[d], rv[d]) <= 1e-12
+
+
+@pytest.mark.timeout(600)
+def test_grf256_ten_steps_vs_oracle(nyx, orc):
+    # the drop-in configuration at full size: 256^3 cells and particles, lattice +
+    # GRF displacement (P(k)~k^-2, rms 0.5 cell, v = 0.1 psi), FP64, ten KDK steps
+    # with dt 1e-3, compared with the oracle after steps 1 and 10
+    n = (256, 256, 256)
+    pos, vel, m = nyx.ics.grf_lattice(256, nyx.ics.SEED_BASE + 1, -2.0, 0.5, 0.1)
+    ref = _oracle_steps(orc, n, pos, vel, m, 10, 1e-3, 1e-10, keep=(1, 10))
+    with ctx1(nyx, n) as c:
+        c.upload(0, pos, vel, m_const=m)
+        c.pm_init(1.0, 1.0, 1e-10)
+        for s in (1, 10):
+            its = c.pm_steps(1 if s == 1 else 9, 1e-3, rel_tol=1e-10)
+            p, v = c.download()
+            g = c.grav_vector()
+            rp, rv, rg, _, rit = ref[s - 1]
+            assert int(its[0]) == rit
             assert relmax(g, rg) <= 1e-12
             for d in range(3):
                 assert relmax(p[d], rp[d]) <= 1e-12
