@@ -90,6 +90,9 @@ typedef struct nyxpm_opts {
     int separate_kicks; /* 0 = nyxpm_pm_steps runs the closing kick of a step and
                            the next step's kick-drift as one particle pass
                            (bit-identical); 1 = separate passes */
+    int coarse;        /* 0: the levels of at most 4096 cells (down to the bottom
+                          solve) run as one workgroup per box (default);
+                          1: one launch per level and step (same results) */
 } nyxpm_opts;
 
 /* ---- lifecycle ---------------------------------------------------------- */

@@ -43,7 +43,7 @@ class Opts(ctypes.Structure):
                 ("max_iter", ctypes.c_int), ("tile", ctypes.c_int * 3), ("verbose", ctypes.c_int),
                 ("smoother", ctypes.c_int), ("kick", ctypes.c_int), ("sweep_tx", ctypes.c_int),
                 ("sweep_ty", ctypes.c_int), ("sweep_zchunk", ctypes.c_int), ("kick_chunk", ctypes.c_int),
-                ("separate_kicks", ctypes.c_int)]
+                ("separate_kicks", ctypes.c_int), ("coarse", ctypes.c_int)]
 
 
 # (name, restype, argtypes) for every function in include/nyxpm.h

@@ -328,6 +328,156 @@ __global__ void __launch_bounds__(kBlock) k_bottom(const CellSeg* segs, int nseg
     }
 }
 
+// The coarse end of a V-cycle in one workgroup per box: every level from the
+// first one with at most kCoarseCells cells (segs[first[b]], "top") down to the
+// bottom (segs[first[b+1]-1]), and back up.  Corrections e of all these levels
+// and residuals r of the levels below the top live in LDS; the top's r is read
+// from the pool and the top's final e is written to the pool (buffer e), where
+// the prolongation of the next finer level finds it.  Per cell the arithmetic is
+// that of k_fill, k_gsrb_half, k_resid_restrict, k_bottom and k_prolong, in the
+// same order, so the result equals those launches bit for bit; what it saves is
+// ~30 latency-bound launches per cycle.
+constexpr int kCoarseCells = 4096;
+constexpr int kCoarseThreads = 1024;
+
+template <typename T>
+__device__ __forceinline__ void coarse_half(const CellSeg& sg, T* E, const T* R, int colour) {
+    const int nx = sg.n[0], ny = sg.n[1], nz = sg.n[2];
+    const int hx = nx >> 1;
+    const int nh = hx * ny * nz;
+    const T dhx = T(sg.dh[0]), dhy = T(sg.dh[1]), dhz = T(sg.dh[2]);
+    const T gamma = T(sg.gamma), inv_gamma = T(sg.inv_gamma);
+    for (int q = threadIdx.x; q < nh; q += blockDim.x) {
+        const int ih = q % hx;
+        const int t = q / hx;
+        const int j = t % ny, k = t / ny;
+        const int i = 2 * ih + ((j + k + colour) & 1);
+        int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
+        int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
+        int km = k == 0 ? nz - 1 : k - 1, kp = k == nz - 1 ? 0 : k + 1;
+        const int c = (k * ny + j) * nx + i;
+        T ec = E[c];
+        T res = R[c] - gamma * ec
+              - dhx * (E[(k * ny + j) * nx + im] + E[(k * ny + j) * nx + ip])
+              - dhy * (E[(k * ny + jm) * nx + i] + E[(k * ny + jp) * nx + i])
+              - dhz * (E[(km * ny + j) * nx + i] + E[(kp * ny + j) * nx + i]);
+        E[c] = ec + res * inv_gamma;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kCoarseThreads) k_coarse_vcycle(const CellSeg* segs, const int* first,
+                                                                  const int* active, T* __restrict__ e,
+                                                                  const T* __restrict__ r,
+                                                                  const T* __restrict__ G, int nu1, int nu2) {
+    extern __shared__ double coarse_smem[];
+    T* lds = reinterpret_cast<T*>(coarse_smem);
+    const int l0 = first[blockIdx.x], l1 = first[blockIdx.x + 1];
+    if (!active[segs[l0].box]) return;
+    const int nl = l1 - l0;
+    auto ncell = [&](int l) { return segs[l].n[0] * segs[l].n[1] * segs[l].n[2]; };
+    // LDS: e of levels l0..l1-1, then r of levels l0+1..l1-1
+    auto eoff = [&](int l) {
+        int o = 0;
+        for (int q = l0; q < l; ++q) o += ncell(q);
+        return o;
+    };
+    auto roff = [&](int l) {
+        int o = eoff(l1);
+        for (int q = l0 + 1; q < l; ++q) o += ncell(q);
+        return o;
+    };
+    auto rptr = [&](int l) -> const T* { return l == l0 ? r + segs[l0].off : lds + roff(l); };
+    const int tid = threadIdx.x, nth = blockDim.x;
+    // down: zero guess, nu1 red-black sweeps, residual restricted to the next level
+    for (int l = l0; l + 1 < l1; ++l) {
+        const CellSeg& sg = segs[l];
+        T* E = lds + eoff(l);
+        const T* R = rptr(l);
+        const int nc = ncell(l);
+        for (int q = tid; q < nc; q += nth) E[q] = T(0);
+        __syncthreads();
+        for (int it = 0; it < nu1; ++it)
+            for (int col = 0; col < 2; ++col) {
+                coarse_half(sg, E, R, col);
+                __syncthreads();
+            }
+        const int nx = sg.n[0], ny = sg.n[1], nz = sg.n[2];
+        const int cx = nx >> 1, cy = ny >> 1, cz = nz >> 1;
+        const T dhx = T(sg.dh[0]), dhy = T(sg.dh[1]), dhz = T(sg.dh[2]);
+        T* Rc = lds + roff(l + 1);
+        for (int q = tid; q < cx * cy * cz; q += nth) {
+            const int I = q % cx;
+            const int t = q / cx;
+            const int J = t % cy, K = t / cy;
+            T acc = T(0);
+            for (int kk = 0; kk < 2; ++kk)
+                for (int jj = 0; jj < 2; ++jj)
+                    for (int ii = 0; ii < 2; ++ii) {
+                        const int i = 2 * I + ii, j = 2 * J + jj, k = 2 * K + kk;
+                        T rc = R[(k * ny + j) * nx + i] - lap7(E, i, j, k, nx, ny, nz, dhx, dhy, dhz);
+                        acc += rc;
+                    }
+            Rc[q] = T(0.125) * acc;
+        }
+        __syncthreads();
+    }
+    // bottom: e = G * r (periodic convolution, natural order)
+    {
+        const int lb = l1 - 1;
+        const CellSeg& sg = segs[lb];
+        const int nx = sg.n[0], ny = sg.n[1], nz = sg.n[2];
+        const int nc = nx * ny * nz;
+        const T* R = rptr(lb);
+        const T* Gb = G + sg.coff;
+        T* E = lds + eoff(lb);
+        for (int c = tid; c < nc; c += nth) {
+            const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+            T acc = T(0);
+            for (int qz = 0; qz < nz; ++qz) {
+                int dz = cz - qz; dz = dz < 0 ? dz + nz : dz;
+                for (int qy = 0; qy < ny; ++qy) {
+                    int dy = cy - qy; dy = dy < 0 ? dy + ny : dy;
+                    for (int qx = 0; qx < nx; ++qx) {
+                        int dxx = cx - qx; dxx = dxx < 0 ? dxx + nx : dxx;
+                        acc += Gb[(dz * ny + dy) * nx + dxx] * R[(qz * ny + qy) * nx + qx];
+                    }
+                }
+            }
+            E[c] = acc;
+        }
+        __syncthreads();
+    }
+    // up: piecewise-constant prolongation, nu2 red-black sweeps
+    for (int l = l1 - 2; l >= l0; --l) {
+        const CellSeg& sg = segs[l];
+        T* E = lds + eoff(l);
+        const T* Ec = lds + eoff(l + 1);
+        const int nx = sg.n[0], ny = sg.n[1], nz = sg.n[2];
+        const int nc = nx * ny * nz;
+        for (int q = tid; q < nc; q += nth) {
+            const int i = q % nx;
+            const int t = q / nx;
+            const int j = t % ny, k = t / ny;
+            E[q] = E[q] + Ec[((k >> 1) * (ny >> 1) + (j >> 1)) * (nx >> 1) + (i >> 1)];
+        }
+        __syncthreads();
+        const T* R = rptr(l);
+        for (int it = 0; it < nu2; ++it)
+            for (int col = 0; col < 2; ++col) {
+                coarse_half(sg, E, R, col);
+                __syncthreads();
+            }
+    }
+    {
+        T* Eg = e + segs[l0].off;
+        const T* E = lds + eoff(l0);
+        const int nc = ncell(l0);
+        for (int q = tid; q < nc; q += nth) Eg[q] = E[q];
+    }
+    (void)nl;
+}
+
 // Level-0 residual r = f - L phi (FP64), stored as T.  Threads own (i, j)
 // columns (64 x 4 per block) and march up a chunk of zc planes keeping
 // phi(k-1), phi(k), phi(k+1) in registers; in-plane neighbours come from cache.

@@ -48,6 +48,7 @@ struct Box {
     double m_const = 0.0, mass_total = 0.0, mass_offset = 0.0;
     int fx_exp = 0;
     long long G_off;
+    int dc = 0;  // first level of the one-workgroup coarse V-cycle (nlev-1: bottom only)
 };
 
 struct EventPair {
@@ -127,6 +128,7 @@ struct nyxpm_ctx {
     bool no_fused = false;  // force half-sweep smoothing (for comparisons)
     bool untiled_kick = false;  // force the thread-per-particle kick (for comparisons)
     bool separate_kicks = false;  // nyxpm_pm_steps without the kick / kick-drift pairing
+    bool per_level_coarse = false;  // coarse levels by per-level launches, not k_coarse_vcycle
     int sweep_zchunk = 0;       // fixed z-chunk of the fused sweep (0: chosen per level)
     int force_tx[2] = {0, 0}, force_ty[2] = {8, 8};  // fixed sweep shape per precision (0: chosen)
     bool fuse_update = true;    // phi += e in the last finest-level sweep
@@ -194,6 +196,11 @@ struct nyxpm_ctx {
     std::vector<int> pre_buf;    // buffer holding e after pre-smoothing, per depth
     std::vector<int> fin_buf;    // buffer holding e after post-smoothing, per depth
     Table t_bottom, t_l0, t_res;  // t_res: level 0, at most kReduceBlocks blocks per box
+    // one-workgroup coarse V-cycle: per box, levels dc..nlev-1 (cc_first[b] ..
+    // cc_first[b+1]-1 of table t_cc) and the LDS it needs per precision
+    Table t_cc;
+    int* cc_first = nullptr;
+    size_t cc_lds[2] = {0, 0};
     Table t_resz;                 // level 0, z-marching residual columns
     BoxSeg* pseg = nullptr;
     int pseg_n = 0, pseg_blocks = 0;
@@ -446,12 +453,23 @@ int build_tables(nyxpm_ctx* c) {
     static const int shapes64[][2] = {{64, 16}, {64, 8}, {32, 8}, {16, 8}};
     static const int shapes32[][2] = {{128, 8}, {64, 16}, {64, 8}, {32, 8}, {16, 8}};
     for (int pr = 0; pr < 2; ++pr) c->sw[pr].assign(c->dmax, {});
+    // levels below the first with at most kCoarseCells cells run in one workgroup
+    for (int b = 0; b < nbox; ++b) {
+        Box& B = c->boxes[b];
+        B.dc = B.nlev - 1;
+        if (!c->per_level_coarse)
+            for (int l = 0; l < B.nlev; ++l)
+                if (B.lev[l].nc <= kCoarseCells) {
+                    B.dc = l;
+                    break;
+                }
+    }
     std::vector<bool> fusable(c->dmax, false);
     for (int d = 0; d + 1 < c->dmax; ++d) {
         bool any = false, even = true;
         for (int b = 0; b < nbox; ++b) {
             const Box& B = c->boxes[b];
-            if (d + 1 >= B.nlev) continue;
+            if (d >= B.dc) continue;
             any = true;
             const Level& L = B.lev[d];
             if (L.n[0] % 2 || L.n[1] % 2 || L.n[2] % 2) even = false;
@@ -466,7 +484,7 @@ int build_tables(nyxpm_ctx* c) {
                 bool ok = true;
                 for (int b = 0; b < nbox && ok; ++b) {
                     const Box& B = c->boxes[b];
-                    if (d + 1 >= B.nlev) continue;
+                    if (d >= B.dc) continue;
                     const Level& L = B.lev[d];
                     if (L.n[0] % tx || L.n[1] % ty) ok = false;
                     // the wide shapes only where they still give >= 64 columns
@@ -492,19 +510,19 @@ int build_tables(nyxpm_ctx* c) {
             c->fin_buf[d] = cur;
         }
     }
-    // a box solved entirely by the bottom solve has no finest-level sweep to carry
-    // the phi update
+    // a box whose whole V-cycle runs in the coarse workgroup has no finest-level
+    // sweep to carry the phi update
     c->fuse_update = true;
     for (int b = 0; b < nbox; ++b)
-        if (c->boxes[b].nlev == 1) c->fuse_update = false;
+        if (c->boxes[b].dc == 0) c->fuse_update = false;
     auto coarse_buf = [&](const Box& B, int d) {  // buffer of level d+1's result
-        return (d + 1 == B.nlev - 1) ? 0 : c->fin_buf[d + 1];
+        return (d + 1 == B.dc) ? 0 : c->fin_buf[d + 1];
     };
     for (int d = 0; d + 1 < c->dmax; ++d) {
         std::vector<CellSeg> vf, vh, vc, vs[2];
         for (int b = 0; b < nbox; ++b) {
             const Box& B = c->boxes[b];
-            if (d + 1 >= B.nlev) continue;
+            if (d >= B.dc) continue;
             const Level& L = B.lev[d];
             CellSeg f = mk(b, d, L.nc);
             f.ebuf = coarse_buf(B, d);
@@ -536,11 +554,34 @@ int build_tables(nyxpm_ctx* c) {
         c->t_bottom = add_table(vb);
     }
     {
+        std::vector<CellSeg> vcc;
+        std::vector<int> firsts;
+        c->cc_lds[0] = c->cc_lds[1] = 0;
+        for (int b = 0; b < nbox; ++b) {
+            const Box& B = c->boxes[b];
+            firsts.push_back((int)vcc.size());
+            long long cells = 0;
+            for (int l = B.dc; l < B.nlev; ++l) {
+                CellSeg s = mk(b, l, 1);
+                if (l == B.nlev - 1) s.coff = B.G_off;
+                vcc.push_back(s);
+                cells += B.lev[l].nc * (l == B.dc ? 1 : 2);  // e of every level, r below the top
+            }
+            for (int pr = 0; pr < 2; ++pr)
+                c->cc_lds[pr] = std::max(c->cc_lds[pr], (size_t)cells * (pr == 0 ? 8 : 4));
+        }
+        firsts.push_back((int)vcc.size());
+        c->t_cc = add_table(vcc);
+        int rc0 = dalloc(c, &c->cc_first, firsts.size());
+        if (rc0) return rc0;
+        HIPCK(c, hipMemcpy(c->cc_first, firsts.data(), firsts.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+    {
         std::vector<CellSeg> v0;
         for (int b = 0; b < nbox; ++b) {
             CellSeg s = mk(b, 0, c->boxes[b].lev[0].nc);
             s.coff = c->boxes[b].cell_off;  // level-0 field pools
-            s.ebuf = c->boxes[b].nlev == 1 ? 0 : c->fin_buf[0];
+            s.ebuf = c->boxes[b].dc == 0 ? 0 : c->fin_buf[0];
             v0.push_back(s);
         }
         c->t_l0 = add_table(v0);
@@ -778,7 +819,8 @@ int phase_sort(nyxpm_ctx* c) {
     return NYXPM_OK;
 }
 
-int phase_deposit(nyxpm_ctx* c) {
+// want_rho: also materialise the density (AssignDensity); the step needs only f.
+int phase_deposit(nyxpm_ctx* c, bool want_rho) {
     if (!c->sorted_valid) {
         int rc;
         if (!c->keys_valid && (rc = phase_keys(c))) return rc;
@@ -793,11 +835,16 @@ int phase_deposit(nyxpm_ctx* c) {
     }
     {
         Timed t(c, "fold");
-        hipLaunchKernelGGL(k_fold, dim3(c->t_res.blocks), dim3(kBlock), 0, c->stream, c->segs(c->t_res),
-                           c->t_res.count, c->dboxes, c->own, c->halo, c->rho, c->f, c->partial);
+        if (want_rho)
+            hipLaunchKernelGGL(k_fold<true>, dim3(c->t_res.blocks), dim3(kBlock), 0, c->stream, c->segs(c->t_res),
+                               c->t_res.count, c->dboxes, c->own, c->halo, c->rho, c->f, c->partial);
+        else
+            hipLaunchKernelGGL(k_fold<false>, dim3(c->t_res.blocks), dim3(kBlock), 0, c->stream,
+                               c->segs(c->t_res), c->t_res.count, c->dboxes, c->own, c->halo, c->rho, c->f,
+                               c->partial);
     }
     LAUNCHCK(c);
-    c->density_valid = true;
+    c->density_valid = want_rho;
     c->f1_valid = true;
     return NYXPM_OK;
 }
@@ -904,10 +951,15 @@ int vcycle(nyxpm_ctx* c, bool* updated) {
                                tc.count, c->active, eb[cur], r);
         }
     }
-    {
+    if (c->per_level_coarse) {
         Timed t(c, "bottom");
         hipLaunchKernelGGL(k_bottom<T>, dim3(c->t_bottom.count), dim3(kBlock), 0, c->stream, c->segs(c->t_bottom),
                            c->t_bottom.count, c->active, eb[0], r, Pools<T>::G(c));
+    } else {
+        Timed t(c, "coarse");
+        hipLaunchKernelGGL(k_coarse_vcycle<T>, dim3((unsigned)c->boxes.size()), dim3(kCoarseThreads),
+                           c->cc_lds[sizeof(T) == 8 ? 0 : 1], c->stream, c->segs(c->t_cc), c->cc_first, c->active,
+                           eb[0], r, Pools<T>::G(c), c->nu1, c->nu2);
     }
     for (int d = D - 2; d >= 0; --d) {
         auto& tf = c->t_full[d];
@@ -1124,7 +1176,7 @@ int step_body(nyxpm_ctx* c, double dt, double a_old, double a_new, double four_p
         // Nyx_advance_particles.cpp:148-170: old phi re-solved from a zero guess
         if ((rc = phase_keys(c))) return rc;
         if ((rc = phase_sort(c))) return rc;
-        if ((rc = phase_deposit(c))) return rc;
+        if ((rc = phase_deposit(c, false))) return rc;
         if ((rc = phase_rhs(c, a_old, four_pi_G))) return rc;
         hipLaunchKernelGGL(k_fill_dbl, dim3(nblocks(c->ncell_total)), dim3(kBlock), 0, c->stream, c->phi,
                            c->ncell_total, 0.0);
@@ -1133,7 +1185,7 @@ int step_body(nyxpm_ctx* c, double dt, double a_old, double a_new, double four_p
     }
     if ((rc = phase_kick<kKickDrift>(c, dt, a_old, a_half))) return rc;
     if ((rc = phase_sort(c))) return rc;
-    if ((rc = phase_deposit(c))) return rc;
+    if ((rc = phase_deposit(c, false))) return rc;
     if ((rc = phase_rhs(c, a_new, four_pi_G))) return rc;
     if ((rc = solve(c, rel, abs_tol, 0, graph_cycles))) return rc;
     if ((rc = phase_kick<kKick>(c, dt, a_half, a_new))) return rc;
@@ -1148,7 +1200,7 @@ int step_tail(nyxpm_ctx* c, double dt, double a_old, double a_new, double four_p
     int rc;
     const double a_half = 0.5 * (a_old + a_new);
     if ((rc = phase_sort(c))) return rc;
-    if ((rc = phase_deposit(c))) return rc;
+    if ((rc = phase_deposit(c, false))) return rc;
     if ((rc = phase_rhs(c, a_new, four_pi_G))) return rc;
     if ((rc = solve(c, rel, abs_tol, 0, graph_cycles))) return rc;
     if (pair_next) return phase_kick<kKickPair>(c, dt, a_half, a_new, a_old, a_half);
@@ -1179,6 +1231,7 @@ int nyxpm_create_batch(nyxpm_ctx** out, int nbox, const nyxpm_geom* geoms, const
         c->no_fused = opts->smoother == 1;
         c->untiled_kick = opts->kick == 1;
         c->separate_kicks = opts->separate_kicks != 0;
+        c->per_level_coarse = opts->coarse == 1;
         if (opts->sweep_tx > 0) {
             c->force_tx[0] = c->force_tx[1] = opts->sweep_tx;
             c->force_ty[0] = c->force_ty[1] = opts->sweep_ty > 0 ? opts->sweep_ty : 8;
@@ -1220,7 +1273,7 @@ int nyxpm_destroy(nyxpm_ctx* c) {
     dfree(c->r32); dfree(c->e32); dfree(c->e32b); dfree(c->G32);
     dfree(c->active); dfree(c->iters); dfree(c->failed); dfree(c->err_flag); dfree(c->nitems); dfree(c->nr); dfree(c->nf);
     dfree(c->state); dfree(c->solv); dfree(c->dv_size); dfree(c->dboxes);
-    dfree(c->seg_pool); dfree(c->pseg); dfree(c->tseg);
+    dfree(c->seg_pool); dfree(c->pseg); dfree(c->tseg); dfree(c->cc_first);
     if (c->h_flags) hipHostFree(c->h_flags);
     if (c->h_state) hipHostFree(c->h_state);
     for (auto& kv : c->pending)
@@ -1435,7 +1488,7 @@ int nyxpm_assign_density(nyxpm_ctx* c, int box, double* rho_out) {
     if ((rc = require_particles(c))) return rc;
     if (box >= 0 && (rc = check_box(c, box))) return rc;
     if (rho_out && box < 0) FAIL(c, NYXPM_ERR_ARG, "rho_out needs a box index");
-    if ((rc = phase_deposit(c))) return rc;
+    if ((rc = phase_deposit(c, true))) return rc;
     if (rho_out) {
         const Box& B = c->boxes[box];
         HIPCK(c, hipMemcpyAsync(rho_out, c->rho + B.cell_off, B.lev[0].nc * sizeof(double), hipMemcpyDeviceToHost,
@@ -1596,7 +1649,7 @@ int nyxpm_pm_init(nyxpm_ctx* c, double a, double four_pi_G, double rel_tol, doub
     if (!c) return NYXPM_ERR_ARG;
     int rc;
     if ((rc = require_particles(c))) return rc;
-    if ((rc = phase_deposit(c))) return rc;
+    if ((rc = phase_deposit(c, true))) return rc;
     if ((rc = phase_rhs(c, a, four_pi_G))) return rc;
     hipLaunchKernelGGL(k_fill_dbl, dim3(nblocks(c->ncell_total)), dim3(kBlock), 0, c->stream, c->phi,
                        c->ncell_total, 0.0);

@@ -296,6 +296,11 @@ __device__ __forceinline__ double block_sum_fixed(double v) {
 
 // rho = (own + halos of the up-to-7 upper neighbour tiles, fixed order) * 1/vol;
 // f = rho + (-mass_offset); per-block sum of f for the solvability correction.
+// A block covers R = max(1, kBlock / nx) consecutive x-rows per iteration (thread
+// t: row offset t / nx, x = t % nx, striding by kBlock along x when nx > kBlock),
+// so all index arithmetic is per row and 32-bit.  rho itself is written only when
+// the caller wants the density (AssignDensity); the step needs only f.
+template <bool WRITE_RHO>
 __global__ void __launch_bounds__(kBlock) k_fold(const CellSeg* segs, int nseg, const BoxDev* boxes,
                                                  const double* __restrict__ own, const double* __restrict__ halo,
                                                  double* __restrict__ rho, double* __restrict__ f,
@@ -303,39 +308,53 @@ __global__ void __launch_bounds__(kBlock) k_fold(const CellSeg* segs, int nseg, 
     const CellSeg& sg = segs[find_seg(segs, nseg, blockIdx.x)];
     const BoxDev& B = boxes[sg.box];
     const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
-    double acc = 0.0;  // this thread's cells, in increasing order
-    for (long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x; q < (long long)nx * ny * nz;
-         q += (long long)sg.nblk * kBlock) {
-        const int i = (int)(q % nx);
-        const long long t = q / nx;
-        const int j = (int)(t % ny), k = (int)(t / ny);
-        const int T0 = B.tile[0], T1 = B.tile[1], T2 = B.tile[2];
-        const int tx = i / T0, ty = j / T1, tz = k / T2;
-        const int li = i - tx * T0, lj = j - ty * T1, lk = k - tz * T2;
-        const int bx = min(T0, nx - tx * T0), by = min(T1, ny - ty * T1), bz = min(T2, nz - tz * T2);
-        const bool topx = li == bx - 1, topy = lj == by - 1, topz = lk == bz - 1;
-        double v = own[B.cell_off + q];
-        // subsets of {x,y,z}: x, y, z, xy, xz, yz, xyz
-        const int sel[7] = {1, 2, 4, 3, 5, 6, 7};
+    const int T0 = B.tile[0], T1 = B.tile[1], T2 = B.tile[2];
+    const int R = nx >= kBlock ? 1 : kBlock / nx;
+    const int roff = nx >= kBlock ? 0 : (int)threadIdx.x / nx;
+    const int i0 = nx >= kBlock ? (int)threadIdx.x : (int)threadIdx.x - roff * nx;
+    const bool lane_on = roff < R;
+    const int nrow = ny * nz;
+    double acc = 0.0;  // this thread's cells, in a fixed order
+    for (int r0 = (int)(blockIdx.x - sg.blk0) * R; r0 < nrow; r0 += sg.nblk * R) {
+        const int r = r0 + roff;
+        if (!lane_on || r >= nrow) continue;
+        const int j = r % ny, k = r / ny;
+        const int ty = j / T1, tz = k / T2;
+        const int lj = j - ty * T1, lk = k - tz * T2;
+        const int by = min(T1, ny - ty * T1), bz = min(T2, nz - tz * T2);
+        const bool topy = lj == by - 1, topz = lk == bz - 1;
+        const long long row = B.cell_off + (long long)r * nx;
+        for (int i = i0; i < nx; i += kBlock) {
+            const int tx = i / T0;
+            const int li = i - tx * T0;
+            const int bx = min(T0, nx - tx * T0);
+            const bool topx = li == bx - 1;
+            double v = own[row + i];
+            if (topx || topy || topz) {
+                // subsets of {x,y,z}: x, y, z, xy, xz, yz, xyz
+                const int sel[7] = {1, 2, 4, 3, 5, 6, 7};
 #pragma unroll
-        for (int s = 0; s < 7; ++s) {
-            const int S = sel[s];
-            const bool sx = S & 1, sy = S & 2, sz = S & 4;
-            if ((sx && !topx) || (sy && !topy) || (sz && !topz)) continue;
-            int ntx = tx + (sx ? 1 : 0), nty = ty + (sy ? 1 : 0), ntz = tz + (sz ? 1 : 0);
-            if (ntx == B.ntile[0]) ntx = 0;
-            if (nty == B.ntile[1]) nty = 0;
-            if (ntz == B.ntile[2]) ntz = 0;
-            const int nbx = min(T0, nx - ntx * T0), nby = min(T1, ny - nty * T1), nbz = min(T2, nz - ntz * T2);
-            const int hx = sx ? 0 : li + 1, hy = sy ? 0 : lj + 1, hz = sz ? 0 : lk + 1;
-            const long long nt = B.tile_off + ((long long)ntz * B.ntile[1] + nty) * B.ntile[0] + ntx;
-            v += halo[nt * B.halo_stride + halo_index(hx, hy, hz, nbx, nby, nbz)];
+                for (int s = 0; s < 7; ++s) {
+                    const int S = sel[s];
+                    const bool sx = S & 1, sy = S & 2, sz = S & 4;
+                    if ((sx && !topx) || (sy && !topy) || (sz && !topz)) continue;
+                    int ntx = tx + (sx ? 1 : 0), nty = ty + (sy ? 1 : 0), ntz = tz + (sz ? 1 : 0);
+                    if (ntx == B.ntile[0]) ntx = 0;
+                    if (nty == B.ntile[1]) nty = 0;
+                    if (ntz == B.ntile[2]) ntz = 0;
+                    const int nbx = min(T0, nx - ntx * T0), nby = min(T1, ny - nty * T1),
+                              nbz = min(T2, nz - ntz * T2);
+                    const int hx = sx ? 0 : li + 1, hy = sy ? 0 : lj + 1, hz = sz ? 0 : lk + 1;
+                    const long long nt = B.tile_off + ((long long)ntz * B.ntile[1] + nty) * B.ntile[0] + ntx;
+                    v += halo[nt * B.halo_stride + halo_index(hx, hy, hz, nbx, nby, nbz)];
+                }
+            }
+            const double rv = v * B.inv_vol;
+            if (WRITE_RHO) rho[row + i] = rv;
+            const double fv = rv + (-B.mass_offset);
+            f[row + i] = fv;
+            acc += fv;
         }
-        const double rv = v * B.inv_vol;
-        rho[B.cell_off + q] = rv;
-        const double fv = rv + (-B.mass_offset);
-        f[B.cell_off + q] = fv;
-        acc += fv;
     }
     const double bs = block_sum_fixed(acc);
     if (threadIdx.x == 0) partial[blockIdx.x] = bs;

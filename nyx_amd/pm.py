@@ -48,7 +48,8 @@ class PMContext:
     """One library context: a batch of independent periodic boxes on one GPU."""
 
     def __init__(self, geoms, precision="fp64", tile=None, nu1=2, nu2=2, max_iter=200, smoother="fused",
-                 kick="tiled", sweep_shape=None, sweep_zchunk=0, kick_chunk=0, pair_kicks=True):
+                 kick="tiled", sweep_shape=None, sweep_zchunk=0, kick_chunk=0, pair_kicks=True,
+                 coarse="workgroup"):
         if isinstance(geoms, BoxGeom):
             geoms = [geoms]
         self.geoms = list(geoms)
@@ -69,6 +70,7 @@ class PMContext:
         o.sweep_zchunk = int(sweep_zchunk)
         o.kick_chunk = int(kick_chunk)
         o.separate_kicks = 0 if pair_kicks else 1
+        o.coarse = 1 if coarse == "per_level" else 0
         if tile is not None:
             for d in range(3):
                 o.tile[d] = int(tile[d])

@@ -144,3 +144,35 @@ def test_pair_kicks_equal_separate(nyx, precision, steps, use_graph):
         for d in range(3):
             assert np.array_equal(sep[b][0][d], fus[b][0][d])
             assert np.array_equal(sep[b][1][d], fus[b][1][d])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_coarse_workgroup_equals_per_level(nyx, precision):
+    # the one-workgroup coarse V-cycle (levels of <= 4096 cells down to the bottom)
+    # gives the same bits as one launch per level; 16^3 runs its whole cycle there,
+    # 48x24x40 reaches it at 12x6x10 above an odd 6x3x5 bottom
+    geoms = [nyx.BoxGeom((16, 16, 16)), nyx.BoxGeom((32, 32, 32)),
+             nyx.BoxGeom((48, 24, 40), (0.0, 0.0, 0.0), (1.0, 0.5, 40 / 48)), nyx.BoxGeom((96, 96, 96))]
+    inputs = []
+    for i, g in enumerate(geoms):
+        r = np.random.default_rng(600 + i)
+        npart = g.ncell
+        pos = [r.uniform(g.prob_lo[d], g.prob_hi[d], npart) for d in range(3)]
+        vel = [r.normal(0.0, 0.5, npart) for _ in range(3)]
+        inputs.append((pos, vel, None, 1.0 / npart))
+    outs = []
+    for coarse in ("per_level", "workgroup"):
+        with nyx.PMContext(geoms, precision=precision, coarse=coarse) as c:
+            for b, inp in enumerate(inputs):
+                _upload(c, b, inp)
+            c.pm_init(1.0, 1.0, 1e-10)
+            its = c.pm_steps(2, 1e-3, rel_tol=1e-10)
+            outs.append(([c.download(b) for b in range(len(geoms))], [c.get_phi(b) for b in range(len(geoms))],
+                         list(its)))
+    (a, aphi, ait), (b_, bphi, bit) = outs
+    assert ait == bit
+    for b in range(len(geoms)):
+        assert np.array_equal(aphi[b], bphi[b])
+        for d in range(3):
+            assert np.array_equal(a[b][0][d], b_[b][0][d])
+            assert np.array_equal(a[b][1][d], b_[b][1][d])
