@@ -159,6 +159,8 @@ struct nyxpm_ctx {
     int scan_blocks = 0;
     TileItem* items = nullptr;  // tile chunks of the last sort (tile-staged kernels)
     int* nitems = nullptr;
+    long long* defer = nullptr;  // particles handed from the tile-staged kick to k_kick_deferred
+    int* ndefer = nullptr;
     long long items_cap = 0;
     bool items_valid = false;
 
@@ -333,6 +335,8 @@ int setup_boxes(nyxpm_ctx* c, int nbox, const nyxpm_geom* geoms) {
         const nyxpm_geom& g = geoms[b];
         for (int d = 0; d < 3; ++d) {
             if (g.n_cell[d] < 2) FAIL(c, NYXPM_ERR_ARG, "box %d: n_cell[%d]=%d must be >= 2", b, d, g.n_cell[d]);
+            if (g.n_cell[d] > 65536)
+                FAIL(c, NYXPM_ERR_UNSUPPORTED, "box %d: n_cell[%d]=%d above 65536", b, d, g.n_cell[d]);
             if (!(g.prob_hi[d] > g.prob_lo[d])) FAIL(c, NYXPM_ERR_ARG, "box %d: prob_hi <= prob_lo", b);
             B.n[d] = g.n_cell[d];
             B.plo[d] = g.prob_lo[d];
@@ -639,6 +643,7 @@ int upload_boxdev(nyxpm_ctx* c) {
             D.len[d] = B.len[d];
         }
         D.fx_exp = B.fx_exp;
+        for (int d = 0; d < 3; ++d) D.tdiv[d] = (unsigned)(((1ull << 31) + B.tile[d] - 1) / B.tile[d]);
         D.cell_off = B.cell_off;
         D.tile_off = B.tile_off;
         D.part_off = B.part_off;
@@ -702,6 +707,7 @@ int alloc_fields(nyxpm_ctx* c) {
     if ((rc = dalloc(c, &c->failed, nbox))) return rc;
     if ((rc = dalloc(c, &c->err_flag, 1))) return rc;
     if ((rc = dalloc(c, &c->nitems, 1))) return rc;
+    if ((rc = dalloc(c, &c->ndefer, 1))) return rc;
     if ((rc = dalloc(c, &c->nr, (size_t)nbox * kNormSlots))) return rc;
     if ((rc = dalloc(c, &c->nf, (size_t)nbox * kNormSlots))) return rc;
     if ((rc = dalloc(c, &c->state, 4 * nbox))) return rc;
@@ -744,6 +750,7 @@ void free_particles(nyxpm_ctx* c) {
     }
     dfree(c->key);
     dfree(c->items);
+    dfree(c->defer);
 }
 
 int alloc_particle_set(nyxpm_ctx* c, PartPtrs& P, long long n, bool mass) {
@@ -1122,9 +1129,13 @@ int phase_kick(nyxpm_ctx* c, double dt, double a_prev, double a_cur, double a_pr
         if (MOVE)
             hipLaunchKernelGGL(k_fill_int, dim3(nblocks(c->ntiles_total)), dim3(kBlock), 0, c->stream, c->counts,
                                c->ntiles_total, 0);
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(kBlock), 0, c->stream, c->ndefer, 1LL, 0);
         hipLaunchKernelGGL(k_kick_tiled<MODE>, dim3((unsigned)c->items_cap), dim3(kBlock), lds, c->stream, c->tseg,
                            c->tseg_n, c->dboxes, c->items, c->nitems, c->phi, A.x, A.y, A.z, A.vx, A.vy, A.vz, dt,
-                           a_prev, a_cur, a_prev2, a_cur2, c->key, c->counts, c->err_flag);
+                           a_prev, a_cur, a_prev2, a_cur2, c->key, c->counts, c->defer, c->ndefer, c->err_flag);
+        hipLaunchKernelGGL(k_kick_deferred<MODE>, dim3(256), dim3(kBlock), 0, c->stream, c->defer, c->ndefer,
+                           c->dboxes, c->phi, A.x, A.y, A.z, A.vx, A.vy, A.vz, dt, a_prev, a_cur, a_prev2, a_cur2,
+                           c->key, c->counts, c->err_flag);
         LAUNCHCK(c);
         c->keys_valid = MOVE;
         c->sorted_valid = false;
@@ -1272,6 +1283,7 @@ int nyxpm_destroy(nyxpm_ctx* c) {
     dfree(c->r64); dfree(c->e64); dfree(c->e64b); dfree(c->G64);
     dfree(c->r32); dfree(c->e32); dfree(c->e32b); dfree(c->G32);
     dfree(c->active); dfree(c->iters); dfree(c->failed); dfree(c->err_flag); dfree(c->nitems); dfree(c->nr); dfree(c->nf);
+    dfree(c->ndefer);
     dfree(c->state); dfree(c->solv); dfree(c->dv_size); dfree(c->dboxes);
     dfree(c->seg_pool); dfree(c->pseg); dfree(c->tseg); dfree(c->cc_first);
     if (c->h_flags) hipHostFree(c->h_flags);
@@ -1396,12 +1408,15 @@ static int upload_impl(nyxpm_ctx* c, int box, int64_t np, const double* x, const
     if (icap > 0x7fffffffLL) FAIL(c, NYXPM_ERR_ARG, "too many tile chunks");
     TileItem* nitm = nullptr;
     if ((rc = dalloc(c, &nitm, icap))) return rc;
+    long long* ndef = nullptr;
+    if ((rc = dalloc(c, &ndef, std::max(tot, 1LL)))) return rc;
     free_particles(c);
     c->P[0] = N[0];
     c->P[1] = N[1];
     c->key = nkey;
     c->items = nitm;
     c->items_cap = icap;
+    c->defer = ndef;
     c->items_valid = false;
     c->cur = 0;
     c->part_cap = cap;

@@ -42,6 +42,7 @@ struct BoxDev {
     double m_const;        // mass of every particle when has_mass == 0
     int has_mass;
     int halo_stride;       // halo slots per tile
+    unsigned tdiv[3];      // ceil(2^31 / tile[d]): i / tile[d] == (i * tdiv[d]) >> 31 for 0 <= i < 2^17
 };
 
 // A (box, level) piece of a launch over cells.  blk0..blk0+nblk-1 are the

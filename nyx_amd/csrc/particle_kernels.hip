@@ -63,12 +63,18 @@ __device__ __forceinline__ int agg_atomic_add(int* counters, int key, bool valid
     return result;
 }
 
+// i / T for 0 <= i < 2^17 and T <= 32 with magic = ceil(2^31 / T) (exact over that
+// range; the bound is checked when a context is created).
+__device__ __forceinline__ int tile_div(int i, unsigned magic) {
+    return (int)(((unsigned long long)(unsigned)i * magic) >> 31);
+}
+
 __device__ __forceinline__ int tile_of(const BoxDev& B, double x, double y, double z, int* err) {
     double f;
     const int i = wrap_checked(cic_index(x, B.plo[0], B.dxi[0], &f), B.n[0], err);
     const int j = wrap_checked(cic_index(y, B.plo[1], B.dxi[1], &f), B.n[1], err);
     const int k = wrap_checked(cic_index(z, B.plo[2], B.dxi[2], &f), B.n[2], err);
-    const int tx = i / B.tile[0], ty = j / B.tile[1], tz = k / B.tile[2];
+    const int tx = tile_div(i, B.tdiv[0]), ty = tile_div(j, B.tdiv[1]), tz = tile_div(k, B.tdiv[2]);
     return (int)(B.tile_off + ((long long)tz * B.ntile[1] + ty) * B.ntile[0] + tx);
 }
 
@@ -239,22 +245,24 @@ __global__ void __launch_bounds__(kBlock) k_deposit(const BoxSeg* segs, int nseg
     const long long gt = B.tile_off + lt;
     const long long p0 = start[gt], p1 = start[gt + 1];
     const int E = B.fx_exp;
-    for (long long p = p0 + threadIdx.x; p < p1; p += kBlock) {
+    // one particle's CIC contributions as two fixed-point words: w * 2^(31-E) =
+    // hi + lo * 2^-32 (hi < 2^32 since w < 2^(E+1); lo rounded, a rounding up to
+    // 2^32 carried into hi), both converted with the native f64 -> u32 instruction
+    auto dep = [&](double px, double py, double pz, double mp) {
         double xint, yint, zint;
-        const int i = cic_index(x[p], B.plo[0], B.dxi[0], &xint);
-        const int j = cic_index(y[p], B.plo[1], B.dxi[1], &yint);
-        const int k = cic_index(z[p], B.plo[2], B.dxi[2], &zint);
+        const int i = cic_index(px, B.plo[0], B.dxi[0], &xint);
+        const int j = cic_index(py, B.plo[1], B.dxi[1], &yint);
+        const int k = cic_index(pz, B.plo[2], B.dxi[2], &zint);
         const int iw = wrap_checked(i, B.n[0], err) - o0;
         const int jw = wrap_checked(j, B.n[1], err) - o1;
         const int kw = wrap_checked(k, B.n[2], err) - o2;
         if (iw < 0 || iw >= bx || jw < 0 || jw >= by || kw < 0 || kw >= bz) {
             atomicOr(err, 2);
-            continue;
+            return;
         }
         const double sx[2] = {1.0 - xint, xint};
         const double sy[2] = {1.0 - yint, yint};
         const double sz[2] = {1.0 - zint, zint};
-        const double mp = B.has_mass ? m[p] : B.m_const;
 #pragma unroll
         for (int kk = 0; kk < 2; ++kk)
 #pragma unroll
@@ -264,12 +272,28 @@ __global__ void __launch_bounds__(kBlock) k_deposit(const BoxSeg* segs, int nseg
                     const double w = sx[ii] * sy[jj] * sz[kk] * mp;
                     const double s = ldexp(w, 31 - E);
                     const double hf = floor(s);
-                    const unsigned long long hi = (unsigned long long)hf;
-                    const unsigned long long lo = (unsigned long long)rint(ldexp(s - hf, 32));
+                    const double lr = rint(ldexp(s - hf, 32));
+                    const bool carry = lr >= 4294967296.0;
+                    const unsigned long long hi = (unsigned long long)(unsigned)hf + (carry ? 1ull : 0ull);
+                    const unsigned long long lo = carry ? 0ull : (unsigned long long)(unsigned)lr;
                     const int idx = ((kw + kk) * Ly + (jw + jj)) * Lx + (iw + ii);
                     atomicAdd(&acc[idx], hi);
                     atomicAdd(&acc[LC + idx], lo);
                 }
+    };
+    // two particles per thread, both loaded before either is deposited
+    for (long long pa = p0 + threadIdx.x; pa < p1; pa += 2 * kBlock) {
+        const long long pb = pa + kBlock;
+        const bool hb = pb < p1;
+        const double xa = x[pa], ya = y[pa], za = z[pa];
+        const double ma = B.has_mass ? m[pa] : B.m_const;
+        double xb = 0.0, yb = 0.0, zb = 0.0, mb = 0.0;
+        if (hb) {
+            xb = x[pb]; yb = y[pb]; zb = z[pb];
+            mb = B.has_mass ? m[pb] : B.m_const;
+        }
+        dep(xa, ya, za, ma);
+        if (hb) dep(xb, yb, zb, mb);
     }
     __syncthreads();
     for (int c = threadIdx.x; c < LC; c += kBlock) {
@@ -296,10 +320,11 @@ __device__ __forceinline__ double block_sum_fixed(double v) {
 
 // rho = (own + halos of the up-to-7 upper neighbour tiles, fixed order) * 1/vol;
 // f = rho + (-mass_offset); per-block sum of f for the solvability correction.
-// A block covers R = max(1, kBlock / nx) consecutive x-rows per iteration (thread
-// t: row offset t / nx, x = t % nx, striding by kBlock along x when nx > kBlock),
-// so all index arithmetic is per row and 32-bit.  rho itself is written only when
-// the caller wants the density (AssignDensity); the step needs only f.
+// Tile-centric: block b of a box takes a contiguous range of tiles (so the two
+// halves of a 128-byte line of own/f, and the halo arrays of the x neighbours,
+// are touched by the same block close in time), one thread per cell of the tile.
+// rho itself is written only when the caller wants the density (AssignDensity);
+// the step needs only f.
 template <bool WRITE_RHO>
 __global__ void __launch_bounds__(kBlock) k_fold(const CellSeg* segs, int nseg, const BoxDev* boxes,
                                                  const double* __restrict__ own, const double* __restrict__ halo,
@@ -309,27 +334,21 @@ __global__ void __launch_bounds__(kBlock) k_fold(const CellSeg* segs, int nseg, 
     const BoxDev& B = boxes[sg.box];
     const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
     const int T0 = B.tile[0], T1 = B.tile[1], T2 = B.tile[2];
-    const int R = nx >= kBlock ? 1 : kBlock / nx;
-    const int roff = nx >= kBlock ? 0 : (int)threadIdx.x / nx;
-    const int i0 = nx >= kBlock ? (int)threadIdx.x : (int)threadIdx.x - roff * nx;
-    const bool lane_on = roff < R;
-    const int nrow = ny * nz;
+    const int nt0 = B.ntile[0], nt1 = B.ntile[1], nt2 = B.ntile[2];
+    const int ntiles = nt0 * nt1 * nt2;
+    const int per = (ntiles + sg.nblk - 1) / sg.nblk;
+    const int t_begin = (blockIdx.x - sg.blk0) * per;
+    const int t_end = min(ntiles, t_begin + per);
     double acc = 0.0;  // this thread's cells, in a fixed order
-    for (int r0 = (int)(blockIdx.x - sg.blk0) * R; r0 < nrow; r0 += sg.nblk * R) {
-        const int r = r0 + roff;
-        if (!lane_on || r >= nrow) continue;
-        const int j = r % ny, k = r / ny;
-        const int ty = j / T1, tz = k / T2;
-        const int lj = j - ty * T1, lk = k - tz * T2;
-        const int by = min(T1, ny - ty * T1), bz = min(T2, nz - tz * T2);
-        const bool topy = lj == by - 1, topz = lk == bz - 1;
-        const long long row = B.cell_off + (long long)r * nx;
-        for (int i = i0; i < nx; i += kBlock) {
-            const int tx = i / T0;
-            const int li = i - tx * T0;
-            const int bx = min(T0, nx - tx * T0);
-            const bool topx = li == bx - 1;
-            double v = own[row + i];
+    for (int lt = t_begin; lt < t_end; ++lt) {
+        const int tx = lt % nt0, ty = (lt / nt0) % nt1, tz = lt / (nt0 * nt1);
+        const int bx = min(T0, nx - tx * T0), by = min(T1, ny - ty * T1), bz = min(T2, nz - tz * T2);
+        const int ncel = bx * by * bz;
+        for (int c = threadIdx.x; c < ncel; c += kBlock) {
+            const int li = c % bx, lj = (c / bx) % by, lk = c / (bx * by);
+            const bool topx = li == bx - 1, topy = lj == by - 1, topz = lk == bz - 1;
+            const size_t q = B.cell_off + cidx(tx * T0 + li, ty * T1 + lj, tz * T2 + lk, nx, ny);
+            double v = own[q];
             if (topx || topy || topz) {
                 // subsets of {x,y,z}: x, y, z, xy, xz, yz, xyz
                 const int sel[7] = {1, 2, 4, 3, 5, 6, 7};
@@ -339,20 +358,20 @@ __global__ void __launch_bounds__(kBlock) k_fold(const CellSeg* segs, int nseg, 
                     const bool sx = S & 1, sy = S & 2, sz = S & 4;
                     if ((sx && !topx) || (sy && !topy) || (sz && !topz)) continue;
                     int ntx = tx + (sx ? 1 : 0), nty = ty + (sy ? 1 : 0), ntz = tz + (sz ? 1 : 0);
-                    if (ntx == B.ntile[0]) ntx = 0;
-                    if (nty == B.ntile[1]) nty = 0;
-                    if (ntz == B.ntile[2]) ntz = 0;
+                    if (ntx == nt0) ntx = 0;
+                    if (nty == nt1) nty = 0;
+                    if (ntz == nt2) ntz = 0;
                     const int nbx = min(T0, nx - ntx * T0), nby = min(T1, ny - nty * T1),
                               nbz = min(T2, nz - ntz * T2);
                     const int hx = sx ? 0 : li + 1, hy = sy ? 0 : lj + 1, hz = sz ? 0 : lk + 1;
-                    const long long nt = B.tile_off + ((long long)ntz * B.ntile[1] + nty) * B.ntile[0] + ntx;
+                    const long long nt = B.tile_off + ((long long)ntz * nt1 + nty) * nt0 + ntx;
                     v += halo[nt * B.halo_stride + halo_index(hx, hy, hz, nbx, nby, nbz)];
                 }
             }
             const double rv = v * B.inv_vol;
-            if (WRITE_RHO) rho[row + i] = rv;
+            if (WRITE_RHO) rho[q] = rv;
             const double fv = rv + (-B.mass_offset);
-            f[row + i] = fv;
+            f[q] = fv;
             acc += fv;
         }
     }
@@ -608,18 +627,31 @@ constexpr int kKick = 0;       // moveKick: kick, then the periodic shift of Red
 constexpr int kKickDrift = 1;  // moveKickDrift: kick, drift, new tile key
 constexpr int kKickPair = 2;   // moveKick of one step followed by moveKickDrift of the next
 
-// Tile-staged gather + kick(-drift): phi over the tile plus a halo of 2 is
-// staged in LDS once per item and g (the average of the face gradients) is
-// formed there on the (tile+1)^3 cells that the CIC stencils of the tile's
-// particles touch; particles whose cell index falls outside the tile (binning
-// older than a periodic shift) take the global-memory path.  Same arithmetic as
-// k_kick, bit for bit.
+// Particles the tile-staged kernel hands over to k_kick_deferred: index (bits
+// 0-39), box (bits 40-59) and what is left to do (bit 62: 0 = the whole update of
+// the launch, 1 = the kick-drift half of a pair whose kick is already stored).
+constexpr long long kDeferPairKD = 1LL << 62;
+__device__ __forceinline__ long long defer_entry(long long p, int box) {
+    return p | ((long long)box << 40);
+}
+
+// Tile-staged gather + kick(-drift).  phi over the tile plus a halo of 2 is
+// staged in LDS once per item (rows of the staged box spread over the lanes of a
+// wave, so the index arithmetic is per row), and g (the average of the face
+// gradients, the three components of a cell side by side) is formed there on the
+// (tile+1)^3 cells the CIC stencils of the tile's particles touch.  Same
+// arithmetic as k_kick, bit for bit.
 //
 // kKickPair runs the closing kick of step n (a_prev, a_cur) and the opening
 // kick-drift of step n+1 (a_prev2, a_cur2) in one pass: both use g from the same
-// phi at the same position, so the interpolated value is reused unless the
-// periodic shift moved the particle, in which case it is interpolated again at
-// the shifted position (as the separate kernels would).
+// phi at the same position.  A particle moved by the periodic shift in between
+// has its kick stored and its kick-drift deferred (to be interpolated again at
+// the shifted position, as the separate kernels would); so is any particle whose
+// cell falls outside the item's tile (binning older than a shift).  Deferral
+// keeps those rare paths out of this kernel's registers.
+//
+// Tile counts for the next sort: particles that stay in the item's tile are
+// summed per block (one atomic), the others counted individually.
 template <int MODE>
 __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int ntseg, const BoxDev* boxes,
                                                        const TileItem* __restrict__ items, const int* nitems,
@@ -628,12 +660,18 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
                                                        double* __restrict__ vx, double* __restrict__ vy,
                                                        double* __restrict__ vz, double dt, double a_prev,
                                                        double a_cur, double a_prev2, double a_cur2,
-                                                       int* __restrict__ key, int* counts, int* err) {
+                                                       int* __restrict__ key, int* counts,
+                                                       long long* __restrict__ defer, int* ndefer, int* err) {
     extern __shared__ double sphi[];
-    if ((int)blockIdx.x >= *nitems) return;
-    const TileItem it = items[blockIdx.x];
+    __shared__ int s_same[kBlock / kWave];
+    // items are in (roughly) tile order; giving each XCD a contiguous range of them
+    // lets neighbouring tiles share the phi halo lines in that XCD's L2
+    const int n_items = *nitems;
+    if ((int)blockIdx.x >= n_items) return;
+    const TileItem it = items[xcd_block(blockIdx.x, n_items)];
     const int sgi = find_seg(tsegs, ntseg, it.tile);
-    const BoxDev& B = boxes[tsegs[sgi].box];
+    const int box = tsegs[sgi].box;
+    const BoxDev& B = boxes[box];
     const int lt = it.tile - tsegs[sgi].blk0;
     const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
     const int tx = lt % B.ntile[0], ty = (lt / B.ntile[0]) % B.ntile[1], tz = lt / (B.ntile[0] * B.ntile[1]);
@@ -641,145 +679,273 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
     const int bx = min(B.tile[0], nx - o0), by = min(B.tile[1], ny - o1), bz = min(B.tile[2], nz - o2);
     const int SX = bx + 3, SY = by + 3, SZ = bz + 3;
     const double* P = phi + B.cell_off;
-    // the first particle of this thread is loaded before the staging of phi so the
-    // two latencies overlap; later ones are prefetched one iteration ahead
-    long long p = it.p0 + threadIdx.x;
-    double px = 0.0, py = 0.0, pz = 0.0, qx = 0.0, qy = 0.0, qz = 0.0;
-    if ((int)threadIdx.x < it.np) {
-        px = x[p]; py = y[p]; pz = z[p];
-        qx = vx[p]; qy = vy[p]; qz = vz[p];
-    }
-    for (int idx = threadIdx.x; idx < SX * SY * SZ; idx += kBlock) {
-        const int lx = idx % SX, ly = (idx / SX) % SY, lz = idx / (SX * SY);
-        const int gx = wrap1(o0 - 2 + lx, nx), gy = wrap1(o1 - 2 + ly, ny), gz = wrap1(o2 - 2 + lz, nz);
-        sphi[idx] = P[cidx(gx, gy, gz, nx, ny)];
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    // the item's particles through block-uniform base pointers and 32-bit
+    // indices (scalar base + vector offset addressing)
+    double* __restrict__ X = x + it.p0;
+    double* __restrict__ Y = y + it.p0;
+    double* __restrict__ Z = z + it.p0;
+    double* __restrict__ VX = vx + it.p0;
+    double* __restrict__ VY = vy + it.p0;
+    double* __restrict__ VZ = vz + it.p0;
+    int* __restrict__ KEY = key + it.p0;
+    // particles are taken two per thread at a time (q and q + kBlock), both loaded
+    // before they are needed: the first pair before the staging of phi, so all
+    // these latencies overlap
+    double ax = 0.0, ay = 0.0, az = 0.0, aux = 0.0, auy = 0.0, auz = 0.0;
+    double bx_ = 0.0, by_ = 0.0, bz_ = 0.0, bux = 0.0, buy = 0.0, buz = 0.0;
+    auto load2 = [&](int q0) {
+        const int q1 = q0 + kBlock;
+        if (q0 < it.np) {
+            ax = X[q0]; ay = Y[q0]; az = Z[q0];
+            aux = VX[q0]; auy = VY[q0]; auz = VZ[q0];
+        }
+        if (q1 < it.np) {
+            bx_ = X[q1]; by_ = Y[q1]; bz_ = Z[q1];
+            bux = VX[q1]; buy = VY[q1]; buz = VZ[q1];
+        }
+    };
+    load2(threadIdx.x);
+    {
+        // rows of SX cells: a wave takes rpw rows per pass (SX <= 35 < 64)
+        const int rpw = kWave / SX;
+        const int lrow = lane / SX, lx = lane - lrow * SX;
+        if (lrow < rpw) {
+            const size_t gx = (size_t)wrap1(o0 - 2 + lx, nx);
+            const int step = (kBlock / kWave) * rpw;
+            const int r0 = wave * rpw + lrow;
+            int ly = r0 % SY, lz = r0 / SY;
+            const int dly = step % SY, dlz = step / SY;
+            while (lz < SZ) {
+                const int gy = wrap1(o1 - 2 + ly, ny), gz = wrap1(o2 - 2 + lz, nz);
+                sphi[(lz * SY + ly) * SX + lx] = P[((size_t)gz * ny + gy) * nx + gx];
+                ly += dly;
+                lz += dlz;
+                if (ly >= SY) {
+                    ly -= SY;
+                    ++lz;
+                }
+            }
+        }
     }
     __syncthreads();
     const double dxi0 = B.dxi[0], dxi1 = B.dxi[1], dxi2 = B.dxi[2];
     // g at the cells the tile's particles interpolate from (local -1 .. b-1 per
-    // dimension), once per block
-    const int GX = bx + 1, GY = by + 1, GZ = bz + 1, GN = GX * GY * GZ;
-    double* sgx = sphi + SX * SY * SZ;
-    double* sgy = sgx + GN;
-    double* sgz = sgy + GN;
-    for (int idx = threadIdx.x; idx < GN; idx += kBlock) {
-        const int cx = idx % GX, cy = (idx / GX) % GY, cz = idx / (GX * GY);
-        const int o = ((cz + 1) * SY + (cy + 1)) * SX + (cx + 1);  // staged index of the cell
-        const double pc = sphi[o];
-        sgx[idx] = 0.5 * (dxi0 * (pc - sphi[o - 1]) + dxi0 * (sphi[o + 1] - pc));
-        sgy[idx] = 0.5 * (dxi1 * (pc - sphi[o - SX]) + dxi1 * (sphi[o + SX] - pc));
-        sgz[idx] = 0.5 * (dxi2 * (pc - sphi[o - SX * SY]) + dxi2 * (sphi[o + SX * SY] - pc));
-    }
-    __syncthreads();
-    // val_d = sum_{kk,jj,ii} sx[1-ii]*sy[1-jj]*sz[1-kk] * g_d(i-ii, j-jj, k-kk)
-    auto interp = [&](double ax, double ay, double az, double val[3]) {
-        double xint, yint, zint;
-        const int i = cic_index(ax, B.plo[0], dxi0, &xint);
-        const int j = cic_index(ay, B.plo[1], dxi1, &yint);
-        const int k = cic_index(az, B.plo[2], dxi2, &zint);
-        const int li = wrap1(i, nx) - o0, lj = wrap1(j, ny) - o1, lk = wrap1(k, nz) - o2;
-        if (li >= 0 && li < bx && lj >= 0 && lj < by && lk >= 0 && lk < bz) {
-            // cell i-ii has g index li+1-ii
-            double v0 = 0.0, v1 = 0.0, v2 = 0.0;
-#pragma unroll
-            for (int kk = 0; kk <= 1; ++kk) {
-                const double wz = kk ? 1.0 - zint : zint;
-#pragma unroll
-                for (int jj = 0; jj <= 1; ++jj) {
-                    const double wy = jj ? 1.0 - yint : yint;
-                    const int row = ((lk + 1 - kk) * GY + (lj + 1 - jj)) * GX + li;
-                    {
-                        const double w = xint * wy * wz;  // ii = 0: cell i
-                        v0 += w * sgx[row + 1];
-                        v1 += w * sgy[row + 1];
-                        v2 += w * sgz[row + 1];
-                    }
-                    {
-                        const double w = (1.0 - xint) * wy * wz;  // ii = 1: cell i-1
-                        v0 += w * sgx[row];
-                        v1 += w * sgy[row];
-                        v2 += w * sgz[row];
-                    }
+    // dimension), components of a cell adjacent
+    const int GX = bx + 1, GY = by + 1, GZ = bz + 1;
+    double* sg = sphi + SX * SY * SZ;
+    {
+        const int rpw = kWave / GX;
+        const int lrow = lane / GX, cx = lane - lrow * GX;
+        if (lrow < rpw) {
+            const int step = (kBlock / kWave) * rpw;
+            const int r0 = wave * rpw + lrow;
+            int cy = r0 % GY, cz = r0 / GY;
+            const int dcy = step % GY, dcz = step / GY;
+            while (cz < GZ) {
+                const int o = ((cz + 1) * SY + (cy + 1)) * SX + (cx + 1);  // staged index of the cell
+                const double pc = sphi[o];
+                double* g = sg + 3 * ((cz * GY + cy) * GX + cx);
+                g[0] = 0.5 * (dxi0 * (pc - sphi[o - 1]) + dxi0 * (sphi[o + 1] - pc));
+                g[1] = 0.5 * (dxi1 * (pc - sphi[o - SX]) + dxi1 * (sphi[o + SX] - pc));
+                g[2] = 0.5 * (dxi2 * (pc - sphi[o - SX * SY]) + dxi2 * (sphi[o + SX * SY] - pc));
+                cy += dcy;
+                cz += dcz;
+                if (cy >= GY) {
+                    cy -= GY;
+                    ++cz;
                 }
             }
-            val[0] = v0;
-            val[1] = v1;
-            val[2] = v2;
-        } else {
-            // binned in another tile
-            if (i < -nx + 2 || i >= 2 * nx - 1 || j < -ny + 2 || j >= 2 * ny - 1 || k < -nz + 2 || k >= 2 * nz - 1)
-                atomicOr(err, 4);
-            auto ld = [&](int a, int b, int c) -> double {
-                return P[cidx(wrap1(i - 2 + a, nx), wrap1(j - 2 + b, ny), wrap1(k - 2 + c, nz), nx, ny)];
-            };
-            cic_grad_lines(ld, xint, yint, zint, dxi0, dxi1, dxi2, val);
         }
-    };
+    }
+    __syncthreads();
     const double half_dt = 0.5 * dt;
     const double a_cur_inv = 1.0 / a_cur;
     const double dt_a_cur_inv = dt * a_cur_inv;
     const double a_cur2_inv = 1.0 / a_cur2;
     const double dt_a_cur2_inv = dt * a_cur2_inv;
-    for (int q = threadIdx.x; q < it.np; q += kBlock) {
-        // prefetch the next particle of this thread
-        const long long pn = p + kBlock;
-        const bool vn = q + kBlock < it.np;
-        double nxp = 0.0, nyp = 0.0, nzp = 0.0, nqx = 0.0, nqy = 0.0, nqz = 0.0;
-        if (vn) {
-            nxp = x[pn]; nyp = y[pn]; nzp = z[pn];
-            nqx = vx[pn]; nqy = vy[pn]; nqz = vz[pn];
-        }
-        double val[3];
-        interp(px, py, pz, val);
-        double ux = a_prev * qx + half_dt * val[0];
-        ux = ux * a_cur_inv;
-        double uy = a_prev * qy + half_dt * val[1];
-        uy = uy * a_cur_inv;
-        double uz = a_prev * qz + half_dt * val[2];
-        uz = uz * a_cur_inv;
-        if (MODE == kKickDrift) {
-            px = px + dt_a_cur_inv * ux;
-            py = py + dt_a_cur_inv * uy;
-            pz = pz + dt_a_cur_inv * uz;
+    int same = 0;
+    // one particle: position (px, py, pz), velocity (qx, qy, qz), index q in the item
+    auto one = [&](int q, double px, double py, double pz, double qx, double qy, double qz) {
+        double xint, yint, zint;
+        const int i = cic_index(px, B.plo[0], dxi0, &xint);
+        const int j = cic_index(py, B.plo[1], dxi1, &yint);
+        const int k = cic_index(pz, B.plo[2], dxi2, &zint);
+        const int li = wrap1(i, nx) - o0, lj = wrap1(j, ny) - o1, lk = wrap1(k, nz) - o2;
+        if (li >= 0 && li < bx && lj >= 0 && lj < by && lk >= 0 && lk < bz) {
+            // val_d = sum_{kk,jj,ii} sx[1-ii]*sy[1-jj]*sz[1-kk] * g_d(i-ii, j-jj, k-kk);
+            // cell i-ii has g index li+1-ii
+            const double mx = 1.0 - xint, my = 1.0 - yint, mz = 1.0 - zint;
+            double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+#pragma unroll
+            for (int kk = 0; kk <= 1; ++kk) {
+                const double wz = kk ? mz : zint;
+#pragma unroll
+                for (int jj = 0; jj <= 1; ++jj) {
+                    const double wy = jj ? my : yint;
+                    const double* g = sg + 3 * (((lk + 1 - kk) * GY + (lj + 1 - jj)) * GX + li);
+                    const double w0 = xint * wy * wz;  // ii = 0: cell i
+                    v0 += w0 * g[3];
+                    v1 += w0 * g[4];
+                    v2 += w0 * g[5];
+                    const double w1 = mx * wy * wz;  // ii = 1: cell i-1
+                    v0 += w1 * g[0];
+                    v1 += w1 * g[1];
+                    v2 += w1 * g[2];
+                }
+            }
+            double ux = a_prev * qx + half_dt * v0;
+            ux = ux * a_cur_inv;
+            double uy = a_prev * qy + half_dt * v1;
+            uy = uy * a_cur_inv;
+            double uz = a_prev * qz + half_dt * v2;
+            uz = uz * a_cur_inv;
+            bool stay = true;  // the update of this launch completes here
+            if (MODE == kKickDrift) {
+                px = px + dt_a_cur_inv * ux;
+                py = py + dt_a_cur_inv * uy;
+                pz = pz + dt_a_cur_inv * uz;
+            } else {
+                const double sxp = periodic_shift(px, B.plo[0], B.rhi[0], B.len[0], err);
+                const double syp = periodic_shift(py, B.plo[1], B.rhi[1], B.len[1], err);
+                const double szp = periodic_shift(pz, B.plo[2], B.rhi[2], B.len[2], err);
+                const bool moved = sxp != px || syp != py || szp != pz;
+                if (MODE == kKick) {
+                    if (sxp != px) X[q] = sxp;
+                    if (syp != py) Y[q] = syp;
+                    if (szp != pz) Z[q] = szp;
+                } else if (moved) {
+                    // store the kick; the kick-drift follows in k_kick_deferred
+                    X[q] = sxp;
+                    Y[q] = syp;
+                    Z[q] = szp;
+                    stay = false;
+                    defer[atomicAdd(ndefer, 1)] = defer_entry(it.p0 + q, box) | kDeferPairKD;
+                } else {
+                    double wx = a_prev2 * ux + half_dt * v0;
+                    ux = wx * a_cur2_inv;
+                    double wy = a_prev2 * uy + half_dt * v1;
+                    uy = wy * a_cur2_inv;
+                    double wz = a_prev2 * uz + half_dt * v2;
+                    uz = wz * a_cur2_inv;
+                    px = px + dt_a_cur2_inv * ux;
+                    py = py + dt_a_cur2_inv * uy;
+                    pz = pz + dt_a_cur2_inv * uz;
+                }
+            }
+            VX[q] = ux;
+            VY[q] = uy;
+            VZ[q] = uz;
+            if (MODE != kKick && stay) {
+                X[q] = px;
+                Y[q] = py;
+                Z[q] = pz;
+                const int t = tile_of(B, px, py, pz, err);
+                KEY[q] = t;
+                if (t == it.tile)
+                    ++same;
+                else
+                    atomicAdd(&counts[t], 1);
+            }
         } else {
+            // binned in another tile: the whole update in k_kick_deferred
+            defer[atomicAdd(ndefer, 1)] = defer_entry(it.p0 + q, box);
+        }
+    };
+    for (int q0 = threadIdx.x; q0 < it.np; q0 += 2 * kBlock) {
+        if (q0 != (int)threadIdx.x) load2(q0);
+        one(q0, ax, ay, az, aux, auy, auz);
+        if (q0 + kBlock < it.np) one(q0 + kBlock, bx_, by_, bz_, bux, buy, buz);
+    }
+    if (MODE != kKick) {
+        for (int o = kWave / 2; o > 0; o >>= 1) same += __shfl_xor(same, o);
+        if (lane == 0) s_same[wave] = same;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int tot = 0;
+            for (int w = 0; w < kBlock / kWave; ++w) tot += s_same[w];
+            if (tot) atomicAdd(&counts[it.tile], tot);
+        }
+    }
+}
+
+// The particles k_kick_tiled deferred, one thread each, phi read from global
+// memory (gather_grad: same arithmetic as the staged path).
+template <int MODE>
+__global__ void __launch_bounds__(kBlock) k_kick_deferred(const long long* __restrict__ defer, const int* ndefer,
+                                                          const BoxDev* boxes, const double* __restrict__ phi,
+                                                          double* __restrict__ x, double* __restrict__ y,
+                                                          double* __restrict__ z, double* __restrict__ vx,
+                                                          double* __restrict__ vy, double* __restrict__ vz,
+                                                          double dt, double a_prev, double a_cur, double a_prev2,
+                                                          double a_cur2, int* __restrict__ key, int* counts,
+                                                          int* err) {
+    const int n = *ndefer;
+    const double half_dt = 0.5 * dt;
+    for (int q = blockIdx.x * kBlock + threadIdx.x; q < n; q += gridDim.x * kBlock) {
+        const long long e = defer[q];
+        const long long p = e & ((1LL << 40) - 1);
+        const BoxDev& B = boxes[(int)((e >> 40) & 0xFFFFF)];
+        const double* P = phi + B.cell_off;
+        double px = x[p], py = y[p], pz = z[p];
+        double ux = vx[p], uy = vy[p], uz = vz[p];
+        double val[3];
+        if (!(e & kDeferPairKD)) {
+            const double a_cur_inv = 1.0 / a_cur;
+            gather_grad(B, P, px, py, pz, val, err);
+            ux = a_prev * ux + half_dt * val[0];
+            ux = ux * a_cur_inv;
+            uy = a_prev * uy + half_dt * val[1];
+            uy = uy * a_cur_inv;
+            uz = a_prev * uz + half_dt * val[2];
+            uz = uz * a_cur_inv;
+            if (MODE == kKickDrift) {
+                const double dt_a_cur_inv = dt * a_cur_inv;
+                px = px + dt_a_cur_inv * ux;
+                py = py + dt_a_cur_inv * uy;
+                pz = pz + dt_a_cur_inv * uz;
+                vx[p] = ux; vy[p] = uy; vz[p] = uz;
+                x[p] = px; y[p] = py; z[p] = pz;
+                const int t = tile_of(B, px, py, pz, err);
+                key[p] = t;
+                atomicAdd(&counts[t], 1);
+                continue;
+            }
             const double sxp = periodic_shift(px, B.plo[0], B.rhi[0], B.len[0], err);
             const double syp = periodic_shift(py, B.plo[1], B.rhi[1], B.len[1], err);
             const double szp = periodic_shift(pz, B.plo[2], B.rhi[2], B.len[2], err);
-            const bool moved = sxp != px || syp != py || szp != pz;
             if (MODE == kKick) {
+                vx[p] = ux; vy[p] = uy; vz[p] = uz;
                 if (sxp != px) x[p] = sxp;
                 if (syp != py) y[p] = syp;
                 if (szp != pz) z[p] = szp;
-            } else {
-                px = sxp;
-                py = syp;
-                pz = szp;
-                if (moved) interp(px, py, pz, val);
-                double wx = a_prev2 * ux + half_dt * val[0];
-                ux = wx * a_cur2_inv;
-                double wy = a_prev2 * uy + half_dt * val[1];
-                uy = wy * a_cur2_inv;
-                double wz = a_prev2 * uz + half_dt * val[2];
-                uz = wz * a_cur2_inv;
-                px = px + dt_a_cur2_inv * ux;
-                py = py + dt_a_cur2_inv * uy;
-                pz = pz + dt_a_cur2_inv * uz;
+                continue;
             }
+            const bool moved = sxp != px || syp != py || szp != pz;
+            px = sxp;
+            py = syp;
+            pz = szp;
+            if (moved) gather_grad(B, P, px, py, pz, val, err);
+        } else {
+            // pair whose kick is stored: kick-drift at the shifted position
+            gather_grad(B, P, px, py, pz, val, err);
         }
-        vx[p] = ux;
-        vy[p] = uy;
-        vz[p] = uz;
-        if (MODE != kKick) {
-            x[p] = px;
-            y[p] = py;
-            z[p] = pz;
-            const int t = tile_of(B, px, py, pz, err);
-            key[p] = t;
-            agg_atomic_add(counts, t, true);
-        }
-        p = pn;
-        px = nxp; py = nyp; pz = nzp;
-        qx = nqx; qy = nqy; qz = nqz;
+        const double a_cur2_inv = 1.0 / a_cur2;
+        const double dt_a_cur2_inv = dt * a_cur2_inv;
+        double wx = a_prev2 * ux + half_dt * val[0];
+        ux = wx * a_cur2_inv;
+        double wy = a_prev2 * uy + half_dt * val[1];
+        uy = wy * a_cur2_inv;
+        double wz = a_prev2 * uz + half_dt * val[2];
+        uz = wz * a_cur2_inv;
+        px = px + dt_a_cur2_inv * ux;
+        py = py + dt_a_cur2_inv * uy;
+        pz = pz + dt_a_cur2_inv * uz;
+        vx[p] = ux; vy[p] = uy; vz[p] = uz;
+        x[p] = px; y[p] = py; z[p] = pz;
+        const int t = tile_of(B, px, py, pz, err);
+        key[p] = t;
+        atomicAdd(&counts[t], 1);
     }
 }
 
@@ -848,19 +1014,33 @@ __global__ void __launch_bounds__(kBlock) k_fill_dbl(double* a, long long n, dou
 }
 
 // f = rho + (-mass_offset) with per-block sums (used when the density was set
-// from outside rather than by the fold).
+// from outside rather than by the fold).  Cells are visited exactly as k_fold
+// visits them (same tiles per block, same cells per thread, same order), so the
+// solvability sum of a density does not depend on which of the two produced f.
 __global__ void __launch_bounds__(kBlock) k_rhs1(const CellSeg* segs, int nseg, const BoxDev* boxes,
                                                  const double* __restrict__ rho, double* __restrict__ f,
                                                  double* __restrict__ partial) {
     const CellSeg& sg = segs[find_seg(segs, nseg, blockIdx.x)];
     const BoxDev& B = boxes[sg.box];
-    const long long n = (long long)B.n[0] * B.n[1] * B.n[2];
+    const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
+    const int T0 = B.tile[0], T1 = B.tile[1], T2 = B.tile[2];
+    const int nt0 = B.ntile[0], nt1 = B.ntile[1], nt2 = B.ntile[2];
+    const int ntiles = nt0 * nt1 * nt2;
+    const int per = (ntiles + sg.nblk - 1) / sg.nblk;
+    const int t_begin = (blockIdx.x - sg.blk0) * per;
+    const int t_end = min(ntiles, t_begin + per);
     double acc = 0.0;
-    for (long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x; q < n;
-         q += (long long)sg.nblk * kBlock) {
-        const double fv = rho[B.cell_off + q] + (-B.mass_offset);
-        f[B.cell_off + q] = fv;
-        acc += fv;
+    for (int lt = t_begin; lt < t_end; ++lt) {
+        const int tx = lt % nt0, ty = (lt / nt0) % nt1, tz = lt / (nt0 * nt1);
+        const int bx = min(T0, nx - tx * T0), by = min(T1, ny - ty * T1), bz = min(T2, nz - tz * T2);
+        const int ncel = bx * by * bz;
+        for (int c = threadIdx.x; c < ncel; c += kBlock) {
+            const int li = c % bx, lj = (c / bx) % by, lk = c / (bx * by);
+            const size_t q = B.cell_off + cidx(tx * T0 + li, ty * T1 + lj, tz * T2 + lk, nx, ny);
+            const double fv = rho[q] + (-B.mass_offset);
+            f[q] = fv;
+            acc += fv;
+        }
     }
     const double bs = block_sum_fixed(acc);
     if (threadIdx.x == 0) partial[blockIdx.x] = bs;
