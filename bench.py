@@ -129,10 +129,11 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the smoother-shape autotuner")
     ap.add_argument("--kick-chunk", type=int, default=0, help="particles per kick work item (0: default)")
     ap.add_argument("--workload", default="clustered512",
-                    choices=["clustered512", "batch48", "grf256"],
+                    choices=["clustered512", "batch48", "grf256", "grf1024"],
                     help="clustered512: the headline line; batch48: 48 boxes (16 each of 64^3, 96^3, "
                          "128^3) in one launch per phase vs one after another; grf256: FP64 drop-in, "
-                         "256^3 GRF, 10 steps")
+                         "256^3 GRF, 10 steps; grf1024: 1024^3 GRF, 50 mixed steps with per-step "
+                         "accel rel-L2 against FP64 and the step-50 power spectrum check (--n to shrink)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md, "
@@ -140,6 +141,8 @@ def main():
         sys.exit(2)
     if a.workload == "batch48":
         return main_batch(a)
+    if a.workload == "grf1024":
+        return main_grf1024(a)
     if a.workload == "grf256":
         return main_grf256(a)
 
@@ -352,6 +355,128 @@ def main_grf256(a):
            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": "256^3 cells/particles, GRF k^-2 rms 0.5 cell, 10 steps", "graph": bool(a.graph)},
            "vcycles_last_step": int(it[0])}
+    emit(out, a)
+
+
+def power_spectrum_device(rho, n):
+    """P(k) of delta = rho/mean - 1 on an n^3 periodic grid (torch, on rho's
+    device): |delta_k|^2 averaged over |k| shells of width 2 pi / L (integer
+    bins of |k| L / 2 pi, rounded), rfft modes with 0 < kx < n/2 counted twice.
+    Returns (bin centres, P) as numpy arrays for bins 1 .. n/2."""
+    import torch
+    d = rho.view(n, n, n)
+    d = d / d.mean() - 1.0
+    dk = torch.fft.rfftn(d)
+    del d
+    p = (dk.real * dk.real + dk.imag * dk.imag) / float(n) ** 6
+    del dk
+    f = torch.fft.fftfreq(n, d=1.0 / n, dtype=torch.float64, device=rho.device)
+    fr = torch.fft.rfftfreq(n, d=1.0 / n, dtype=torch.float64, device=rho.device)
+    kz, ky, kx = f.view(n, 1, 1), f.view(1, n, 1), fr.view(1, 1, n // 2 + 1)
+    kb = torch.round(torch.sqrt(kx * kx + ky * ky + kz * kz)).to(torch.int64)
+    wt = torch.where((kx > 0) & (kx < n / 2), 2.0, 1.0).to(torch.float64).expand_as(p)
+    nb = n // 2 + 1
+    kb = kb.clamp_max(nb).reshape(-1)
+    ps = torch.zeros(nb + 1, dtype=torch.float64, device=rho.device).index_add_(0, kb, (p * wt).reshape(-1))
+    cnt = torch.zeros(nb + 1, dtype=torch.float64, device=rho.device).index_add_(0, kb, wt.reshape(-1))
+    ps, cnt = ps.cpu().numpy(), cnt.cpu().numpy()
+    k = np.arange(1, nb)
+    return k, ps[1:nb] / np.maximum(cnt[1:nb], 1.0)
+
+
+def main_grf1024(a):
+    """1024^3 mixed precision: lattice + GRF (P(k) ~ k^-1.5, rms 1 cell, v = 0.1
+    psi), dt 5e-4, 50 KDK steps.  After every step of the mixed trajectory an
+    FP64 solve of the same density from a zero guess (FP64 V-cycles only) gives
+    the reference accelerations: rel-L2 = |g_mixed - g_fp64|_2 / |g_fp64|_2 over
+    all particles (the mixed phi is put back before the next step).  Then an FP64 trajectory
+    from the same initial conditions; the step-50 density power spectra (CIC rho
+    -> FFT -> |delta_k|^2 in |k| shells of width 2 pi) of the two must agree to
+    0.1% per bin below half the Nyquist wavenumber.  Initial conditions are made
+    on the GPU (ics.grf_lattice_device) and handed over as device pointers."""
+    import torch
+    import nyx_amd
+    from nyx_amd import ics
+    n = a.n if a.n != 512 else 1024
+    steps = a.steps if a.steps != 5 else 50
+    dt, tol = 5e-4, a.tol
+    N = n ** 3
+    seed = ics.SEED_BASE + 4
+
+    def fresh_ics(ctx):
+        pos, vel, m = ics.grf_lattice_device(n, seed, -1.5, 1.0, 0.1)
+        ctx.upload_device(0, N, [t.data_ptr() for t in pos + vel], m_const=m)
+        ctx.sync()
+        del pos, vel
+        torch.cuda.empty_cache()
+
+    log(f"grf1024: n={n}, {steps} steps, dt={dt}")
+    ctx = nyx_amd.PMContext(nyx_amd.BoxGeom((n, n, n)), precision="mixed")
+    tuned = ctx.autotune(fp64=True, fp32=True) if not a.no_tune else None
+    fresh_ics(ctx)
+    log("ICs uploaded; device bytes", ctx.device_bytes())
+    ctx.pm_init(1.0, 1.0, tol)
+    gm = torch.empty(3 * N, dtype=torch.float64, device="cuda")
+    g64 = torch.empty(3 * N, dtype=torch.float64, device="cuda")
+    phi_m = torch.empty(N, dtype=torch.float64, device="cuda")
+    rel, t_steps, its = [], 0.0, []
+    for s in range(steps):
+        ctx.sync()
+        t0 = time.perf_counter()
+        it = ctx.pm_steps(1, dt, rel_tol=tol)
+        ctx.sync()
+        t_steps += time.perf_counter() - t0
+        its.append(int(it[0]))
+        ctx.gather_accel_device(gm.data_ptr())
+        ctx.get_phi_device(phi_m.data_ptr())
+        ctx.set_precision("fp64")
+        ctx.assign_density(want=False)
+        ctx.solve_for_phi(1.0, 1.0, tol, 0.0, use_guess=False)
+        ctx.gather_accel_device(g64.data_ptr())
+        ctx.set_phi_device(phi_m.data_ptr())
+        ctx.set_precision("mixed")
+        num = torch.linalg.vector_norm(gm - g64).item()
+        den = torch.linalg.vector_norm(g64).item()
+        rel.append(num / den)
+        log(f"  step {s + 1}: {its[-1]} V-cycles, accel rel-L2 {rel[-1]:.3e}")
+    del gm, g64, phi_m
+    torch.cuda.empty_cache()
+    rho = torch.empty(N, dtype=torch.float64, device="cuda")
+    ctx.assign_density_device(rho.data_ptr())
+    k, p_mix = power_spectrum_device(rho, n)
+    del rho
+    ctx.close()  # a fresh context for the FP64 trajectory (one set of particle buffers at a time)
+    torch.cuda.empty_cache()
+    log("mixed trajectory done; FP64 trajectory")
+    ctx = nyx_amd.PMContext(nyx_amd.BoxGeom((n, n, n)), precision="fp64")
+    if not a.no_tune:
+        ctx.autotune(fp64=True, fp32=False)
+    fresh_ics(ctx)
+    ctx.pm_init(1.0, 1.0, tol)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.pm_steps(steps, dt, rel_tol=tol)
+    ctx.sync()
+    t64 = time.perf_counter() - t0
+    rho = torch.empty(N, dtype=torch.float64, device="cuda")
+    ctx.assign_density_device(rho.data_ptr())
+    _, p_64 = power_spectrum_device(rho, n)
+    del rho
+    ctx.close()
+    low = k < n / 4  # below half the Nyquist wavenumber (k_Nyq = n/2 in these units)
+    dev = np.abs(p_mix[low] / p_64[low] - 1.0)
+    out = {"metric": "PM KDK at 1024^3 GRF: mixed vs FP64 accel rel-L2 per step and step-50 P(k)",
+           "value": round(N * steps / t_steps, 1), "unit": "particle-steps/s (mixed, one step per call)",
+           "n_gpus": 1, "steps": steps, "warmup": 0, "ms_per_step": round(t_steps / steps * 1e3, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+           "data": "synthetic",
+           "config": {"workload": f"{n}^3 cells/particles, GRF k^-1.5 rms 1 cell, v = 0.1 psi, dt {dt}, "
+                                  f"{steps} steps, rel tol {tol:g}", "autotuned": tuned},
+           "fp64": {"value": round(N * steps / t64, 1), "ms_per_step": round(t64 / steps * 1e3, 3)},
+           "accel_rel_l2": {"max": max(rel), "last": rel[-1], "per_step": [float(f"{v:.3e}") for v in rel]},
+           "vcycles_mixed": its,
+           "power_spectrum": {"bins_compared": int(low.sum()), "max_rel_dev": float(dev.max()),
+                              "pass": bool(dev.max() <= 1e-3)}}
     emit(out, a)
 
 

@@ -126,8 +126,9 @@ int64_t nyxpm_num_particles(const nyxpm_ctx *ctx, int box);
 
 /* ---- AssignDensity / sumParticleMass ------------------------------------ */
 /* Deposit every box's particles with CIC weights (deterministic), periodic,
- * times 1/cell volume.  rho_out (host, n cells of box `box`) may be NULL to keep
- * the result on the device only; box = -1 with rho_out NULL deposits all boxes. */
+ * times 1/cell volume.  rho_out (n cells of box `box`, host or device memory) may
+ * be NULL to keep the result in the context only; box = -1 with rho_out NULL
+ * deposits all boxes. */
 int nyxpm_assign_density(nyxpm_ctx *ctx, int box, double *rho_out);
 int nyxpm_sum_particle_mass(nyxpm_ctx *ctx, int box, double *mass_out);
 
@@ -149,9 +150,11 @@ int nyxpm_solve_poisson(nyxpm_ctx *ctx, int box, const double *rhs, double *phi_
  * DarkMatterParticleContainer.cpp:276-292), without changing the particles:
  * g_out holds 3*np doubles, component-major, in upload (id) order. */
 int nyxpm_gather_accel(nyxpm_ctx *ctx, int box, double *g_out);
-/* Cell-centred gravity g = average of face gradients of phi (3 fields, host). */
+/* Cell-centred gravity g = average of face gradients of phi (3 fields). */
 int nyxpm_get_grav_vector(nyxpm_ctx *ctx, int box, double *g_out);
-/* Field access for box `box` (host arrays of n cells). */
+/* Field access for box `box` (arrays of n cells).  The array arguments of
+ * assign_density, gather_accel, get_grav_vector and these may be host or device
+ * memory (HIP unified addressing): device memory is written or read directly. */
 int nyxpm_get_phi(nyxpm_ctx *ctx, int box, double *phi_out);
 int nyxpm_set_phi(nyxpm_ctx *ctx, int box, const double *phi_in);
 int nyxpm_set_density(nyxpm_ctx *ctx, int box, const double *rho_in);

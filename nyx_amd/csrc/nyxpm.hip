@@ -159,14 +159,12 @@ struct nyxpm_ctx {
     int scan_blocks = 0;
     TileItem* items = nullptr;  // tile chunks of the last sort (tile-staged kernels)
     int* nitems = nullptr;
-    long long* defer = nullptr;  // particles handed from the tile-staged kick to k_kick_deferred
-    int* ndefer = nullptr;
+    int* ndefer = nullptr;  // entries of the list handed from the tile-staged kick to k_kick_deferred
     long long items_cap = 0;
     bool items_valid = false;
 
     // level-0 fields
     double *own = nullptr, *halo = nullptr, *rho = nullptr, *f = nullptr, *phi = nullptr, *partial = nullptr;
-    double* gtmp = nullptr;  // 3 fields of the largest box (grav vector / downloads)
     // MG pools
     double *r64 = nullptr, *e64 = nullptr, *e64b = nullptr, *G64 = nullptr;
     float *r32 = nullptr, *e32 = nullptr, *e32b = nullptr, *G32 = nullptr;
@@ -683,13 +681,10 @@ int build_pseg(nyxpm_ctx* c) {
 int alloc_fields(nyxpm_ctx* c) {
     const int nbox = (int)c->boxes.size();
     int rc;
-    long long maxc = 0;
-    for (auto& B : c->boxes) maxc = std::max(maxc, B.lev[0].nc);
     if ((rc = dalloc(c, &c->own, c->ncell_total))) return rc;
-    if ((rc = dalloc(c, &c->rho, c->ncell_total))) return rc;
+    c->rho = c->own;  // the fold turns the deposit's own cells into rho in place
     if ((rc = dalloc(c, &c->f, c->ncell_total))) return rc;
     if ((rc = dalloc(c, &c->phi, c->ncell_total))) return rc;
-    if ((rc = dalloc(c, &c->gtmp, 3 * maxc))) return rc;
     if ((rc = dalloc(c, &c->halo, (size_t)c->ntiles_total * c->halo_stride))) return rc;
     if ((rc = dalloc(c, &c->partial, (size_t)c->t_l0.blocks))) return rc;
     if ((rc = dalloc(c, &c->counts, c->ntiles_total))) return rc;
@@ -750,7 +745,6 @@ void free_particles(nyxpm_ctx* c) {
     }
     dfree(c->key);
     dfree(c->items);
-    dfree(c->defer);
 }
 
 int alloc_particle_set(nyxpm_ctx* c, PartPtrs& P, long long n, bool mass) {
@@ -1129,11 +1123,14 @@ int phase_kick(nyxpm_ctx* c, double dt, double a_prev, double a_cur, double a_pr
         if (MOVE)
             hipLaunchKernelGGL(k_fill_int, dim3(nblocks(c->ntiles_total)), dim3(kBlock), 0, c->stream, c->counts,
                                c->ntiles_total, 0);
+        // the deferred-particle list lives in the sort target's x array (as many
+        // 8-byte slots as particles), idle until the next sort overwrites it
+        long long* defer = reinterpret_cast<long long*>(c->P[1 - c->cur].x);
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(kBlock), 0, c->stream, c->ndefer, 1LL, 0);
         hipLaunchKernelGGL(k_kick_tiled<MODE>, dim3((unsigned)c->items_cap), dim3(kBlock), lds, c->stream, c->tseg,
                            c->tseg_n, c->dboxes, c->items, c->nitems, c->phi, A.x, A.y, A.z, A.vx, A.vy, A.vz, dt,
-                           a_prev, a_cur, a_prev2, a_cur2, c->key, c->counts, c->defer, c->ndefer, c->err_flag);
-        hipLaunchKernelGGL(k_kick_deferred<MODE>, dim3(256), dim3(kBlock), 0, c->stream, c->defer, c->ndefer,
+                           a_prev, a_cur, a_prev2, a_cur2, c->key, c->counts, defer, c->ndefer, c->err_flag);
+        hipLaunchKernelGGL(k_kick_deferred<MODE>, dim3(256), dim3(kBlock), 0, c->stream, defer, c->ndefer,
                            c->dboxes, c->phi, A.x, A.y, A.z, A.vx, A.vy, A.vz, dt, a_prev, a_cur, a_prev2, a_cur2,
                            c->key, c->counts, c->err_flag);
         LAUNCHCK(c);
@@ -1218,6 +1215,17 @@ int step_tail(nyxpm_ctx* c, double dt, double a_old, double a_new, double four_p
     return phase_kick<kKick>(c, dt, a_half, a_new);
 }
 
+// Whether p is device memory (pointer attributes; pageable host memory is not
+// known to HIP, which reports an error that is cleared here).
+bool is_device_ptr(const void* p) {
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1278,7 +1286,8 @@ int nyxpm_destroy(nyxpm_ctx* c) {
     if (c->graph) hipGraphDestroy(c->graph);
     free_particles(c);
     dfree(c->counts); dfree(c->cursor); dfree(c->tstart); dfree(c->bsum);
-    dfree(c->own); dfree(c->halo); dfree(c->rho); dfree(c->f); dfree(c->phi); dfree(c->partial); dfree(c->gtmp);
+    c->rho = nullptr;
+    dfree(c->own); dfree(c->halo); dfree(c->f); dfree(c->phi); dfree(c->partial);
 
     dfree(c->r64); dfree(c->e64); dfree(c->e64b); dfree(c->G64);
     dfree(c->r32); dfree(c->e32); dfree(c->e32b); dfree(c->G32);
@@ -1408,15 +1417,12 @@ static int upload_impl(nyxpm_ctx* c, int box, int64_t np, const double* x, const
     if (icap > 0x7fffffffLL) FAIL(c, NYXPM_ERR_ARG, "too many tile chunks");
     TileItem* nitm = nullptr;
     if ((rc = dalloc(c, &nitm, icap))) return rc;
-    long long* ndef = nullptr;
-    if ((rc = dalloc(c, &ndef, std::max(tot, 1LL)))) return rc;
     free_particles(c);
     c->P[0] = N[0];
     c->P[1] = N[1];
     c->key = nkey;
     c->items = nitm;
     c->items_cap = icap;
-    c->defer = ndef;
     c->items_valid = false;
     c->cur = 0;
     c->part_cap = cap;
@@ -1506,7 +1512,7 @@ int nyxpm_assign_density(nyxpm_ctx* c, int box, double* rho_out) {
     if ((rc = phase_deposit(c, true))) return rc;
     if (rho_out) {
         const Box& B = c->boxes[box];
-        HIPCK(c, hipMemcpyAsync(rho_out, c->rho + B.cell_off, B.lev[0].nc * sizeof(double), hipMemcpyDeviceToHost,
+        HIPCK(c, hipMemcpyAsync(rho_out, c->rho + B.cell_off, B.lev[0].nc * sizeof(double), hipMemcpyDefault,
                                 c->stream));
     }
     HIPCK(c, hipStreamSynchronize(c->stream));
@@ -1526,7 +1532,7 @@ int nyxpm_set_density(nyxpm_ctx* c, int box, const double* rho_in) {
     int rc;
     if ((rc = check_box(c, box))) return rc;
     const Box& B = c->boxes[box];
-    HIPCK(c, hipMemcpyAsync(c->rho + B.cell_off, rho_in, B.lev[0].nc * sizeof(double), hipMemcpyHostToDevice,
+    HIPCK(c, hipMemcpyAsync(c->rho + B.cell_off, rho_in, B.lev[0].nc * sizeof(double), hipMemcpyDefault,
                             c->stream));
     HIPCK(c, hipStreamSynchronize(c->stream));
     c->density_valid = true;
@@ -1578,11 +1584,20 @@ int nyxpm_get_grav_vector(nyxpm_ctx* c, int box, double* g_out) {
     int rc;
     if ((rc = check_box(c, box))) return rc;
     const Box& B = c->boxes[box];
+    const bool dev = is_device_ptr(g_out);
+    double* out = g_out;
+    if (!dev && (rc = dalloc(c, &out, 3 * B.lev[0].nc))) return rc;
     hipLaunchKernelGGL(k_grav_vector, dim3(nblocks(B.lev[0].nc)), dim3(kBlock), 0, c->stream, c->dboxes, box, c->phi,
-                       c->gtmp);
-    LAUNCHCK(c);
-    HIPCK(c, hipMemcpyAsync(g_out, c->gtmp, 3 * B.lev[0].nc * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCK(c, hipStreamSynchronize(c->stream));
+                       out);
+    hipError_t e = hipGetLastError();
+    if (!dev && e == hipSuccess)
+        e = hipMemcpyAsync(g_out, out, 3 * B.lev[0].nc * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (!dev) {
+        hipFree(out);
+        c->device_bytes -= 3 * B.lev[0].nc * (long long)sizeof(double);
+    }
+    if (e != hipSuccess) FAIL(c, NYXPM_ERR_HIP, "get_grav_vector: %s", hipGetErrorString(e));
     return NYXPM_OK;
 }
 
@@ -1594,15 +1609,19 @@ int nyxpm_gather_accel(nyxpm_ctx* c, int box, double* g_out) {
     const Box& B = c->boxes[box];
     if (B.np == 0) return NYXPM_OK;
     PartPtrs& A = c->P[c->cur];
-    double* tmp = nullptr;
-    if ((rc = dalloc(c, &tmp, 3 * B.np))) return rc;
+    const bool dev = is_device_ptr(g_out);
+    double* out = g_out;
+    if (!dev && (rc = dalloc(c, &out, 3 * B.np))) return rc;
     hipLaunchKernelGGL(k_gather, dim3(nblocks(B.np)), dim3(kBlock), 0, c->stream, c->dboxes, box, c->phi, A.x, A.y,
-                       A.z, A.id, tmp, c->err_flag);
+                       A.z, A.id, out, c->err_flag);
     hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(g_out, tmp, 3 * B.np * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (!dev && e == hipSuccess)
+        e = hipMemcpyAsync(g_out, out, 3 * B.np * sizeof(double), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    hipFree(tmp);
-    c->device_bytes -= 3 * B.np * (long long)sizeof(double);
+    if (!dev) {
+        hipFree(out);
+        c->device_bytes -= 3 * B.np * (long long)sizeof(double);
+    }
     if (e != hipSuccess) FAIL(c, NYXPM_ERR_HIP, "gather_accel: %s", hipGetErrorString(e));
     return err_check(c);
 }
@@ -1612,11 +1631,12 @@ static int field_io(nyxpm_ctx* c, int box, double* field, double* host, bool to_
     int rc;
     if ((rc = check_box(c, box))) return rc;
     const Box& B = c->boxes[box];
+    // host or device memory on the caller's side (unified addressing)
     if (to_host)
-        HIPCK(c, hipMemcpyAsync(host, field + B.cell_off, B.lev[0].nc * sizeof(double), hipMemcpyDeviceToHost,
+        HIPCK(c, hipMemcpyAsync(host, field + B.cell_off, B.lev[0].nc * sizeof(double), hipMemcpyDefault,
                                 c->stream));
     else
-        HIPCK(c, hipMemcpyAsync(field + B.cell_off, host, B.lev[0].nc * sizeof(double), hipMemcpyHostToDevice,
+        HIPCK(c, hipMemcpyAsync(field + B.cell_off, host, B.lev[0].nc * sizeof(double), hipMemcpyDefault,
                                 c->stream));
     HIPCK(c, hipStreamSynchronize(c->stream));
     return NYXPM_OK;

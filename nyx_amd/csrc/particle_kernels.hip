@@ -327,9 +327,10 @@ __device__ __forceinline__ double block_sum_fixed(double v) {
 // the step needs only f.
 template <bool WRITE_RHO>
 __global__ void __launch_bounds__(kBlock) k_fold(const CellSeg* segs, int nseg, const BoxDev* boxes,
-                                                 const double* __restrict__ own, const double* __restrict__ halo,
-                                                 double* __restrict__ rho, double* __restrict__ f,
+                                                 const double* own, const double* __restrict__ halo,
+                                                 double* rho, double* __restrict__ f,
                                                  double* __restrict__ partial) {
+    // rho may be own itself (each cell is read, then written, by one thread)
     const CellSeg& sg = segs[find_seg(segs, nseg, blockIdx.x)];
     const BoxDev& B = boxes[sg.box];
     const int nx = B.n[0], ny = B.n[1], nz = B.n[2];

@@ -183,6 +183,21 @@ class PMContext:
         self._ck(self.L.nyxpm_gather_accel(self.h, box, ptr(g)))
         return g
 
+    # -- device-resident variants: `addr` is a device address (e.g. a torch
+    #    tensor's data_ptr()) of an FP64 array of the right length; the library
+    #    reads or writes it in place (unified addressing).
+    def gather_accel_device(self, addr, box=0):
+        self._ck(self.L.nyxpm_gather_accel(self.h, box, ctypes.c_void_p(int(addr))))
+
+    def get_phi_device(self, addr, box=0):
+        self._ck(self.L.nyxpm_get_phi(self.h, box, ctypes.c_void_p(int(addr))))
+
+    def set_phi_device(self, addr, box=0):
+        self._ck(self.L.nyxpm_set_phi(self.h, box, ctypes.c_void_p(int(addr))))
+
+    def assign_density_device(self, addr, box=0):
+        self._ck(self.L.nyxpm_assign_density(self.h, box, ctypes.c_void_p(int(addr))))
+
     def get_phi(self, box=0):
         a = np.empty(self.geoms[box].shape)
         self._ck(self.L.nyxpm_get_phi(self.h, box, ptr(a)))
