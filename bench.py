@@ -63,6 +63,18 @@ def kernel_bytes(name, ncell, npart, halo_cells, mixed_sz):
     return table.get(name)
 
 
+def measured_traffic(stage, n):
+    """HBM bytes per launch of `stage` at 512^3 from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated against a copy of known size;
+    profiles/pm_step_512_clustered.md gives the commands), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_512_clustered.json")
+    if n != 512 or not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        e = json.load(fh).get("stages", {}).get(stage)
+    return e["traffic_bytes"] if e else None
+
+
 def run_mode(ctx, precision, steps, warmup, dt, tol, use_graph):
     ctx.set_precision(precision)
     if warmup:
@@ -217,7 +229,8 @@ def main():
     achieved = b / (d["mean_ms"] * 1e-3) / 1e9 if b else None
     roof = {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 1) if achieved else None,
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-            "traffic": None, "bytes_per_launch": int(b) if b else None, "mean_launch_ms": d["mean_ms"]}
+            "traffic": measured_traffic(dname, n), "bytes_per_launch": int(b) if b else None,
+            "mean_launch_ms": d["mean_ms"]}
 
     def smoother_roof(st, sz):
         e = st.get("sweep.d0")
