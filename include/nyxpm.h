@@ -93,6 +93,11 @@ typedef struct nyxpm_opts {
     int coarse;        /* 0: the levels of at most 4096 cells (down to the bottom
                           solve) run as one workgroup per box (default);
                           1: one launch per level and step (same results) */
+    double particle_cfl; /* > 0: every closing kick (nyxpm_move_kick, the end of each
+                          nyxpm_pm_steps step) also computes estTimestep of the
+                          kicked particles with this cfl and a = a_new, which
+                          nyxpm_est_time_step then returns without another pass;
+                          0 = off */
 } nyxpm_opts;
 
 /* ---- lifecycle ---------------------------------------------------------- */
@@ -150,6 +155,13 @@ int nyxpm_solve_poisson(nyxpm_ctx *ctx, int box, const double *rhs, double *phi_
  * DarkMatterParticleContainer.cpp:276-292), without changing the particles:
  * g_out holds 3*np doubles, component-major, in upload (id) order. */
 int nyxpm_gather_accel(nyxpm_ctx *ctx, int box, double *g_out);
+/* NyxParticleContainer::estTimestep(grav, a, 0, cfl)
+ * (NyxParticleContainer.H:327-410, called by Nyx::particle_est_time_step,
+ * NyxParticles.cpp:1028-1036) with grav = the cell-centred gravity of the current
+ * phi: per particle cfl / max_d(|v_d| dxi_d / a), capped by 1/sqrt(|g| dxi_x) with g
+ * at the particle's cell floor((x - plo) dxi); minimum over the box's particles,
+ * -1e50 for a box without particles.  dt_out holds nbox values. */
+int nyxpm_est_time_step(nyxpm_ctx *ctx, double a, double cfl, double *dt_out);
 /* Cell-centred gravity g = average of face gradients of phi (3 fields). */
 int nyxpm_get_grav_vector(nyxpm_ctx *ctx, int box, double *g_out);
 /* Field access for box `box` (arrays of n cells).  The array arguments of

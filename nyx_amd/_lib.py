@@ -43,7 +43,7 @@ class Opts(ctypes.Structure):
                 ("max_iter", ctypes.c_int), ("tile", ctypes.c_int * 3), ("verbose", ctypes.c_int),
                 ("smoother", ctypes.c_int), ("kick", ctypes.c_int), ("sweep_tx", ctypes.c_int),
                 ("sweep_ty", ctypes.c_int), ("sweep_zchunk", ctypes.c_int), ("kick_chunk", ctypes.c_int),
-                ("separate_kicks", ctypes.c_int), ("coarse", ctypes.c_int)]
+                ("separate_kicks", ctypes.c_int), ("coarse", ctypes.c_int), ("particle_cfl", ctypes.c_double)]
 
 
 # (name, restype, argtypes) for every function in include/nyxpm.h
@@ -71,6 +71,7 @@ SIGNATURES = [
     ("nyxpm_solve_poisson", _i, [_vp, _i, _vp, _vp, _d, _d, _i, _vp, _vp]),
     ("nyxpm_get_grav_vector", _i, [_vp, _i, _vp]),
     ("nyxpm_gather_accel", _i, [_vp, _i, _vp]),
+    ("nyxpm_est_time_step", _i, [_vp, _d, _d, _vp]),
     ("nyxpm_get_phi", _i, [_vp, _i, _vp]),
     ("nyxpm_set_phi", _i, [_vp, _i, _vp]),
     ("nyxpm_set_density", _i, [_vp, _i, _vp]),

@@ -160,6 +160,12 @@ struct nyxpm_ctx {
     TileItem* items = nullptr;  // tile chunks of the last sort (tile-staged kernels)
     int* nitems = nullptr;
     int* ndefer = nullptr;  // entries of the list handed from the tile-staged kick to k_kick_deferred
+    // particle time step (estTimestep): per-box minimum as bits, kept by the closing
+    // kicks when cfl > 0; valid for the state after the last such kick (a = dt_a)
+    unsigned long long* dtmin = nullptr;
+    double cfl = 0.0;
+    bool dt_valid = false;
+    double dt_a = 0.0;
     long long items_cap = 0;
     bool items_valid = false;
 
@@ -703,6 +709,7 @@ int alloc_fields(nyxpm_ctx* c) {
     if ((rc = dalloc(c, &c->err_flag, 1))) return rc;
     if ((rc = dalloc(c, &c->nitems, 1))) return rc;
     if ((rc = dalloc(c, &c->ndefer, 1))) return rc;
+    if ((rc = dalloc(c, &c->dtmin, nbox))) return rc;
     if ((rc = dalloc(c, &c->nr, (size_t)nbox * kNormSlots))) return rc;
     if ((rc = dalloc(c, &c->nf, (size_t)nbox * kNormSlots))) return rc;
     if ((rc = dalloc(c, &c->state, 4 * nbox))) return rc;
@@ -1080,6 +1087,7 @@ int solve_impl(nyxpm_ctx* c, double rel, double abs_tol, int fixed_iters, int gr
 }
 
 int solve(nyxpm_ctx* c, double rel, double abs_tol, int fixed_iters, int graph_cycles) {
+    c->dt_valid = false;  // phi changes
     if (c->precision == NYXPM_MIXED) {
         int rc = ensure_mixed_pools(c);
         if (rc) return rc;
@@ -1126,14 +1134,22 @@ int phase_kick(nyxpm_ctx* c, double dt, double a_prev, double a_cur, double a_pr
         // the deferred-particle list lives in the sort target's x array (as many
         // 8-byte slots as particles), idle until the next sort overwrites it
         long long* defer = reinterpret_cast<long long*>(c->P[1 - c->cur].x);
+        // the closing kick (alone or in a pair) also estimates the next time step
+        const double cfl = MODE != kKickDrift ? c->cfl : 0.0;
+        if (cfl > 0)
+            hipLaunchKernelGGL(k_fill_dbl, dim3(1), dim3(kBlock), 0, c->stream, reinterpret_cast<double*>(c->dtmin),
+                               (long long)c->boxes.size(), 1e50);
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(kBlock), 0, c->stream, c->ndefer, 1LL, 0);
-        hipLaunchKernelGGL(k_kick_tiled<MODE>, dim3((unsigned)c->items_cap), dim3(kBlock), lds, c->stream, c->tseg,
-                           c->tseg_n, c->dboxes, c->items, c->nitems, c->phi, A.x, A.y, A.z, A.vx, A.vy, A.vz, dt,
-                           a_prev, a_cur, a_prev2, a_cur2, c->key, c->counts, defer, c->ndefer, c->err_flag);
+        auto kern = cfl > 0 ? k_kick_tiled<MODE, MODE != kKickDrift> : k_kick_tiled<MODE, false>;
+        hipLaunchKernelGGL(kern, dim3((unsigned)c->items_cap), dim3(kBlock), lds, c->stream, c->tseg, c->tseg_n,
+                           c->dboxes, c->items, c->nitems, c->phi, A.x, A.y, A.z, A.vx, A.vy, A.vz, dt, a_prev,
+                           a_cur, a_prev2, a_cur2, c->key, c->counts, defer, c->ndefer, cfl, c->dtmin, c->err_flag);
         hipLaunchKernelGGL(k_kick_deferred<MODE>, dim3(256), dim3(kBlock), 0, c->stream, defer, c->ndefer,
                            c->dboxes, c->phi, A.x, A.y, A.z, A.vx, A.vy, A.vz, dt, a_prev, a_cur, a_prev2, a_cur2,
-                           c->key, c->counts, c->err_flag);
+                           c->key, c->counts, cfl, c->dtmin, c->err_flag);
         LAUNCHCK(c);
+        c->dt_valid = cfl > 0;
+        c->dt_a = a_cur;
         c->keys_valid = MOVE;
         c->sorted_valid = false;
         c->density_valid = false;
@@ -1143,6 +1159,7 @@ int phase_kick(nyxpm_ctx* c, double dt, double a_prev, double a_cur, double a_pr
         int rc = phase_kick<kKick>(c, dt, a_prev, a_cur);
         return rc ? rc : phase_kick<kKickDrift>(c, dt, a_prev2, a_cur2);
     }
+    c->dt_valid = false;
     if (MOVE) {
         Timed t(c, "kick_drift");
         hipLaunchKernelGGL(k_fill_int, dim3(nblocks(c->ntiles_total)), dim3(kBlock), 0, c->stream, c->counts,
@@ -1250,6 +1267,7 @@ int nyxpm_create_batch(nyxpm_ctx** out, int nbox, const nyxpm_geom* geoms, const
         c->no_fused = opts->smoother == 1;
         c->untiled_kick = opts->kick == 1;
         c->separate_kicks = opts->separate_kicks != 0;
+        if (opts->particle_cfl > 0) c->cfl = opts->particle_cfl;
         c->per_level_coarse = opts->coarse == 1;
         if (opts->sweep_tx > 0) {
             c->force_tx[0] = c->force_tx[1] = opts->sweep_tx;
@@ -1292,7 +1310,7 @@ int nyxpm_destroy(nyxpm_ctx* c) {
     dfree(c->r64); dfree(c->e64); dfree(c->e64b); dfree(c->G64);
     dfree(c->r32); dfree(c->e32); dfree(c->e32b); dfree(c->G32);
     dfree(c->active); dfree(c->iters); dfree(c->failed); dfree(c->err_flag); dfree(c->nitems); dfree(c->nr); dfree(c->nf);
-    dfree(c->ndefer);
+    dfree(c->ndefer); dfree(c->dtmin);
     dfree(c->state); dfree(c->solv); dfree(c->dv_size); dfree(c->dboxes);
     dfree(c->seg_pool); dfree(c->pseg); dfree(c->tseg); dfree(c->cc_first);
     if (c->h_flags) hipHostFree(c->h_flags);
@@ -1424,6 +1442,7 @@ static int upload_impl(nyxpm_ctx* c, int box, int64_t np, const double* x, const
     c->items = nitm;
     c->items_cap = icap;
     c->items_valid = false;
+    c->dt_valid = false;
     c->cur = 0;
     c->part_cap = cap;
     c->any_mass = need_mass;
@@ -1626,6 +1645,29 @@ int nyxpm_gather_accel(nyxpm_ctx* c, int box, double* g_out) {
     return err_check(c);
 }
 
+int nyxpm_est_time_step(nyxpm_ctx* c, double a, double cfl, double* dt_out) {
+    if (!c || !dt_out) return NYXPM_ERR_ARG;
+    if (!(a > 0.0) || !(cfl > 0.0)) FAIL(c, NYXPM_ERR_ARG, "a and cfl must be > 0");
+    int rc;
+    if ((rc = require_particles(c))) return rc;
+    const int nbox = (int)c->boxes.size();
+    // the estimate the last closing kick kept, if it is of this state, a and cfl
+    if (!(c->dt_valid && a == c->dt_a && cfl == c->cfl)) {
+        PartPtrs& A = c->P[c->cur];
+        hipLaunchKernelGGL(k_fill_dbl, dim3(1), dim3(kBlock), 0, c->stream, reinterpret_cast<double*>(c->dtmin),
+                           (long long)nbox, 1e50);
+        if (c->pseg_blocks > 0)
+            hipLaunchKernelGGL(k_est_dt, dim3(c->pseg_blocks), dim3(kBlock), 0, c->stream, c->pseg, c->pseg_n,
+                               c->dboxes, c->phi, A.x, A.y, A.z, A.vx, A.vy, A.vz, a, cfl, c->dtmin);
+        LAUNCHCK(c);
+    }
+    std::vector<double> v(nbox);
+    HIPCK(c, hipMemcpyAsync(v.data(), c->dtmin, nbox * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    for (int b = 0; b < nbox; ++b) dt_out[b] = c->boxes[b].np > 0 ? v[b] : -1e50;
+    return err_check(c);
+}
+
 static int field_io(nyxpm_ctx* c, int box, double* field, double* host, bool to_host) {
     if (!c || !host) return NYXPM_ERR_ARG;
     int rc;
@@ -1645,7 +1687,10 @@ static int field_io(nyxpm_ctx* c, int box, double* field, double* host, bool to_
 int nyxpm_get_phi(nyxpm_ctx* c, int box, double* out) { return field_io(c, box, c ? c->phi : nullptr, out, true); }
 int nyxpm_set_phi(nyxpm_ctx* c, int box, const double* in) {
     int rc = field_io(c, box, c ? c->phi : nullptr, const_cast<double*>(in), false);
-    if (!rc) c->phi_valid = true;
+    if (!rc) {
+        c->phi_valid = true;
+        c->dt_valid = false;
+    }
     return rc;
 }
 int nyxpm_get_rhs(nyxpm_ctx* c, int box, double* out) { return field_io(c, box, c ? c->f : nullptr, out, true); }
@@ -1677,6 +1722,7 @@ int nyxpm_redistribute(nyxpm_ctx* c) {
                        A.y, A.z, c->err_flag);
     LAUNCHCK(c);
     c->keys_valid = c->sorted_valid = c->density_valid = false;
+    c->dt_valid = false;
     return nyxpm_sync(c);
 }
 

@@ -623,15 +623,99 @@ __global__ void __launch_bounds__(kBlock) k_make_items(const long long* __restri
     }
 }
 
+// NyxParticleContainer::estTimestep (NyxParticleContainer.H:327-410) for one
+// particle: cfl / max_d(|v_d| dxi_d / a), capped by 1 / sqrt(|g| dxi_x) with g the
+// cell-centred gravity of the particle's cell (cell = floor((x - plo) dxi)).
+__device__ __forceinline__ double particle_dt(double ux, double uy, double uz, double adxi0, double adxi1,
+                                              double adxi2, double ax, double ay, double az, double dxi0,
+                                              double cfl) {
+    const double mx = fabs(ux) * adxi0;
+    const double my = fabs(uy) * adxi1;
+    const double mz = fabs(uz) * adxi2;
+    double m = mx > my ? mx : my;
+    m = m > mz ? m : mz;
+    double dtp = (m > 0) ? (cfl / m) : 1e50;
+    const double mag = sqrt(ax * ax + ay * ay + az * az);
+    if (mag > 0) {
+        const double da = 1 / sqrt(mag * dxi0);
+        dtp = dtp < da ? dtp : da;
+    }
+    return dtp;
+}
+
+// Cell-centred gravity (k_grav_vector's arithmetic) at cell (i, j, k) of a box.
+__device__ __forceinline__ void grav_at(const BoxDev& B, const double* __restrict__ P, int i, int j, int k,
+                                        double& gx, double& gy, double& gz) {
+    const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
+    const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
+    const int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
+    const int km = k == 0 ? nz - 1 : k - 1, kp = k == nz - 1 ? 0 : k + 1;
+    const double pc = P[cidx(i, j, k, nx, ny)];
+    gx = 0.5 * (B.dxi[0] * (pc - P[cidx(im, j, k, nx, ny)]) + B.dxi[0] * (P[cidx(ip, j, k, nx, ny)] - pc));
+    gy = 0.5 * (B.dxi[1] * (pc - P[cidx(i, jm, k, nx, ny)]) + B.dxi[1] * (P[cidx(i, jp, k, nx, ny)] - pc));
+    gz = 0.5 * (B.dxi[2] * (pc - P[cidx(i, j, km, nx, ny)]) + B.dxi[2] * (P[cidx(i, j, kp, nx, ny)] - pc));
+}
+
+// The estimate from global phi (any position; cells wrapped periodically).
+__device__ __forceinline__ double particle_dt_global(const BoxDev& B, const double* __restrict__ P, double px,
+                                                     double py, double pz, double ux, double uy, double uz,
+                                                     double a, double cfl) {
+    const int i = wrap1((int)floor((px - B.plo[0]) * B.dxi[0]), B.n[0]);
+    const int j = wrap1((int)floor((py - B.plo[1]) * B.dxi[1]), B.n[1]);
+    const int k = wrap1((int)floor((pz - B.plo[2]) * B.dxi[2]), B.n[2]);
+    double gx, gy, gz;
+    grav_at(B, P, i, j, k, gx, gy, gz);
+    return particle_dt(ux, uy, uz, B.dxi[0] / a, B.dxi[1] / a, B.dxi[2] / a, gx, gy, gz, B.dxi[0], cfl);
+}
+
+// Minimum over a block into a per-box slot (bit pattern of a positive double:
+// atomicMin is exact and order independent).
+__device__ __forceinline__ void block_min_to(double v, unsigned long long* slot) {
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const double w = __shfl_xor(v, o);
+        v = w < v ? w : v;
+    }
+    __shared__ double wmin[kBlock / kWave];
+    __syncthreads();
+    if ((threadIdx.x & (kWave - 1)) == 0) wmin[threadIdx.x / kWave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = wmin[0];
+        for (int w = 1; w < kBlock / kWave; ++w) m = wmin[w] < m ? wmin[w] : m;
+        if (m < 1e50) atomicMin(slot, dbits(m));
+    }
+}
+
+// Standalone estimate over every particle (nyxpm_est_time_step when no fused
+// estimate of the current state is kept).
+__global__ void __launch_bounds__(kBlock) k_est_dt(const BoxSeg* segs, int nseg, const BoxDev* boxes,
+                                                   const double* __restrict__ phi, const double* __restrict__ x,
+                                                   const double* __restrict__ y, const double* __restrict__ z,
+                                                   const double* __restrict__ vx, const double* __restrict__ vy,
+                                                   const double* __restrict__ vz, double a, double cfl,
+                                                   unsigned long long* dtmin) {
+    const BoxSeg sg = segs[find_seg(segs, nseg, blockIdx.x)];
+    const BoxDev& B = boxes[sg.box];
+    const long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x;
+    double d = 1e50;
+    if (q < B.np) {
+        const long long p = B.part_off + q;
+        d = particle_dt_global(B, phi + B.cell_off, x[p], y[p], z[p], vx[p], vy[p], vz[p], a, cfl);
+    }
+    block_min_to(d, dtmin + sg.box);
+}
+
 // Kick variants of the tile-staged particle kernel.
 constexpr int kKick = 0;       // moveKick: kick, then the periodic shift of Redistribute
 constexpr int kKickDrift = 1;  // moveKickDrift: kick, drift, new tile key
 constexpr int kKickPair = 2;   // moveKick of one step followed by moveKickDrift of the next
 
 // Particles the tile-staged kernel hands over to k_kick_deferred: index (bits
-// 0-39), box (bits 40-59) and what is left to do (bit 62: 0 = the whole update of
-// the launch, 1 = the kick-drift half of a pair whose kick is already stored).
+// 0-39), box (bits 40-59) and what is left to do (bit 62: the kick-drift half of
+// a pair whose kick is already stored; bit 61: only the time-step estimate of
+// the stored state; neither: the whole update of the launch).
 constexpr long long kDeferPairKD = 1LL << 62;
+constexpr long long kDeferDtOnly = 1LL << 61;  // update done; only the time-step estimate is left
 __device__ __forceinline__ long long defer_entry(long long p, int box) {
     return p | ((long long)box << 40);
 }
@@ -653,7 +737,11 @@ __device__ __forceinline__ long long defer_entry(long long p, int box) {
 //
 // Tile counts for the next sort: particles that stay in the item's tile are
 // summed per block (one atomic), the others counted individually.
-template <int MODE>
+//
+// EST (closing kicks only): also NyxParticleContainer::estTimestep of the kicked
+// particles (cfl, a = a_cur) into dtmin, from the staged g; a compile-time
+// variant so that the kernels without it keep their register budget.
+template <int MODE, bool EST>
 __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int ntseg, const BoxDev* boxes,
                                                        const TileItem* __restrict__ items, const int* nitems,
                                                        const double* __restrict__ phi, double* __restrict__ x,
@@ -662,7 +750,8 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
                                                        double* __restrict__ vz, double dt, double a_prev,
                                                        double a_cur, double a_prev2, double a_cur2,
                                                        int* __restrict__ key, int* counts,
-                                                       long long* __restrict__ defer, int* ndefer, int* err) {
+                                                       long long* __restrict__ defer, int* ndefer, double cfl,
+                                                       unsigned long long* dtmin, int* err) {
     extern __shared__ double sphi[];
     __shared__ int s_same[kBlock / kWave];
     // items are in (roughly) tile order; giving each XCD a contiguous range of them
@@ -766,6 +855,10 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
     const double a_cur2_inv = 1.0 / a_cur2;
     const double dt_a_cur2_inv = dt * a_cur2_inv;
     int same = 0;
+    // time-step estimate of the state after the (closing) kick, a = a_cur
+    constexpr bool est = EST && MODE != kKickDrift;
+    const double adxi0 = dxi0 / a_cur, adxi1 = dxi1 / a_cur, adxi2 = dxi2 / a_cur;
+    double dtmin_t = 1e50;
     // one particle: position (px, py, pz), velocity (qx, qy, qz), index q in the item
     auto one = [&](int q, double px, double py, double pz, double qx, double qy, double qz) {
         double xint, yint, zint;
@@ -811,10 +904,20 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
                 const double syp = periodic_shift(py, B.plo[1], B.rhi[1], B.len[1], err);
                 const double szp = periodic_shift(pz, B.plo[2], B.rhi[2], B.len[2], err);
                 const bool moved = sxp != px || syp != py || szp != pz;
+                if (est && !moved) {
+                    // cell = floor((x - plo) dxi) is i or i-1 of the CIC index: in the staged g
+                    const int ci = (int)floor((px - B.plo[0]) * dxi0) - (i - (li + o0));
+                    const int cj = (int)floor((py - B.plo[1]) * dxi1) - (j - (lj + o1));
+                    const int ck = (int)floor((pz - B.plo[2]) * dxi2) - (k - (lk + o2));
+                    const double* gc = sg + 3 * (((ck - o2 + 1) * GY + (cj - o1 + 1)) * GX + (ci - o0 + 1));
+                    const double d = particle_dt(ux, uy, uz, adxi0, adxi1, adxi2, gc[0], gc[1], gc[2], dxi0, cfl);
+                    dtmin_t = d < dtmin_t ? d : dtmin_t;
+                }
                 if (MODE == kKick) {
                     if (sxp != px) X[q] = sxp;
                     if (syp != py) Y[q] = syp;
                     if (szp != pz) Z[q] = szp;
+                    if (est && moved) defer[atomicAdd(ndefer, 1)] = defer_entry(it.p0 + q, box) | kDeferDtOnly;
                 } else if (moved) {
                     // store the kick; the kick-drift follows in k_kick_deferred
                     X[q] = sxp;
@@ -858,6 +961,7 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
         one(q0, ax, ay, az, aux, auy, auz);
         if (q0 + kBlock < it.np) one(q0 + kBlock, bx_, by_, bz_, bux, buy, buz);
     }
+    if (est) block_min_to(dtmin_t, dtmin + box);
     if (MODE != kKick) {
         for (int o = kWave / 2; o > 0; o >>= 1) same += __shfl_xor(same, o);
         if (lane == 0) s_same[wave] = same;
@@ -880,17 +984,23 @@ __global__ void __launch_bounds__(kBlock) k_kick_deferred(const long long* __res
                                                           double* __restrict__ vy, double* __restrict__ vz,
                                                           double dt, double a_prev, double a_cur, double a_prev2,
                                                           double a_cur2, int* __restrict__ key, int* counts,
-                                                          int* err) {
+                                                          double cfl, unsigned long long* dtmin, int* err) {
     const int n = *ndefer;
+    const bool est = MODE != kKickDrift && cfl > 0;
     const double half_dt = 0.5 * dt;
     for (int q = blockIdx.x * kBlock + threadIdx.x; q < n; q += gridDim.x * kBlock) {
         const long long e = defer[q];
         const long long p = e & ((1LL << 40) - 1);
-        const BoxDev& B = boxes[(int)((e >> 40) & 0xFFFFF)];
+        const int box = (int)((e >> 40) & 0xFFFFF);
+        const BoxDev& B = boxes[box];
         const double* P = phi + B.cell_off;
         double px = x[p], py = y[p], pz = z[p];
         double ux = vx[p], uy = vy[p], uz = vz[p];
         double val[3];
+        if (e & kDeferDtOnly) {
+            atomicMin(dtmin + box, dbits(particle_dt_global(B, P, px, py, pz, ux, uy, uz, a_cur, cfl)));
+            continue;
+        }
         if (!(e & kDeferPairKD)) {
             const double a_cur_inv = 1.0 / a_cur;
             gather_grad(B, P, px, py, pz, val, err);
@@ -915,6 +1025,8 @@ __global__ void __launch_bounds__(kBlock) k_kick_deferred(const long long* __res
             const double sxp = periodic_shift(px, B.plo[0], B.rhi[0], B.len[0], err);
             const double syp = periodic_shift(py, B.plo[1], B.rhi[1], B.len[1], err);
             const double szp = periodic_shift(pz, B.plo[2], B.rhi[2], B.len[2], err);
+            if (est)
+                atomicMin(dtmin + box, dbits(particle_dt_global(B, P, sxp, syp, szp, ux, uy, uz, a_cur, cfl)));
             if (MODE == kKick) {
                 vx[p] = ux; vy[p] = uy; vz[p] = uz;
                 if (sxp != px) x[p] = sxp;
@@ -928,7 +1040,9 @@ __global__ void __launch_bounds__(kBlock) k_kick_deferred(const long long* __res
             pz = szp;
             if (moved) gather_grad(B, P, px, py, pz, val, err);
         } else {
-            // pair whose kick is stored: kick-drift at the shifted position
+            // pair whose kick is stored: its estimate, then the kick-drift at the
+            // shifted position
+            if (est) atomicMin(dtmin + box, dbits(particle_dt_global(B, P, px, py, pz, ux, uy, uz, a_cur, cfl)));
             gather_grad(B, P, px, py, pz, val, err);
         }
         const double a_cur2_inv = 1.0 / a_cur2;

@@ -49,7 +49,7 @@ class PMContext:
 
     def __init__(self, geoms, precision="fp64", tile=None, nu1=2, nu2=2, max_iter=200, smoother="fused",
                  kick="tiled", sweep_shape=None, sweep_zchunk=0, kick_chunk=0, pair_kicks=True,
-                 coarse="workgroup"):
+                 coarse="workgroup", particle_cfl=0.0):
         if isinstance(geoms, BoxGeom):
             geoms = [geoms]
         self.geoms = list(geoms)
@@ -71,6 +71,7 @@ class PMContext:
         o.kick_chunk = int(kick_chunk)
         o.separate_kicks = 0 if pair_kicks else 1
         o.coarse = 1 if coarse == "per_level" else 0
+        o.particle_cfl = float(particle_cfl)
         if tile is not None:
             for d in range(3):
                 o.tile[d] = int(tile[d])
@@ -182,6 +183,13 @@ class PMContext:
         g = np.empty((3, self.num_particles(box)))
         self._ck(self.L.nyxpm_gather_accel(self.h, box, ptr(g)))
         return g
+
+    def est_time_step(self, a=1.0, cfl=0.5):
+        """NyxParticleContainer::estTimestep per box (array of nbox; -1e50 for a box
+        without particles)."""
+        out = np.zeros(self.nbox)
+        self._ck(self.L.nyxpm_est_time_step(self.h, float(a), float(cfl), ptr(out)))
+        return out
 
     # -- device-resident variants: `addr` is a device address (e.g. a torch
     #    tensor's data_ptr()) of an FP64 array of the right length; the library
@@ -312,6 +320,12 @@ class DarkMatterParticleContainer:
         self.ctx.redistribute()
 
     RedistributeLocal = Redistribute
+
+    def estTimestep(self, acceleration=None, a=1.0, lev=0, cfl=0.5):
+        """NyxParticleContainer::estTimestep (NyxParticleContainer.H:327-410).  The
+        acceleration argument is not read: the gravity of the context's phi is
+        used, as Nyx passes get_new_data(Gravity_Type).  -1e50 without particles."""
+        return float(self.ctx.est_time_step(a, cfl)[self.box])
 
     def sumParticleMass(self, level=0):
         """NyxParticleContainer.H:70-79."""

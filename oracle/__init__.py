@@ -66,6 +66,8 @@ def lib():
     L.orc_mg_solve.restype = _int
     L.orc_grav_vector.argtypes = [_vp, _vp, _vp, _vp, _vp]
     L.orc_grav_vector.restype = None
+    L.orc_est_time_step.argtypes = [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d]
+    L.orc_est_time_step.restype = _d
     L.orc_update_particles.argtypes = [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp,
                                        _vp, _d, _d, _d, _int]
     L.orc_update_particles.restype = None
@@ -186,6 +188,17 @@ def grav_vector(n, plo, phi_hi, phi):
     g = np.empty((3,) + grid_shape(n), dtype=np.float64)
     lib().orc_grav_vector(_p(n), _p(plo), _p(phi_hi), _p(phi), _p(g))
     return g
+
+
+def est_time_step(n, plo, phi_hi, pos, vel, g, a=1.0, cfl=0.5):
+    """NyxParticleContainer::estTimestep with the gravity field g (3, nz, ny, nx)."""
+    n, plo, phi_hi = _geom(n, plo, phi_hi)
+    pos = [_c64(p) for p in pos]
+    vel = [_c64(v) for v in vel]
+    g = _c64(g)
+    return lib().orc_est_time_step(_p(n), _p(plo), _p(phi_hi), len(pos[0]), _p(pos[0]), _p(pos[1]),
+                                   _p(pos[2]), _p(vel[0]), _p(vel[1]), _p(vel[2]), _p(g), float(a),
+                                   float(cfl))
 
 
 def update_particles(n, plo, phi_hi, pos, vel, g, dt, a_prev, a_cur, do_move):

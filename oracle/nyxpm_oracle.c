@@ -599,6 +599,50 @@ void orc_grav_vector(const int n[3], const double plo[3], const double phi_hi[3]
 }
 
 /*
+ * NyxParticleContainer::estTimestep(acceleration, a, lev, cfl)
+ * (NyxParticleContainer.H:327-410), called by Nyx::particle_est_time_step with the
+ * new cell-centred gravity (NyxParticles.cpp:1028-1036).  Per particle:
+ *   cell = floor((x - plo) * dxi)                    (no CIC shift; wrapped here,
+ *                                                    the reference reads ghosts)
+ *   dt_p = cfl / max_d(|v_d| * (dxi_d / a))         (1e50 if that max is 0)
+ *   |g|  = sqrt(gx*gx + gy*gy + gz*gz) at the cell; if > 0,
+ *   dt_p = min(dt_p, 1 / sqrt(|g| * dxi_x))
+ * and the result is the minimum over particles, -1e50 without particles.  g is
+ * the 3-field gravity of orc_grav_vector.
+ */
+double orc_est_time_step(const int n[3], const double plo[3], const double phi_hi[3], int64_t np,
+                         const double *x, const double *y, const double *z, const double *vx,
+                         const double *vy, const double *vz, const double *g, double a, double cfl) {
+    if (np == 0) return -1e50;
+    double dx[3], dxi[3];
+    geom_dx(n, plo, phi_hi, dx, dxi);
+    const double adxi[3] = {dxi[0] / a, dxi[1] / a, dxi[2] / a};
+    const size_t nc = (size_t)n[0] * n[1] * n[2];
+    double dt = 1e50;
+#pragma omp parallel for schedule(static) reduction(min : dt)
+    for (int64_t p = 0; p < np; ++p) {
+        const int i = (int)wrapi((int64_t)floor((x[p] - plo[0]) * dxi[0]), n[0]);
+        const int j = (int)wrapi((int64_t)floor((y[p] - plo[1]) * dxi[1]), n[1]);
+        const int k = (int)wrapi((int64_t)floor((z[p] - plo[2]) * dxi[2]), n[2]);
+        const double mx = fabs(vx[p]) * adxi[0];
+        const double my = fabs(vy[p]) * adxi[1];
+        const double mz = fabs(vz[p]) * adxi[2];
+        double m = mx > my ? mx : my;
+        m = m > mz ? m : mz;
+        double dtp = (m > 0) ? (cfl / m) : 1e50;
+        const size_t c = cidx(i, j, k, n);
+        const double ax = g[c], ay = g[nc + c], az = g[2 * nc + c];
+        const double mag = sqrt(ax * ax + ay * ay + az * az);
+        if (mag > 0) {
+            const double da = 1 / sqrt(mag * dxi[0]);
+            dtp = dtp < da ? dtp : da;
+        }
+        dt = dtp < dt ? dtp : dt;
+    }
+    return dt;
+}
+
+/*
  * update_dm_particle_single (DarkMatterParticleContainer.cpp:249-306), with the
  * ghost-cell lookups of the reference replaced by periodic index wrapping:
  *   l = (x-plo)*dxi + 0.5, i = floor(l), sx = {1-(l-i), l-i}

@@ -176,3 +176,39 @@ def test_coarse_workgroup_equals_per_level(nyx, precision):
         for d in range(3):
             assert np.array_equal(a[b][0][d], b_[b][0][d])
             assert np.array_equal(a[b][1][d], b_[b][1][d])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_est_time_step_fused_equals_standalone(nyx, precision, use_graph):
+    # the estimate the closing kicks keep (particle_cfl) equals a separate pass over
+    # the same state and leaves the trajectory unchanged; fast particles cross the
+    # periodic boundary (deferred paths); a box without particles gives -1e50
+    geoms = [nyx.BoxGeom((32, 32, 32)), nyx.BoxGeom((24, 40, 16), (0.0, 0.0, 0.0), (0.75, 1.25, 0.5)),
+             nyx.BoxGeom((16, 16, 16))]
+    inputs = []
+    for i, g in enumerate(geoms[:2]):
+        r = np.random.default_rng(700 + i)
+        npart = g.ncell
+        pos = [r.uniform(g.prob_lo[d], g.prob_hi[d], npart) for d in range(3)]
+        vel = [r.normal(0.0, 8.0, npart) for _ in range(3)]
+        inputs.append((pos, vel, None, 1.0 / npart))
+    res = []
+    for cfl in (0.3, 0.0):
+        with nyx.PMContext(geoms, precision=precision, particle_cfl=cfl) as c:
+            for b, inp in enumerate(inputs):
+                _upload(c, b, inp)
+            c.pm_init(1.0, 1.0, 1e-10)
+            c.pm_steps(5, 1e-3, rel_tol=1e-10, use_graph=use_graph)
+            dt = c.est_time_step(1.0, 0.3)
+            c.move_kick_drift(1e-3, 1.0, 1.0)
+            c.move_kick(1e-3, 1.0, 1.0)
+            dt2 = c.est_time_step(1.0, 0.3)
+            res.append((dt, dt2, [c.download(b) for b in range(2)]))
+    (fa, fb, fp), (sa, sb, sp) = res
+    assert np.array_equal(fa, sa) and np.array_equal(fb, sb)
+    assert fa[2] == -1e50 and 0 < fa[0] < 1e50 and 0 < fa[1] < 1e50
+    for b in range(2):
+        for d in range(3):
+            assert np.array_equal(fp[b][0][d], sp[b][0][d])
+            assert np.array_equal(fp[b][1][d], sp[b][1][d])

@@ -380,3 +380,25 @@ def test_far_particle_is_an_error(nyx):
         with pytest.raises(nyx.NyxPMError) as ei:
             c.assign_density()
         assert ei.value.code == 6
+
+
+# ---------------------------------------------------------------------------
+# estTimestep
+# ---------------------------------------------------------------------------
+
+def test_est_time_step_vs_oracle(nyx, orc):
+    # the GPU estimate equals the oracle's on the same state, bit for bit (same
+    # per-particle doubles; the minimum is exact)
+    n = (64, 64, 64)
+    pos, vel, m = nyx.ics.grf_lattice(64, 11, -2.0, 0.5, 0.1)
+    with ctx1(nyx, n) as c:
+        c.upload(0, pos, vel, m_const=m)
+        c.pm_init(1.0, 1.0, 1e-10)
+        c.pm_steps(2, 1e-3, rel_tol=1e-10)
+        p, v = c.download()
+        g = orc.grav_vector(n, (0, 0, 0), (1, 1, 1), c.get_phi())
+        for a, cfl in ((1.0, 0.5), (0.3, 0.1), (1.0, 1e3)):
+            dt = c.est_time_step(a, cfl)[0]
+            assert dt == orc.est_time_step(n, (0, 0, 0), (1, 1, 1), p, v, g, a, cfl)
+            pc = nyx.DarkMatterParticleContainer(c)
+            assert pc.estTimestep(None, a, 0, cfl) == dt

@@ -117,3 +117,49 @@ def test_faithful_and_canonical_agree(orc):
     for d in range(3):
         dv = np.abs(v1[d] - v2[d]).max()
         assert dv < 1e-9 * np.abs(v1[d]).max()
+
+
+# ---------------------------------------------------------------------------
+# estTimestep (NyxParticleContainer.H:327-410)
+# ---------------------------------------------------------------------------
+
+def test_est_time_step_velocity_known_answer(orc):
+    # g = 0: dt = cfl / max_d(|v_d| dxi_d / a); n = 8 on the unit box, a = 2:
+    # |v| (0.2, 0.5, 0.1) -> 0.5 * 8 / 2 = 2 -> 0.5 / 2 = 0.25; a slower particle
+    # does not change the minimum
+    n = (8, 8, 8)
+    g = np.zeros((3, 8, 8, 8))
+    pos = [np.array([0.3, 0.9]), np.array([0.5, 0.1]), np.array([0.7, 0.4])]
+    vel = [np.array([0.2, 0.01]), np.array([-0.5, 0.0]), np.array([0.1, -0.02])]
+    assert orc.est_time_step(n, (0, 0, 0), (1, 1, 1), pos, vel, g, a=2.0, cfl=0.5) == 0.25
+    # anisotropic cells: the y spacing (dxi = 16) dominates
+    assert orc.est_time_step((8, 16, 8), (0, 0, 0), (1, 1, 1), pos, vel, np.zeros((3, 8, 16, 8)),
+                             a=1.0, cfl=0.5) == 0.5 / (0.5 * 16)
+
+
+def test_est_time_step_no_particles(orc):
+    e = np.array([])
+    assert orc.est_time_step((8, 8, 8), (0, 0, 0), (1, 1, 1), [e, e, e], [e, e, e],
+                             np.zeros((3, 8, 8, 8))) == -1e50
+
+
+def test_est_time_step_gravity_single_mode(orc):
+    # phi = A cos(2 pi x + s): the face-averaged gradient at cell i is
+    # g_x = -n A sin(2 pi (i + 1/2) / n + s) sin(2 pi / n) (closed form); at rest the
+    # estimate is 1 / sqrt(|g_x| n) of the particle's cell floor(x n) -- cell 3
+    # here, near its upper edge, where the CIC index floor(x n + 1/2) is 4
+    n, A, sh = 8, 0.7, 0.3
+    x = (np.arange(n) + 0.5) / n
+    phi = np.broadcast_to(A * np.cos(2 * np.pi * x + sh)[None, None, :], (n, n, n)).copy()
+    g = orc.grav_vector((n, n, n), (0, 0, 0), (1, 1, 1), phi)
+    gx = lambda i: -n * A * np.sin(2 * np.pi * (i + 0.5) / n + sh) * np.sin(2 * np.pi / n)
+    assert np.allclose(g[0, 0, 0, :], [gx(i) for i in range(n)], rtol=0, atol=1e-13)
+    z = np.zeros(1)
+    pos = [np.array([4.0 / n - 1e-9]), np.array([0.5]), np.array([0.5])]
+    dt = orc.est_time_step((n, n, n), (0, 0, 0), (1, 1, 1), pos, [z, z, z], g, a=1.0, cfl=0.5)
+    assert abs(dt - 1 / np.sqrt(abs(gx(3)) * n)) <= 1e-14
+    assert abs(dt - 1 / np.sqrt(abs(gx(4)) * n)) > 1e-3  # not the CIC cell
+    # a fast particle: the velocity term wins
+    vel = [np.array([100.0]), z, z]
+    dt2 = orc.est_time_step((n, n, n), (0, 0, 0), (1, 1, 1), pos, vel, g, a=1.0, cfl=0.5)
+    assert dt2 == 0.5 / (100.0 * n)
