@@ -122,6 +122,12 @@ int nyxpm_particles_upload_device(nyxpm_ctx *ctx, int box, int64_t np,
                                   const double *x, const double *y, const double *z,
                                   const double *vx, const double *vy, const double *vz,
                                   const double *m, double m_const);
+/* DMPC->InitFromBinaryFile(file, AMREX_SPACEDIM + 1) (NyxParticles.cpp:525-545)
+ * for box `box`: reads Nyx's binary particle file (int64 NP, int32 DM = 3, int32
+ * NX >= 4, then NP records x y z m vx vy vz [more extras, skipped], FP64 or FP32
+ * by file size) and uploads it as nyxpm_particles_upload does (per-particle
+ * mass).  np_out (may be NULL) receives NP. */
+int nyxpm_particles_load_binary(nyxpm_ctx *ctx, int box, const char *path, int64_t *np_out);
 /* Copy box `box`'s particles to host arrays, in upload (id) order.  Any pointer
  * may be NULL to skip that component. */
 int nyxpm_particles_download(nyxpm_ctx *ctx, int box,

@@ -1499,6 +1499,65 @@ int nyxpm_particles_upload_device(nyxpm_ctx* c, int box, int64_t np, const doubl
     return upload_impl(c, box, np, x, y, z, vx, vy, vz, m, m_const, hipMemcpyDeviceToDevice);
 }
 
+// Nyx's binary particle file (particle_init_type = BinaryFile, read by
+// DMPC->InitFromBinaryFile(file, AMREX_SPACEDIM + 1), NyxParticles.cpp:525-545):
+// int64 NP, int32 DM, int32 NX, then NP records of DM positions followed by NX
+// extra values, of which the first DM + 1 are the mass and the velocity (the
+// particle's rdata(0..3), DarkMatterParticleContainer.H:14-18).  Values are
+// FP64 or FP32, told apart by the file size.
+int nyxpm_particles_load_binary(nyxpm_ctx* c, int box, const char* path, int64_t* np_out) {
+    if (!c || !path) return NYXPM_ERR_ARG;
+    int rc;
+    if ((rc = check_box(c, box))) return rc;
+    FILE* f = std::fopen(path, "rb");
+    if (!f) FAIL(c, NYXPM_ERR_ARG, "cannot open %s", path);
+    struct Closer {
+        FILE* f;
+        ~Closer() { std::fclose(f); }
+    } closer{f};
+    long long np = 0;
+    int dm = 0, nx = 0;
+    if (std::fread(&np, sizeof(np), 1, f) != 1 || std::fread(&dm, sizeof(dm), 1, f) != 1 ||
+        std::fread(&nx, sizeof(nx), 1, f) != 1)
+        FAIL(c, NYXPM_ERR_ARG, "%s: short header", path);
+    if (dm != 3 || nx < 4 || np < 0 || np > (long long)0x7fffffff)
+        FAIL(c, NYXPM_ERR_ARG, "%s: header NP=%lld DM=%d NX=%d (need DM = 3, NX >= 4)", path, np, dm, nx);
+    if (std::fseek(f, 0, SEEK_END) != 0) FAIL(c, NYXPM_ERR_ARG, "%s: cannot seek", path);
+    const long long bytes = std::ftell(f) - 16;
+    const long long vals = np * (long long)(dm + nx);
+    const int vsize = vals > 0 && bytes == vals * 8 ? 8 : (vals > 0 && bytes == vals * 4 ? 4 : (vals == 0 ? 8 : 0));
+    if (!vsize)
+        FAIL(c, NYXPM_ERR_ARG, "%s: %lld data bytes do not match %lld values of 4 or 8 bytes", path, bytes, vals);
+    if (std::fseek(f, 16, SEEK_SET) != 0) FAIL(c, NYXPM_ERR_ARG, "%s: cannot seek", path);
+    std::vector<double> a(7 * (size_t)np);
+    const int rec = dm + nx;
+    std::vector<unsigned char> buf((size_t)rec * vsize * 65536);
+    for (long long p0 = 0; p0 < np; p0 += 65536) {
+        const long long cnt = std::min(65536LL, np - p0);
+        if (std::fread(buf.data(), (size_t)rec * vsize, (size_t)cnt, f) != (size_t)cnt)
+            FAIL(c, NYXPM_ERR_ARG, "%s: short read", path);
+        for (long long q = 0; q < cnt; ++q) {
+            for (int v = 0; v < 7; ++v) {  // x y z m vx vy vz
+                const unsigned char* src = buf.data() + ((size_t)q * rec + v) * vsize;
+                double d;
+                if (vsize == 8) {
+                    std::memcpy(&d, src, 8);
+                } else {
+                    float fl;
+                    std::memcpy(&fl, src, 4);
+                    d = fl;
+                }
+                a[(size_t)v * np + p0 + q] = d;
+            }
+        }
+    }
+    const double* A = a.data();
+    rc = upload_impl(c, box, np, A, A + np, A + 2 * np, A + 4 * np, A + 5 * np, A + 6 * np, A + 3 * np, 0.0,
+                     hipMemcpyHostToDevice);
+    if (!rc && np_out) *np_out = np;
+    return rc;
+}
+
 int nyxpm_particles_download(nyxpm_ctx* c, int box, double* x, double* y, double* z, double* vx, double* vy,
                              double* vz) {
     if (!c) return NYXPM_ERR_ARG;

@@ -132,6 +132,12 @@ class PMContext:
         self._ck(self.L.nyxpm_particles_upload_device(self.h, box, int(np_), *[ctypes.c_void_p(p) for p in ptrs],
                                                       ctypes.c_void_p(m_ptr) if m_ptr else None, float(m_const)))
 
+    def load_binary(self, path, box=0):
+        """Nyx's binary particle file (InitFromBinaryFile) into box `box`; returns NP."""
+        n = ctypes.c_int64()
+        self._ck(self.L.nyxpm_particles_load_binary(self.h, box, str(path).encode(), ctypes.byref(n)))
+        return n.value
+
     def num_particles(self, box=0):
         return int(self.L.nyxpm_num_particles(self.h, box))
 
@@ -290,6 +296,12 @@ class DarkMatterParticleContainer:
     def InitFromArrays(self, pos, vel, mass=None, mass_const=1.0):
         """Stands in for the Ascii/Binary particle readers (NyxParticles.cpp:379-744)."""
         self.ctx.upload(self.box, pos, vel, mass, mass_const)
+
+    def InitFromBinaryFile(self, file, extradata=4):
+        """NyxParticleContainer's InitFromBinaryFile(file, AMREX_SPACEDIM + 1)."""
+        if extradata != 4:
+            raise ValueError("extradata must be AMREX_SPACEDIM + 1 (mass and velocity)")
+        return self.ctx.load_binary(file, self.box)
 
     def AssignDensitySingleLevel(self, level=0, ncomp=1, particle_lvl_offset=0):
         """NyxParticleContainer.H:90-93: CIC mass density (x 1/vol), periodic."""
