@@ -25,6 +25,8 @@
  *                                                  (NyxParticleContainer.H:123-136)
  *   nyxpm_pm_init             the phi solve of Nyx::post_init (Source/Driver/Nyx.cpp:1958-2006)
  *   nyxpm_pm_steps            the particle-only advance of Nyx::advance_particles_only
+ *   nyxpm_pm_steps_comoving   the same with a(t) from Nyx::integrate_comoving_a
+ *                             (Source/Driver/comoving.cpp:630-689)
  *                             followed by post_timestep's redistribute
  *                                                  (Source/TimeStep/Nyx_advance_particles.cpp:9-272;
  *                                                   Source/Driver/Nyx.cpp:1421-1436)
@@ -200,6 +202,19 @@ int nyxpm_pm_init(nyxpm_ctx *ctx, double a, double four_pi_G, double rel_tol, do
 int nyxpm_pm_steps(nyxpm_ctx *ctx, int nsteps, double dt, double a_old, double a_new,
                    double four_pi_G, double rel_tol, double abs_tol, int nyx_faithful,
                    int use_graph, int *iters);
+/* Nyx::integrate_comoving_a (comoving.cpp:630-689): a after dt from old_a, RK2 on
+ * da/dt = 100 h sqrt(omega_m/a + omega_r/a^2 + (1-omega_m-omega_r) a^2) with step
+ * doubling to 1e-8 (comoving_type > 0; otherwise da/dt = h).  Host arithmetic only. */
+int nyxpm_integrate_comoving_a(double old_a, double dt, double omega_m, double omega_r,
+                               double h, int comoving_type, double *new_a);
+/* nsteps KDK steps in comoving coordinates: the scale factor is integrated by
+ * nyxpm_integrate_comoving_a each step (a_old = *a_io at the start, *a_io = a at
+ * the end); step s runs from a_s to a_(s+1), a_half = their mean
+ * (Nyx_advance_particles.cpp:180,245), the Rhs is scaled by 1/a_(s+1). */
+int nyxpm_pm_steps_comoving(nyxpm_ctx *ctx, int nsteps, double dt, double *a_io,
+                            double omega_m, double omega_r, double h, int comoving_type,
+                            double four_pi_G, double rel_tol, double abs_tol, int nyx_faithful,
+                            int *iters);
 
 /* ---- tuning ------------------------------------------------------------- */
 /* Autotune the fused smoother: for every multigrid depth (and precision in

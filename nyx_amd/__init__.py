@@ -9,5 +9,6 @@ in ``include/nyxpm.h``).  This package is its host-side interface:
 * ``ics``                          seeded synthetic initial conditions
 """
 from ._lib import NyxPMError, load, LIB_PATH, NYXPM_FP64, NYXPM_MIXED  # noqa: F401
-from .pm import BoxGeom, PMContext, DarkMatterParticleContainer, Gravity, advance_particles_only  # noqa: F401
+from .pm import (BoxGeom, PMContext, DarkMatterParticleContainer, Gravity, advance_particles_only,  # noqa: F401
+                 integrate_comoving_a)
 from . import ics  # noqa: F401

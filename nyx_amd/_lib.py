@@ -82,6 +82,8 @@ SIGNATURES = [
     ("nyxpm_redistribute", _i, [_vp]),
     ("nyxpm_pm_init", _i, [_vp, _d, _d, _d, _d]),
     ("nyxpm_pm_steps", _i, [_vp, _i, _d, _d, _d, _d, _d, _d, _i, _i, _vp]),
+    ("nyxpm_integrate_comoving_a", _i, [_d, _d, _d, _d, _d, _i, _vp]),
+    ("nyxpm_pm_steps_comoving", _i, [_vp, _i, _d, _vp, _d, _d, _d, _i, _d, _d, _d, _i, _vp]),
     ("nyxpm_autotune", _i, [_vp, _i, _vp, _i]),
     ("nyxpm_timing_enable", _i, [_vp, _i]),
     ("nyxpm_timing_get", _i, [_vp, _s, _vp, _vp, _vp]),

@@ -1220,16 +1220,50 @@ int step_body(nyxpm_ctx* c, double dt, double a_old, double a_new, double four_p
 // A single-solve KDK step whose opening kick-drift was already done (by the
 // previous step's pair kick or by the prologue of nyxpm_pm_steps).  With
 // pair_next the closing kick is fused with the next step's kick-drift.
+// next_a_new > 0: pair the closing kick with the next step's kick-drift, whose
+// scale factors are next_a_old -> next_a_new (a_half = their mean).
 int step_tail(nyxpm_ctx* c, double dt, double a_old, double a_new, double four_pi_G, double rel, double abs_tol,
-              int graph_cycles, bool pair_next) {
+              int graph_cycles, double next_a_old, double next_a_new) {
     int rc;
     const double a_half = 0.5 * (a_old + a_new);
     if ((rc = phase_sort(c))) return rc;
     if ((rc = phase_deposit(c, false))) return rc;
     if ((rc = phase_rhs(c, a_new, four_pi_G))) return rc;
     if ((rc = solve(c, rel, abs_tol, 0, graph_cycles))) return rc;
-    if (pair_next) return phase_kick<kKickPair>(c, dt, a_half, a_new, a_old, a_half);
+    if (next_a_new > 0.0)
+        return phase_kick<kKickPair>(c, dt, a_half, a_new, next_a_old, 0.5 * (next_a_old + next_a_new));
     return phase_kick<kKick>(c, dt, a_half, a_new);
+}
+
+// Nyx::integrate_comoving_a (comoving.cpp:630-689): RK2 (Heun) on
+// da/dt = H0 sqrt(OmM/a + OmR/a^2 + OmL a^2), H0 = 100 h, OmL = 1 - OmM - OmR
+// (comoving_type > 0; otherwise da/dt = h), with 2^(iter-1) substeps, doubling
+// until two successive results agree to 1e-8 relative (at most 11 passes).
+double integrate_comoving_a(double old_a, double dt, double om_m, double om_r, double h, int comoving_type) {
+    if (h == 0.0) return old_a;
+    const double H0 = h * 100.0;
+    const double om_l = 1.0 - om_m - om_r;
+    double prev = 2.0, out = old_a;
+    for (int iter = 1; iter <= 11; ++iter) {
+        const int nsteps = 1 << (iter - 1);
+        const double Dt = dt / nsteps;
+        double end_a = old_a;
+        for (int j = 1; j <= nsteps; ++j) {
+            const double start_a = end_a;
+            const double s0 = comoving_type > 0
+                                  ? H0 * std::sqrt(om_m / start_a + om_r / (start_a * start_a) + om_l * start_a * start_a)
+                                  : h;
+            end_a = start_a + s0 * Dt;
+            const double s1 = comoving_type > 0
+                                  ? H0 * std::sqrt(om_m / end_a + om_r / (end_a * end_a) + om_l * end_a * end_a)
+                                  : h;
+            end_a = start_a + 0.5 * (s0 + s1) * Dt;
+        }
+        out = end_a;
+        if (std::fabs(1.0 - out / prev) <= 1e-8) break;
+        prev = out;
+    }
+    return out;
 }
 
 // Whether p is device memory (pointer attributes; pageable host memory is not
@@ -1813,7 +1847,9 @@ int nyxpm_pm_steps(nyxpm_ctx* c, int nsteps, double dt, double a_old, double a_n
     const double a_half = 0.5 * (a_old + a_new);
     if (fused && nsteps > 0 && (rc = phase_kick<kKickDrift>(c, dt, a_old, a_half))) return rc;
     auto one_step = [&](int s) -> int {
-        if (fused) return step_tail(c, dt, a_old, a_new, four_pi_G, rel_tol, abs_tol, 0, s + 1 < nsteps);
+        // every step of this call runs from a_old to a_new
+        if (fused)
+            return step_tail(c, dt, a_old, a_new, four_pi_G, rel_tol, abs_tol, 0, a_old, s + 1 < nsteps ? a_new : 0.0);
         return step_body(c, dt, a_old, a_new, four_pi_G, rel_tol, abs_tol, nyx_faithful, 0);
     };
     for (int s = 0; s < nsteps; ++s) {
@@ -1840,7 +1876,7 @@ int nyxpm_pm_steps(nyxpm_ctx* c, int nsteps, double dt, double a_old, double a_n
             HIPCK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             int cur0 = c->cur;
             for (int r = 0; r < 2 && !rc; ++r)
-                rc = fused ? step_tail(c, dt, a_old, a_new, four_pi_G, rel_tol, abs_tol, c->graph_cycles, true)
+                rc = fused ? step_tail(c, dt, a_old, a_new, four_pi_G, rel_tol, abs_tol, c->graph_cycles, a_old, a_new)
                            : step_body(c, dt, a_old, a_new, four_pi_G, rel_tol, abs_tol, nyx_faithful,
                                        c->graph_cycles);
             hipGraph_t g = nullptr;
@@ -1860,6 +1896,46 @@ int nyxpm_pm_steps(nyxpm_ctx* c, int nsteps, double dt, double a_old, double a_n
         // a box still active after the recorded cycles has not converged
         if ((rc = solve_results(c, iters, nullptr, false))) return rc;
     }
+    c->sorted_valid = false;
+    c->keys_valid = false;
+    c->density_valid = false;
+    return err_check(c);
+}
+
+int nyxpm_integrate_comoving_a(double old_a, double dt, double omega_m, double omega_r, double h,
+                               int comoving_type, double* new_a) {
+    if (!new_a || !(old_a > 0.0)) return NYXPM_ERR_ARG;
+    *new_a = integrate_comoving_a(old_a, dt, omega_m, omega_r, h, comoving_type);
+    return NYXPM_OK;
+}
+
+int nyxpm_pm_steps_comoving(nyxpm_ctx* c, int nsteps, double dt, double* a_io, double omega_m, double omega_r,
+                            double h, int comoving_type, double four_pi_G, double rel_tol, double abs_tol,
+                            int nyx_faithful, int* iters) {
+    if (!c || nsteps < 0 || !a_io) return NYXPM_ERR_ARG;
+    int rc;
+    if ((rc = require_particles(c))) return rc;
+    if (!c->phi_valid && !nyx_faithful) FAIL(c, NYXPM_ERR_STATE, "call nyxpm_pm_init before stepping");
+    if (!(*a_io > 0.0)) FAIL(c, NYXPM_ERR_ARG, "scale factor must be > 0");
+    // a at the step boundaries (Nyx::integrate_comoving_a, once per step)
+    std::vector<double> a(nsteps + 1);
+    a[0] = *a_io;
+    for (int s = 0; s < nsteps; ++s) {
+        a[s + 1] = integrate_comoving_a(a[s], dt, omega_m, omega_r, h, comoving_type);
+        if (!(a[s + 1] > 0.0)) FAIL(c, NYXPM_ERR_ARG, "scale factor became %g", a[s + 1]);
+    }
+    const bool fused = !nyx_faithful && !c->separate_kicks;
+    if (fused && nsteps > 0 && (rc = phase_kick<kKickDrift>(c, dt, a[0], 0.5 * (a[0] + a[1])))) return rc;
+    for (int s = 0; s < nsteps; ++s) {
+        if (fused)
+            rc = step_tail(c, dt, a[s], a[s + 1], four_pi_G, rel_tol, abs_tol, 0, a[s + 1],
+                           s + 1 < nsteps ? a[s + 2] : 0.0);
+        else
+            rc = step_body(c, dt, a[s], a[s + 1], four_pi_G, rel_tol, abs_tol, nyx_faithful, 0);
+        if (rc) return rc;
+        if ((rc = solve_results(c, iters, nullptr, false))) return rc;
+    }
+    *a_io = a[nsteps];
     c->sorted_valid = false;
     c->keys_valid = false;
     c->density_valid = false;

@@ -44,6 +44,16 @@ class BoxGeom:
         return int(np.prod(self.n_cell))
 
 
+def integrate_comoving_a(old_a, dt, omega_m=1.0, omega_r=0.0, h=0.675, comoving_type=1):
+    """Nyx::integrate_comoving_a (Source/Driver/comoving.cpp:630-689)."""
+    out = ctypes.c_double()
+    rc = load().nyxpm_integrate_comoving_a(float(old_a), float(dt), float(omega_m), float(omega_r), float(h),
+                                           int(comoving_type), ctypes.byref(out))
+    if rc:
+        raise NyxPMError(rc, "integrate_comoving_a: invalid arguments")
+    return out.value
+
+
 class PMContext:
     """One library context: a batch of independent periodic boxes on one GPU."""
 
@@ -251,6 +261,16 @@ class PMContext:
         return it
 
     # -- tuning ---------------------------------------------------------------
+    def pm_steps_comoving(self, nsteps, dt, a, omega_m=1.0, omega_r=0.0, h=0.675, comoving_type=1,
+                          four_pi_G=1.0, rel_tol=1e-10, abs_tol=0.0, nyx_faithful=False):
+        """nsteps KDK steps with a(t) integrated as Nyx does; returns (a_end, iters)."""
+        it = np.zeros(self.nbox, dtype=np.int32)
+        av = ctypes.c_double(a)
+        self._ck(self.L.nyxpm_pm_steps_comoving(self.h, int(nsteps), float(dt), ctypes.byref(av), float(omega_m),
+                                                float(omega_r), float(h), int(comoving_type), float(four_pi_G),
+                                                float(rel_tol), float(abs_tol), int(bool(nyx_faithful)), ptr(it)))
+        return av.value, it
+
     def autotune(self, fp64=True, fp32=True):
         """Time every fused-sweep column shape and z-chunk per multigrid depth on
         this context's levels and keep the fastest; returns the choices."""

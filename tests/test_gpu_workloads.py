@@ -68,3 +68,76 @@ def test_mixed_vs_fp64_per_step_and_power_spectrum(nyx):
         rho0 = c.assign_density()
     _, p0 = _power_spectrum(rho0)
     assert np.all(p64[low] != p0[low])
+
+
+# ---------------------------------------------------------------------------
+# comoving a(t): Zel'dovich pancake (Exec/GravityTests/zeldovich_dm)
+# ---------------------------------------------------------------------------
+
+H_SMALL = 0.675
+A_I, A_C, A_F = 1 / 101, 1 / 11, 1 / 21  # z = 100 start, caustic at z = 10, stop near z = 20
+
+
+def _zeldovich(nyx, n):
+    """Einstein-de Sitter, one sheet along x with wavelength L = 1: comoving
+    x = q - (a/a_c) sin(kq)/k, peculiar velocity u = a dx/dt
+    = -H0 sqrt(a)/a_c sin(kq)/k (analysis/zeldovich.py:3-7 and analytic.cpp:20-33
+    with the a*v convention); rho_bar = 1, so 4 pi G = 1.5 H0^2."""
+    H0 = 100 * H_SMALL
+    k = 2 * np.pi
+    g = (np.arange(n) + 0.5) / n
+    Z, Y, X = np.meshgrid(g, g, g, indexing="ij")
+    q = X.ravel()
+    S = np.sin(k * q) / k
+    pos = [np.mod(q - (A_I / A_C) * S, 1.0), Y.ravel().copy(), Z.ravel().copy()]
+    vel = [-H0 * np.sqrt(A_I) / A_C * S, np.zeros(n ** 3), np.zeros(n ** 3)]
+    return pos, vel, q, S, 1.0 / n ** 3, 1.5 * H0 ** 2
+
+
+def test_zeldovich_pancake_comoving(nyx):
+    n = 64
+    H0 = 100 * H_SMALL
+    pos, vel, q, S, m, G4 = _zeldovich(nyx, n)
+    with nyx.PMContext(nyx.BoxGeom((n, n, n))) as c:
+        c.upload(0, pos, vel, m_const=m)
+        c.pm_init(A_I, G4, 1e-10)
+        a = A_I
+        while a < A_F:
+            dt = 0.01 * a ** 1.5 / H0  # about a 1% change of a per step (relative_max_change_a)
+            a, _ = c.pm_steps_comoving(10, dt, a, 1.0, 0.0, H_SMALL, 1, four_pi_G=G4, rel_tol=1e-10)
+        p, v = c.download()
+    assert A_F <= a < A_C
+    D = a / A_C
+    dx = np.mod(p[0] - (q - D * S) + 0.5, 1.0) - 0.5
+    assert np.abs(dx).max() <= 0.02 * (D / (2 * np.pi))  # PM error, 1.2% at 64^3 (2.5% at 32^3)
+    u = -H0 * np.sqrt(a) / A_C * S
+    assert np.abs(v[0] - u).max() <= 0.025 * np.abs(u).max()
+    # a plane wave: nothing moves across the sheet
+    assert np.abs(v[1]).max() < 1e-9 and np.abs(v[2]).max() < 1e-9
+
+
+def test_comoving_steps_vs_oracle(nyx, orc):
+    # the scale factors of nyxpm_pm_steps_comoving, fed to the oracle's step
+    n = 32
+    H0 = 100 * H_SMALL
+    pos, vel, q, S, m, G4 = _zeldovich(nyx, n)
+    dt = 0.01 * A_I ** 1.5 / H0
+    with nyx.PMContext(nyx.BoxGeom((n, n, n))) as c:
+        c.upload(0, pos, vel, m_const=m)
+        c.pm_init(A_I, G4, 1e-10)
+        a_end, _ = c.pm_steps_comoving(6, dt, A_I, 1.0, 0.0, H_SMALL, 1, four_pi_G=G4, rel_tol=1e-10)
+        p, v = c.download()
+    nn = (n, n, n)
+    rp = [x.copy() for x in pos]
+    rv = [x.copy() for x in vel]
+    phi, _, _ = orc.solve_for_phi(nn, (0, 0, 0), (1, 1, 1), rp, m_const=m, four_pi_G=G4, a=A_I, rel_tol=1e-10)
+    a = A_I
+    for _ in range(6):
+        a_new = nyx.integrate_comoving_a(a, dt, 1.0, 0.0, H_SMALL, 1)
+        orc.pm_step(nn, (0, 0, 0), (1, 1, 1), rp, rv, phi, m_const=m, dt=dt, a_old=a, a_new=a_new,
+                    four_pi_G=G4, rel_tol=1e-10)
+        a = a_new
+    assert a == a_end
+    for d in range(3):
+        assert np.abs(p[d] - rp[d]).max() <= 1e-12
+        assert np.abs(v[d] - rv[d]).max() <= 1e-12 * max(np.abs(rv[0]).max(), 1e-30)
