@@ -144,6 +144,14 @@ int64_t nyxpm_num_particles(const nyxpm_ctx *ctx, int box);
  * deposits all boxes. */
 int nyxpm_assign_density(nyxpm_ctx *ctx, int box, double *rho_out);
 int nyxpm_sum_particle_mass(nyxpm_ctx *ctx, int box, double *mass_out);
+/* DarkMatterParticleContainer::AssignDensityAndVels (DarkMatterParticleContainer.cpp:
+ * 456-460, AssignDensity with AMREX_SPACEDIM + 1 components; used by
+ * ParticleDerive.cpp:126-168 and the Santa Barbara initialisation,
+ * NyxParticles.cpp:771-773) for box `box`: out (4 fields of n cells, host or
+ * device memory) receives the CIC mass density (= nyxpm_assign_density) and the
+ * CIC mass-weighted cell velocity sum(w m v_d) / sum(w m) for d = x, y, z, 0 in
+ * cells without mass.  Deterministic (exact fixed-point sums). */
+int nyxpm_assign_density_and_vels(nyxpm_ctx *ctx, int box, double *out);
 
 /* ---- Gravity ------------------------------------------------------------ */
 /* solve_for_phi on every box: Rhs = density of the last assign_density (or

@@ -68,6 +68,7 @@ SIGNATURES = [
     ("nyxpm_num_particles", _i64, [_vp, _i]),
     ("nyxpm_assign_density", _i, [_vp, _i, _vp]),
     ("nyxpm_sum_particle_mass", _i, [_vp, _i, _vp]),
+    ("nyxpm_assign_density_and_vels", _i, [_vp, _i, _vp]),
     ("nyxpm_solve_for_phi", _i, [_vp, _d, _d, _d, _d, _i, _vp, _vp]),
     ("nyxpm_solve_poisson", _i, [_vp, _i, _vp, _vp, _d, _d, _i, _vp, _vp]),
     ("nyxpm_get_grav_vector", _i, [_vp, _i, _vp]),

@@ -1631,6 +1631,79 @@ int nyxpm_assign_density(nyxpm_ctx* c, int box, double* rho_out) {
     return err_check(c);
 }
 
+int nyxpm_assign_density_and_vels(nyxpm_ctx* c, int box, double* out) {
+    if (!c || !out) return NYXPM_ERR_ARG;
+    int rc;
+    if ((rc = require_particles(c))) return rc;
+    if ((rc = check_box(c, box))) return rc;
+    if (!c->sorted_valid) {
+        if (!c->keys_valid && (rc = phase_keys(c))) return rc;
+        if ((rc = phase_sort(c))) return rc;
+    }
+    PartPtrs& A = c->P[c->cur];
+    const long long N = c->ncell_total;
+    const long long H = (long long)c->ntiles_total * c->halo_stride;
+    const int nbox = (int)c->boxes.size();
+    // scratch: mass and three momentum sums (all boxes), the momentum deposit's
+    // own/halo cells, the per-box maxima
+    double *sums = nullptr, *mown = nullptr, *mhalo = nullptr;
+    unsigned long long* maxbits = nullptr;
+    auto release = [&]() {
+        const long long bytes = (sums ? 4 * N * 8 : 0) + (mown ? N * 8 : 0) + (mhalo ? std::max(H, 1LL) * 8 : 0) +
+                                (maxbits ? 3LL * nbox * 8 : 0);
+        dfree(sums);
+        dfree(mown);
+        dfree(mhalo);
+        dfree(maxbits);
+        c->device_bytes -= bytes;
+    };
+    if ((rc = dalloc(c, &sums, 4 * N)) || (rc = dalloc(c, &mown, N)) || (rc = dalloc(c, &mhalo, std::max(H, 1LL))) ||
+        (rc = dalloc(c, &maxbits, 3 * nbox))) {
+        release();
+        return rc;
+    }
+    const size_t lds = 2 * sizeof(unsigned long long) * (c->tile[0] + 1) * (c->tile[1] + 1) * (c->tile[2] + 1);
+    hipLaunchKernelGGL(k_deposit, dim3(c->tseg_blocks), dim3(kBlock), lds, c->stream, c->tseg, c->tseg_n, c->dboxes,
+                       c->tstart, A.x, A.y, A.z, A.m, c->own, c->halo, c->err_flag);
+    hipLaunchKernelGGL(k_fold_raw, dim3(c->t_res.blocks), dim3(kBlock), 0, c->stream, c->segs(c->t_res),
+                       c->t_res.count, c->dboxes, c->own, c->halo, sums);
+    hipLaunchKernelGGL(k_fill_dbl, dim3(1), dim3(kBlock), 0, c->stream, reinterpret_cast<double*>(maxbits),
+                       (long long)(3 * nbox), 0.0);
+    if (c->pseg_blocks > 0)
+        hipLaunchKernelGGL(k_max_mom, dim3(c->pseg_blocks), dim3(kBlock), 0, c->stream, c->pseg, c->pseg_n,
+                           c->dboxes, A.m, A.vx, A.vy, A.vz, maxbits);
+    const double* vel[3] = {A.vx, A.vy, A.vz};
+    for (int d = 0; d < 3; ++d) {
+        hipLaunchKernelGGL(k_deposit_mom, dim3(c->tseg_blocks), dim3(kBlock), lds, c->stream, c->tseg, c->tseg_n,
+                           c->dboxes, c->tstart, A.x, A.y, A.z, A.m, vel[d], maxbits, d, mown, mhalo, c->err_flag);
+        hipLaunchKernelGGL(k_fold_raw, dim3(c->t_res.blocks), dim3(kBlock), 0, c->stream, c->segs(c->t_res),
+                           c->t_res.count, c->dboxes, mown, mhalo, sums + (d + 1) * N);
+    }
+    const Box& B = c->boxes[box];
+    const long long nc = B.lev[0].nc;
+    const bool dev = is_device_ptr(out);
+    double* dst = out;
+    if (!dev && (rc = dalloc(c, &dst, 4 * nc))) {
+        release();
+        return rc;
+    }
+    hipLaunchKernelGGL(k_vels_finish, dim3(nblocks(nc)), dim3(kBlock), 0, c->stream, c->dboxes, box, sums,
+                       sums + N, N, dst);
+    hipError_t e = hipGetLastError();
+    if (!dev && e == hipSuccess) e = hipMemcpyAsync(out, dst, 4 * nc * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (!dev) {
+        dfree(dst);
+        c->device_bytes -= 4 * nc * (long long)sizeof(double);
+    }
+    release();
+    // the deposit buffers now hold raw mass, not rho
+    c->density_valid = false;
+    c->f1_valid = false;
+    if (e != hipSuccess) FAIL(c, NYXPM_ERR_HIP, "assign_density_and_vels: %s", hipGetErrorString(e));
+    return err_check(c);
+}
+
 int nyxpm_sum_particle_mass(nyxpm_ctx* c, int box, double* mass_out) {
     if (!c || !mass_out) return NYXPM_ERR_ARG;
     int rc;

@@ -306,6 +306,97 @@ __global__ void __launch_bounds__(kBlock) k_deposit(const BoxSeg* segs, int nseg
     }
 }
 
+// Momentum deposit for AssignDensityAndVels: one velocity component per launch,
+// contributions (sx*sy*sz*m) * v_d as signed fixed point (two's complement in
+// the u64 words, hi in units 2^(E-30) converted through int32, lo in 2^(E-62),
+// 2^E <= max |m v_d| of the box, from maxbits), so the sums are exact and order
+// independent like the mass deposit's; own cells to `own`, the low halo layer to
+// `halo` (k_deposit's layout).
+__global__ void __launch_bounds__(kBlock) k_deposit_mom(const BoxSeg* segs, int nseg, const BoxDev* boxes,
+                                                        const long long* __restrict__ start,
+                                                        const double* __restrict__ x, const double* __restrict__ y,
+                                                        const double* __restrict__ z, const double* __restrict__ m,
+                                                        const double* __restrict__ v,
+                                                        const unsigned long long* maxbits, int comp,
+                                                        double* __restrict__ own, double* __restrict__ halo,
+                                                        int* err) {
+    extern __shared__ unsigned long long acc[];
+    const BoxSeg sg = segs[find_seg(segs, nseg, blockIdx.x)];
+    const BoxDev& B = boxes[sg.box];
+    const int lt = blockIdx.x - sg.blk0;
+    const int tx = lt % B.ntile[0], ty = (lt / B.ntile[0]) % B.ntile[1], tz = lt / (B.ntile[0] * B.ntile[1]);
+    const int o0 = tx * B.tile[0], o1 = ty * B.tile[1], o2 = tz * B.tile[2];
+    const int bx = min(B.tile[0], B.n[0] - o0), by = min(B.tile[1], B.n[1] - o1), bz = min(B.tile[2], B.n[2] - o2);
+    const int Lx = bx + 1, Ly = by + 1, Lz = bz + 1;
+    const int LC = Lx * Ly * Lz;
+    for (int c = threadIdx.x; c < 2 * LC; c += kBlock) acc[c] = 0ull;
+    __syncthreads();
+    const long long gt = B.tile_off + lt;
+    const long long p0 = start[gt], p1 = start[gt + 1];
+    const double vmax = __longlong_as_double((long long)maxbits[3 * sg.box + comp]);
+    const int E = vmax > 0.0 ? ilogb(vmax) : 0;
+    for (long long p = p0 + threadIdx.x; p < p1; p += kBlock) {
+        double xint, yint, zint;
+        const int i = cic_index(x[p], B.plo[0], B.dxi[0], &xint);
+        const int j = cic_index(y[p], B.plo[1], B.dxi[1], &yint);
+        const int k = cic_index(z[p], B.plo[2], B.dxi[2], &zint);
+        const int iw = wrap_checked(i, B.n[0], err) - o0;
+        const int jw = wrap_checked(j, B.n[1], err) - o1;
+        const int kw = wrap_checked(k, B.n[2], err) - o2;
+        if (iw < 0 || iw >= bx || jw < 0 || jw >= by || kw < 0 || kw >= bz) {
+            atomicOr(err, 2);
+            continue;
+        }
+        const double sx[2] = {1.0 - xint, xint};
+        const double sy[2] = {1.0 - yint, yint};
+        const double sz[2] = {1.0 - zint, zint};
+        const double mp = B.has_mass ? m[p] : B.m_const;
+        const double vp = v[p];
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+            for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+                for (int ii = 0; ii < 2; ++ii) {
+                    const double w = sx[ii] * sy[jj] * sz[kk] * mp;
+                    const double s = ldexp(w * vp, 30 - E);  // |s| < 2^31
+                    const double hf = floor(s);
+                    const double lr = rint(ldexp(s - hf, 32));
+                    const bool carry = lr >= 4294967296.0;
+                    const long long hi = (long long)(int)hf + (carry ? 1 : 0);
+                    const unsigned long long lo = carry ? 0ull : (unsigned long long)(unsigned)lr;
+                    const int idx = ((kw + kk) * Ly + (jw + jj)) * Lx + (iw + ii);
+                    atomicAdd(&acc[idx], (unsigned long long)hi);
+                    atomicAdd(&acc[LC + idx], lo);
+                }
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < LC; c += kBlock) {
+        const int lx = c % Lx, ly = (c / Lx) % Ly, lz = c / (Lx * Ly);
+        const double val = ldexp((double)(long long)acc[c], E - 30) + ldexp((double)acc[LC + c], E - 62);
+        if (lx > 0 && ly > 0 && lz > 0)
+            own[B.cell_off + cidx(o0 + lx - 1, o1 + ly - 1, o2 + lz - 1, B.n[0], B.n[1])] = val;
+        else
+            halo[gt * B.halo_stride + halo_index(lx, ly, lz, bx, by, bz)] = val;
+    }
+}
+
+// max |m v_d| per box and component (bit pattern, atomicMax) for k_deposit_mom.
+__global__ void __launch_bounds__(kBlock) k_max_mom(const BoxSeg* segs, int nseg, const BoxDev* boxes,
+                                                    const double* __restrict__ m, const double* __restrict__ vx,
+                                                    const double* __restrict__ vy, const double* __restrict__ vz,
+                                                    unsigned long long* maxbits) {
+    const BoxSeg sg = segs[find_seg(segs, nseg, blockIdx.x)];
+    const BoxDev& B = boxes[sg.box];
+    const long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x;
+    if (q >= B.np) return;
+    const long long p = B.part_off + q;
+    const double mp = B.has_mass ? m[p] : B.m_const;
+    atomicMax(maxbits + 3 * sg.box + 0, dbits(fabs(mp * vx[p])));
+    atomicMax(maxbits + 3 * sg.box + 1, dbits(fabs(mp * vy[p])));
+    atomicMax(maxbits + 3 * sg.box + 2, dbits(fabs(mp * vz[p])));
+}
+
 // Sum of a block's values in a fixed order (wave butterfly, then waves in order).
 __device__ __forceinline__ double block_sum_fixed(double v) {
     for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -380,7 +471,71 @@ __global__ void __launch_bounds__(kBlock) k_fold(const CellSeg* segs, int nseg, 
     if (threadIdx.x == 0) partial[blockIdx.x] = bs;
 }
 
-// Per box: S = sum of its block partials (fixed order); solvability offset
+// out = own + halos of the up-to-7 upper neighbour tiles (k_fold's order, no
+// scaling): the folded raw sums of a deposit.
+__global__ void __launch_bounds__(kBlock) k_fold_raw(const CellSeg* segs, int nseg, const BoxDev* boxes,
+                                                     const double* __restrict__ own,
+                                                     const double* __restrict__ halo, double* __restrict__ out) {
+    const CellSeg& sg = segs[find_seg(segs, nseg, blockIdx.x)];
+    const BoxDev& B = boxes[sg.box];
+    const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
+    const int T0 = B.tile[0], T1 = B.tile[1], T2 = B.tile[2];
+    const int nt0 = B.ntile[0], nt1 = B.ntile[1], nt2 = B.ntile[2];
+    const int ntiles = nt0 * nt1 * nt2;
+    const int per = (ntiles + sg.nblk - 1) / sg.nblk;
+    const int t_begin = (blockIdx.x - sg.blk0) * per;
+    const int t_end = min(ntiles, t_begin + per);
+    for (int lt = t_begin; lt < t_end; ++lt) {
+        const int tx = lt % nt0, ty = (lt / nt0) % nt1, tz = lt / (nt0 * nt1);
+        const int bx = min(T0, nx - tx * T0), by = min(T1, ny - ty * T1), bz = min(T2, nz - tz * T2);
+        const int ncel = bx * by * bz;
+        for (int c = threadIdx.x; c < ncel; c += kBlock) {
+            const int li = c % bx, lj = (c / bx) % by, lk = c / (bx * by);
+            const bool topx = li == bx - 1, topy = lj == by - 1, topz = lk == bz - 1;
+            const size_t q = B.cell_off + cidx(tx * T0 + li, ty * T1 + lj, tz * T2 + lk, nx, ny);
+            double v = own[q];
+            if (topx || topy || topz) {
+                const int sel[7] = {1, 2, 4, 3, 5, 6, 7};
+#pragma unroll
+                for (int s = 0; s < 7; ++s) {
+                    const int S = sel[s];
+                    const bool sx = S & 1, sy = S & 2, sz = S & 4;
+                    if ((sx && !topx) || (sy && !topy) || (sz && !topz)) continue;
+                    int ntx = tx + (sx ? 1 : 0), nty = ty + (sy ? 1 : 0), ntz = tz + (sz ? 1 : 0);
+                    if (ntx == nt0) ntx = 0;
+                    if (nty == nt1) nty = 0;
+                    if (ntz == nt2) ntz = 0;
+                    const int nbx = min(T0, nx - ntx * T0), nby = min(T1, ny - nty * T1),
+                              nbz = min(T2, nz - ntz * T2);
+                    const int hx = sx ? 0 : li + 1, hy = sy ? 0 : lj + 1, hz = sz ? 0 : lk + 1;
+                    const long long nt = B.tile_off + ((long long)ntz * nt1 + nty) * nt0 + ntx;
+                    v += halo[nt * B.halo_stride + halo_index(hx, hy, hz, nbx, nby, nbz)];
+                }
+            }
+            out[q] = v;
+        }
+    }
+}
+
+// AssignDensityAndVels for box `box`: rho = mass * 1/vol, v_d = mom_d / mass (0
+// without mass), written as 4 fields of the box's cells.
+__global__ void __launch_bounds__(kBlock) k_vels_finish(const BoxDev* boxes, int box,
+                                                        const double* __restrict__ mass,
+                                                        const double* __restrict__ mom,
+                                                        long long mom_stride, double* __restrict__ out) {
+    const BoxDev& B = boxes[box];
+    const long long nc = (long long)B.n[0] * B.n[1] * B.n[2];
+    const long long q = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (q >= nc) return;
+    const double ms = mass[B.cell_off + q];
+    out[q] = ms * B.inv_vol;
+    for (int d = 0; d < 3; ++d) {
+        const double md = mom[d * mom_stride + B.cell_off + q];
+        out[(d + 1) * nc + q] = ms != 0.0 ? md / ms : 0.0;
+    }
+}
+
+
 // sum = (S*dV)/ProbSize, as vol_weight_sum computes it (sum_utils.cpp:85-93).
 __global__ void __launch_bounds__(kBlock) k_rhs_sum(const CellSeg* segs, const double* __restrict__ partial,
                                                     const double* __restrict__ dv_size, double* __restrict__ solv) {

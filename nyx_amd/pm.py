@@ -166,6 +166,12 @@ class PMContext:
         self._ck(self.L.nyxpm_assign_density(self.h, box, ptr(rho)))
         return rho
 
+    def assign_density_and_vels(self, box=0):
+        """(4, nz, ny, nx): CIC density and mass-weighted cell velocity (vx, vy, vz)."""
+        out = np.empty((4,) + self.geoms[box].shape)
+        self._ck(self.L.nyxpm_assign_density_and_vels(self.h, box, ptr(out)))
+        return out
+
     def sum_particle_mass(self, box=0):
         v = ctypes.c_double()
         self._ck(self.L.nyxpm_sum_particle_mass(self.h, box, ctypes.byref(v)))
@@ -328,6 +334,11 @@ class DarkMatterParticleContainer:
         if level != 0 or ncomp != 1:
             raise NotImplementedError("one level, ncomp=1")
         return self.ctx.assign_density(self.box)
+
+    def AssignDensityAndVels(self, lev_min=0):
+        """DarkMatterParticleContainer::AssignDensityAndVels: [density, vx, vy, vz]
+        of the single level, as a list with one (4, nz, ny, nx) array."""
+        return [self.ctx.assign_density_and_vels(self.box)]
 
     def AssignDensity(self, lev_min=0, ncomp=1, finest_level=-1, ngrow=1):
         """NyxParticleContainer.H:94-97; returns [rho] for the single level."""

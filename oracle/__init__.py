@@ -66,6 +66,8 @@ def lib():
     L.orc_mg_solve.restype = _int
     L.orc_grav_vector.argtypes = [_vp, _vp, _vp, _vp, _vp]
     L.orc_grav_vector.restype = None
+    L.orc_deposit_cic_vels.argtypes = [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _vp]
+    L.orc_deposit_cic_vels.restype = _int
     L.orc_est_time_step.argtypes = [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d]
     L.orc_est_time_step.restype = _d
     L.orc_update_particles.argtypes = [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp,
@@ -188,6 +190,20 @@ def grav_vector(n, plo, phi_hi, phi):
     g = np.empty((3,) + grid_shape(n), dtype=np.float64)
     lib().orc_grav_vector(_p(n), _p(plo), _p(phi_hi), _p(phi), _p(g))
     return g
+
+
+def deposit_vels(n, plo, phi, pos, vel, m=None, m_const=1.0):
+    """AssignDensityAndVels: (4, nz, ny, nx) = density, mass-weighted vx, vy, vz."""
+    n, plo, phi = _geom(n, plo, phi)
+    pos = [_c64(p) for p in pos]
+    vel = [_c64(v) for v in vel]
+    m = None if m is None else _c64(m)
+    out = np.empty((4,) + grid_shape(n))
+    rc = lib().orc_deposit_cic_vels(_p(n), _p(plo), _p(phi), len(pos[0]), _p(pos[0]), _p(pos[1]), _p(pos[2]),
+                                    _p(vel[0]), _p(vel[1]), _p(vel[2]), _p(m), float(m_const), _p(out))
+    if rc:
+        raise RuntimeError(f"orc_deposit_cic_vels failed ({rc})")
+    return out
 
 
 def est_time_step(n, plo, phi_hi, pos, vel, g, a=1.0, cfl=0.5):

@@ -599,6 +599,54 @@ void orc_grav_vector(const int n[3], const double plo[3], const double phi_hi[3]
 }
 
 /*
+ * DarkMatterParticleContainer::AssignDensityAndVels (DarkMatterParticleContainer.cpp:
+ * 456-460) = AssignDensity with ncomp = AMREX_SPACEDIM + 1: component 0 is the CIC
+ * mass density (orc_deposit_cic), components 1..3 the CIC mass-weighted cell
+ * velocity sum(w m v_d) / sum(w m) with w m = sx*sy*sz*m (left to right) and
+ * (w m) * v_d the momentum contribution.  The mass and momentum sums for the
+ * ratio are taken serially in particle order; a cell without mass gets velocity
+ * 0.  out holds 4 fields of n cells.  AMReX's own division step (after
+ * SumBoundary) is restated from its documented behaviour, not its source.
+ */
+int orc_deposit_cic_vels(const int n[3], const double plo[3], const double phi[3], int64_t np,
+                         const double *x, const double *y, const double *z, const double *vx,
+                         const double *vy, const double *vz, const double *m, double m_const,
+                         double *out) {
+    const size_t nc = (size_t)n[0] * n[1] * n[2];
+    int rc = orc_deposit_cic(n, plo, phi, np, x, y, z, m, m_const, out);
+    if (rc) return rc;
+    double dx[3], dxi[3];
+    geom_dx(n, plo, phi, dx, dxi);
+    double *mass = (double *)calloc(nc, sizeof(double));
+    if (!mass) return ORC_ERR_NOMEM;
+    double *mom = out + nc;
+    memset(mom, 0, 3 * nc * sizeof(double));
+    const double *v[3] = {vx, vy, vz};
+    for (int64_t p = 0; p < np; ++p) {
+        const double mp = m ? m[p] : m_const;
+        double lx = (x[p] - plo[0]) * dxi[0] + 0.5;
+        double ly = (y[p] - plo[1]) * dxi[1] + 0.5;
+        double lz = (z[p] - plo[2]) * dxi[2] + 0.5;
+        int64_t i = (int64_t)floor(lx), j = (int64_t)floor(ly), k = (int64_t)floor(lz);
+        double xint = lx - i, yint = ly - j, zint = lz - k;
+        double sx[2] = {1.0 - xint, xint}, sy[2] = {1.0 - yint, yint}, sz[2] = {1.0 - zint, zint};
+        for (int kk = 0; kk < 2; ++kk)
+            for (int jj = 0; jj < 2; ++jj)
+                for (int ii = 0; ii < 2; ++ii) {
+                    const size_t c = cidx((int)wrapi(i - 1 + ii, n[0]), (int)wrapi(j - 1 + jj, n[1]),
+                                          (int)wrapi(k - 1 + kk, n[2]), n);
+                    const double w = sx[ii] * sy[jj] * sz[kk] * mp;
+                    mass[c] += w;
+                    for (int d = 0; d < 3; ++d) mom[(size_t)d * nc + c] += w * v[d][p];
+                }
+    }
+    for (size_t c = 0; c < nc; ++c)
+        for (int d = 0; d < 3; ++d) mom[(size_t)d * nc + c] = mass[c] != 0.0 ? mom[(size_t)d * nc + c] / mass[c] : 0.0;
+    free(mass);
+    return ORC_OK;
+}
+
+/*
  * NyxParticleContainer::estTimestep(acceleration, a, lev, cfl)
  * (NyxParticleContainer.H:327-410), called by Nyx::particle_est_time_step with the
  * new cell-centred gravity (NyxParticles.cpp:1028-1036).  Per particle:

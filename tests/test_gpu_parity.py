@@ -402,3 +402,48 @@ def test_est_time_step_vs_oracle(nyx, orc):
             assert dt == orc.est_time_step(n, (0, 0, 0), (1, 1, 1), p, v, g, a, cfl)
             pc = nyx.DarkMatterParticleContainer(c)
             assert pc.estTimestep(None, a, 0, cfl) == dt
+
+
+# ---------------------------------------------------------------------------
+# AssignDensityAndVels
+# ---------------------------------------------------------------------------
+
+def test_density_and_vels_vs_oracle(nyx, orc):
+    # clustered particles with per-particle masses in a non-cubic offset box, part
+    # of it empty: density identical to AssignDensity, velocities to the oracle's
+    r = np.random.default_rng(21)
+    n, plo, phi = (24, 16, 20), (-1.0, 0.5, 2.0), (0.5, 1.5, 3.25)
+    npart = 20000
+    L = [phi[d] - plo[d] for d in range(3)]
+    pos = [plo[d] + L[d] * np.clip(r.normal(0.4, 0.15, npart), 0.0, 0.999) for d in range(3)]
+    vel = [r.normal(0.0, 3.0, npart) for _ in range(3)]
+    m = r.uniform(0.1, 2.0, npart)
+    with ctx1(nyx, n, plo, phi) as c:
+        c.upload(0, pos, vel, m=m)
+        out = c.assign_density_and_vels()
+        out2 = c.assign_density_and_vels()
+        rho = c.assign_density()
+        pc = nyx.DarkMatterParticleContainer(c)
+        out3 = pc.AssignDensityAndVels()[0]
+    assert np.array_equal(out, out2) and np.array_equal(out, out3)
+    assert np.array_equal(out[0], rho)
+    ref = orc.deposit_vels(n, plo, phi, pos, vel, m=m)
+    assert relmax(out[0], ref[0]) <= 1e-13
+    has = ref[0] > 0
+    assert (~has).any()
+    for d in range(3):
+        assert np.abs(out[1 + d] - ref[1 + d]).max() <= 1e-12 * np.abs(vel[d]).max()
+        assert np.all(out[1 + d][~has] == 0)
+
+
+def test_density_and_vels_uniform_velocity(nyx):
+    r = np.random.default_rng(22)
+    n = (32, 32, 32)
+    pos = [r.random(50000) for _ in range(3)]
+    vel = [np.full(50000, v) for v in (0.7, -3.0, 1e-3)]
+    with ctx1(nyx, n) as c:
+        c.upload(0, pos, vel, m_const=1.0 / 50000)
+        out = c.assign_density_and_vels()
+    has = out[0] > 0
+    for d, v in enumerate((0.7, -3.0, 1e-3)):
+        assert np.allclose(out[1 + d][has], v, rtol=1e-14, atol=0)

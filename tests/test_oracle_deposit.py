@@ -108,3 +108,56 @@ def test_empty_and_single_thread_equivalence(orc):
     finally:
         orc.set_num_threads(nt)
     assert np.array_equal(a, b)  # summation order is independent of thread count
+
+
+# ---------------------------------------------------------------------------
+# AssignDensityAndVels (DarkMatterParticleContainer.cpp:456-460)
+# ---------------------------------------------------------------------------
+
+def test_density_and_vels_uniform_velocity(orc):
+    # every particle moves with the same v: every cell with mass has exactly that
+    # velocity (up to the rounding of a weighted mean), empty cells 0; comp 0 is
+    # the plain deposit
+    r = np.random.default_rng(5)
+    n = (8, 6, 10)
+    pos = [r.random(300) * 0.5 for _ in range(3)]  # part of the box stays empty
+    vel = [np.full(300, 0.3), np.full(300, -1.25), np.full(300, 2.0)]
+    m = r.uniform(0.5, 2.0, 300)
+    out = orc.deposit_vels(n, (0, 0, 0), (1, 1, 1), pos, vel, m=m)
+    rho = orc.deposit(n, (0, 0, 0), (1, 1, 1), pos[0], pos[1], pos[2], m=m)
+    assert out.shape == (4, 10, 6, 8)
+    has = out[0] > 0
+    assert has.any() and (~has).any()
+    for d, v in enumerate((0.3, -1.25, 2.0)):
+        assert np.allclose(out[1 + d][has], v, rtol=1e-14, atol=0)
+        assert np.all(out[1 + d][~has] == 0)
+    assert np.array_equal(out[0], rho)
+
+
+def test_density_and_vels_known_answers(orc):
+    n = (4, 4, 4)
+    z = np.zeros(1)
+    # a particle at a cell centre: all its mass and velocity in that cell
+    pos = [np.array([0.375]), np.array([0.625]), np.array([0.125])]
+    out = orc.deposit_vels(n, (0, 0, 0), (1, 1, 1), pos, [np.array([1.5]), np.array([-2.0]), np.array([0.25])],
+                           m_const=2.0)
+    assert out[0, 0, 2, 1] == 2.0 * 64 and np.count_nonzero(out[0]) == 1
+    assert (out[1, 0, 2, 1], out[2, 0, 2, 1], out[3, 0, 2, 1]) == (1.5, -2.0, 0.25)
+    # two particles sharing a cell: the mass-weighted mean of their velocities
+    pos = [np.array([0.375, 0.375]), np.array([0.625, 0.625]), np.array([0.125, 0.125])]
+    out = orc.deposit_vels(n, (0, 0, 0), (1, 1, 1), pos, [np.array([1.0, 4.0]), z.repeat(2), z.repeat(2)],
+                           m=np.array([1.0, 3.0]))
+    assert out[1, 0, 2, 1] == (1.0 * 1.0 + 3.0 * 4.0) / 4.0
+
+
+def test_density_and_vels_momentum_conservation(orc):
+    # sum over cells of (mass-weighted velocity) * (cell mass) = total momentum
+    r = np.random.default_rng(9)
+    n = (12, 12, 12)
+    pos = [r.random(2000) for _ in range(3)]
+    vel = [r.normal(0, 1, 2000) for _ in range(3)]
+    m = r.uniform(0.1, 1.0, 2000)
+    out = orc.deposit_vels(n, (0, 0, 0), (1, 1, 1), pos, vel, m=m)
+    cell_mass = out[0] / 12 ** 3
+    for d in range(3):
+        assert abs(np.sum(out[1 + d] * cell_mass) - np.sum(m * vel[d])) <= 1e-12 * np.sum(m * np.abs(vel[d]))
