@@ -131,24 +131,48 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
     const int tid = threadIdx.x;
 
     // Planes are fetched into registers one iteration ahead of their use, so the
-    // global-memory latency overlaps the previous plane's compute.
-    T pe[NE], pr[NR];
+    // global-memory latency overlaps the previous plane's compute.  A thread's
+    // in-plane offsets (periodic wrap included) are the same for every plane and
+    // are computed once; the coarse values of the prolongation change every
+    // second plane (z0 is even), so they are reloaded only when kz is even.
+    T pe[NE], pr[NR], pc[PRO ? NE : 1];
+    size_t eoff[NE], coff_[PRO ? NE : 1], roff[NR];
+    const size_t plane = (size_t)nx * ny, cplane = (size_t)cnx * cny;
+#pragma unroll
+    for (int u = 0; u < NE; ++u) {
+        const int idx = min(tid + u * NT, EX * EY - 1);
+        const int ly = idx / EX, lx = idx - ly * EX;
+        int gx = x0 + lx - 2, gy = y0 + ly - 2;
+        gx = gx < 0 ? gx + nx : (gx >= nx ? gx - nx : gx);
+        gy = gy < 0 ? gy + ny : (gy >= ny ? gy - ny : gy);
+        eoff[u] = (size_t)gy * nx + gx;
+        if (PRO) coff_[u] = (size_t)(gy >> 1) * cnx + (gx >> 1);
+    }
+#pragma unroll
+    for (int u = 0; u < NR; ++u) {
+        const int idx = min(tid + u * NT, RX * RY - 1);
+        const int ly = idx / RX, lx = idx - ly * RX;
+        int gx = x0 + lx - 1, gy = y0 + ly - 1;
+        gx = gx < 0 ? gx + nx : (gx >= nx ? gx - nx : gx);
+        gy = gy < 0 ? gy + ny : (gy >= ny ? gy - ny : gy);
+        roff[u] = (size_t)gy * nx + gx;
+    }
     auto wrapz = [&](int k) { return k < 0 ? k + nz : (k >= nz ? k - nz : k); };
     auto fetch_e = [&](int kz) {
         const int kw = wrapz(kz);
+        const bool new_coarse = !(kz & 1) || kz == z0 - 2;
 #pragma unroll
         for (int u = 0; u < NE; ++u) {
             const int idx = tid + u * NT;
             if (idx < EX * EY) {
-                const int ly = idx / EX, lx = idx - ly * EX;
-                int gx = x0 + lx - 2, gy = y0 + ly - 2;
-                gx = gx < 0 ? gx + nx : (gx >= nx ? gx - nx : gx);
-                gy = gy < 0 ? gy + ny : (gy >= ny ? gy - ny : gy);
                 if (ZERO_IN) {
                     pe[u] = T(0);
                 } else {
-                    T v = Ein[cidx(gx, gy, kw, nx, ny)];
-                    if (PRO) v = v + Ec[cidx(gx >> 1, gy >> 1, kw >> 1, cnx, cny)];
+                    T v = Ein[(size_t)kw * plane + eoff[u]];
+                    if (PRO) {
+                        if (new_coarse) pc[u] = Ec[(size_t)(kw >> 1) * cplane + coff_[u]];
+                        v = v + pc[u];
+                    }
                     pe[u] = v;
                 }
             }
@@ -167,13 +191,7 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
 #pragma unroll
         for (int u = 0; u < NR; ++u) {
             const int idx = tid + u * NT;
-            if (idx < RX * RY) {
-                const int ly = idx / RX, lx = idx - ly * RX;
-                int gx = x0 + lx - 1, gy = y0 + ly - 1;
-                gx = gx < 0 ? gx + nx : (gx >= nx ? gx - nx : gx);
-                gy = gy < 0 ? gy + ny : (gy >= ny ? gy - ny : gy);
-                pr[u] = R[cidx(gx, gy, kw, nx, ny)];
-            }
+            if (idx < RX * RY) pr[u] = R[(size_t)kw * plane + roff[u]];
         }
     };
     auto put_r = [&](int kz) {
