@@ -20,7 +20,7 @@ constexpr int kWave = 64;          // CDNA wavefront width
 constexpr int kBlock = 256;        // threads per block of the grid kernels
 constexpr int kMaxLevels = 24;
 constexpr int kBottomMaxCells = 4096;
-constexpr int kReduceBlocks = 2048;  // blocks per box of the reducing level-0 kernels
+constexpr int kReduceBlocks = 8192;  // blocks per box of the reducing level-0 kernels
 constexpr int kNormSlots = 64;       // atomicMax slots per box for residual norms
 
 // One periodic box as the particle kernels see it.
