@@ -82,12 +82,18 @@ def run_mode(ctx, precision, steps, warmup, dt, tol, use_graph):
     ctx.sync()
     ctx.timing_reset()
     ctx.timing_enable(True)
+    s0 = ctx.sort_stats()
     t0 = time.perf_counter()
     it = ctx.pm_steps(steps, dt, rel_tol=tol, use_graph=use_graph)
     ctx.sync()
     t1 = time.perf_counter()
     ctx.timing_enable(False)
-    return t1 - t0, int(it[0])
+    s1 = ctx.sort_stats()
+    # particle re-sorts inside the timed region: full counting sorts, incremental
+    # ones, and the particles moved per step by the latter
+    resort = {"full": s1["full"] - s0["full"], "incremental": s1["incremental"] - s0["incremental"],
+              "moved_per_step": round((s1["moved"] - s0["moved"]) / max(1, steps), 1)}
+    return t1 - t0, int(it[0]), resort
 
 
 def stage_table(ctx, names, ncell, npart, halo_cells, sz, elapsed_ms):
@@ -194,7 +200,7 @@ def main():
     modes = a.modes.split(",")
 
     names = ["residual", "norm", "update", "deposit", "fold", "rhs", "kick_drift", "kick_pair", "kick", "scatter",
-             "scan", "coarse",
+             "scan", "coarse", "migrate",
              "items", "keys", "bottom"] + \
             [f"{k}.d{d}" for k in ("sweep0", "sweep", "sweep_p", "sweep_u", "smooth", "resid_restrict", "prolong",
                                     "fill")
@@ -204,22 +210,23 @@ def main():
 
     t_mix = t_64 = None
     it_mix = it_64 = None
+    rs_mix = rs_64 = None
     st_mix, st_64 = {}, {}
     if "mixed" in modes:
         log(f"mixed: {a.warmup} warmup + {a.steps} timed steps")
-        t_mix, it_mix = run_mode(ctx, "mixed", a.steps, a.warmup, dt, a.tol, a.graph)
+        t_mix, it_mix, rs_mix = run_mode(ctx, "mixed", a.steps, a.warmup, dt, a.tol, a.graph)
         st_mix = stage_table(ctx, names, ncell, npart, halo_cells, 4, t_mix * 1e3)
         log(f"  {t_mix / a.steps * 1e3:.2f} ms/step, {it_mix} V-cycles")
     if "fp64" in modes:
         log(f"fp64: {a.warmup} warmup + {a.steps} timed steps")
-        t_64, it_64 = run_mode(ctx, "fp64", a.steps, a.warmup, dt, a.tol, a.graph)
+        t_64, it_64, rs_64 = run_mode(ctx, "fp64", a.steps, a.warmup, dt, a.tol, a.graph)
         st_64 = stage_table(ctx, names, ncell, npart, halo_cells, 8, t_64 * 1e3)
         log(f"  {t_64 / a.steps * 1e3:.2f} ms/step, {it_64} V-cycles")
     ctx.close()
     if t_mix is None:  # FP64 only: report it as the value
-        t_mix, it_mix, st_mix = t_64, it_64, st_64
+        t_mix, it_mix, st_mix, rs_mix = t_64, it_64, st_64, rs_64
     if t_64 is None:
-        t_64, it_64, st_64 = t_mix, it_mix, st_mix
+        t_64, it_64, st_64, rs_64 = t_mix, it_mix, st_mix, rs_mix
 
     v_mix = npart * a.steps / t_mix
     v_64 = npart * a.steps / t_64
@@ -263,7 +270,8 @@ def main():
                                f"(FP32 V-cycles in FP64 refinement, FP64 particles), dt 1e-3, rel tol {a.tol:g}",
                    "cells": ncell, "particles": npart, "graph": bool(a.graph), "autotuned": tuned},
         "fp64": {"value": round(v_64, 1), "ms_per_step": round(t_64 / a.steps * 1e3, 3), "vcycles": it_64,
-                 "dtype": "f64"},
+                 "dtype": "f64", "resort": rs_64},
+        "resort": rs_mix,
         "mixed_over_fp64": round(v_mix / v_64, 3),
         "vcycles_mixed": it_mix,
         "accel_rel_l2": rel_l2,

@@ -100,6 +100,13 @@ typedef struct nyxpm_opts {
                           kicked particles with this cfl and a = a_new, which
                           nyxpm_est_time_step then returns without another pass;
                           0 = off */
+    double resort_slack; /* incremental re-sort: each tile of the full sort gets
+                          cnt * resort_slack + 16 free slots, a kick-drift lists
+                          only the particles that change tile and they move into
+                          that slack; a full sort runs when a tile's slack is
+                          used up or the emptied slots pass np/16.  0 = default
+                          (0.25), < 0 = off (full sort after every kick-drift),
+                          at most 4.  Same results either way. */
 } nyxpm_opts;
 
 /* ---- lifecycle ---------------------------------------------------------- */
@@ -237,15 +244,23 @@ int nyxpm_autotune(nyxpm_ctx *ctx, int precision_mask, char *report, int report_
 /* ---- measurement -------------------------------------------------------- */
 /* Kernel-class timing with HIP events on the context stream.  When enabled, each
  * launch of a timed class is bracketed by events; nyxpm_timing_get returns the
- * number of launches and the mean duration in milliseconds for `name`
- * ("smooth", "resid_restrict", "prolong", "residual", "deposit", "kick_drift",
- * "kick", "sort", "fold_rhs", "bottom"). */
+ * number of launches and the mean duration in milliseconds for `name`:
+ * particle stages "keys", "scan", "scatter" (full sort), "migrate" (incremental
+ * re-sort), "items", "deposit", "fold", "rhs", "kick_drift", "kick",
+ * "kick_pair"; multigrid stages "residual", "update", "coarse", "bottom" and,
+ * per depth d, "sweep0.dD", "sweep.dD", "sweep_p.dD", "sweep_u.dD",
+ * "smooth.dD", "resid_restrict.dD", "prolong.dD", "fill.dD". */
 int nyxpm_timing_enable(nyxpm_ctx *ctx, int on);
 int nyxpm_timing_get(nyxpm_ctx *ctx, const char *name, int64_t *count, double *mean_ms,
                      double *total_ms);
 int nyxpm_timing_reset(nyxpm_ctx *ctx);
 /* Bytes of device memory the context holds. */
 int64_t nyxpm_device_bytes(const nyxpm_ctx *ctx);
+/* Particle re-sort counters since the context was created: out[0] full counting
+ * sorts, out[1] incremental re-sorts that kept the slot layout, out[2] particles
+ * those moved to another tile, out[3] emptied slots in the current layout.
+ * No counterpart in the reference (Redistribute re-bins every particle). */
+int nyxpm_sort_stats(const nyxpm_ctx *ctx, int64_t *out4);
 
 #ifdef __cplusplus
 }

@@ -43,7 +43,8 @@ class Opts(ctypes.Structure):
                 ("max_iter", ctypes.c_int), ("tile", ctypes.c_int * 3), ("verbose", ctypes.c_int),
                 ("smoother", ctypes.c_int), ("kick", ctypes.c_int), ("sweep_tx", ctypes.c_int),
                 ("sweep_ty", ctypes.c_int), ("sweep_zchunk", ctypes.c_int), ("kick_chunk", ctypes.c_int),
-                ("separate_kicks", ctypes.c_int), ("coarse", ctypes.c_int), ("particle_cfl", ctypes.c_double)]
+                ("separate_kicks", ctypes.c_int), ("coarse", ctypes.c_int), ("particle_cfl", ctypes.c_double),
+                ("resort_slack", ctypes.c_double)]
 
 
 # (name, restype, argtypes) for every function in include/nyxpm.h
@@ -90,6 +91,7 @@ SIGNATURES = [
     ("nyxpm_timing_get", _i, [_vp, _s, _vp, _vp, _vp]),
     ("nyxpm_timing_reset", _i, [_vp]),
     ("nyxpm_device_bytes", _i64, [_vp]),
+    ("nyxpm_sort_stats", _i, [_vp, _vp]),
 ]
 
 _LIB = None

@@ -44,6 +44,7 @@ struct Box {
     int tile[3], ntile[3];
     long long tile_off, ntiles;
     long long part_off = 0, np = 0;
+    long long span = 0;  // particle slots (np plus the tile slack)
     bool has_mass = false;
     double m_const = 0.0, mass_total = 0.0, mass_offset = 0.0;
     int fx_exp = 0;
@@ -151,10 +152,27 @@ struct nyxpm_ctx {
     bool phi_valid = false;
     bool f1_valid = false;  // f holds rho - mass_offset and partial holds its block sums
 
-    // tiles
+    // tiles: counts of the last binning, and the slot layout of the current
+    // buffer (tile t owns slots tbase[t] .. tbase[t] + tcap[t] - 1, of which the
+    // first cursor[t] have been handed out)
     int* counts = nullptr;
     int* cursor = nullptr;
     long long* tstart = nullptr;
+    long long* tbase = nullptr;
+    int* tcap = nullptr;
+    // incremental re-sort: tile capacity cnt + cnt * slack_num / 1024 + slack_pad
+    // (slack_num == 0: dense tiles, a full sort after every kick-drift); after a
+    // kick-drift the particles that changed tile wait in a list (in the idle
+    // buffer's y array) for k_migrate
+    int slack_num = 256, slack_pad = 16;
+    int hole_num = 64;            // full sort once the emptied slots exceed np * hole_num / 1024
+    bool moves_pending = false;
+    bool inc_suspended = false;   // no incremental re-sort (graph capture)
+    long long holes = 0;          // slots emptied since the last full sort
+    long long n_full_sorts = 0, n_inc_sorts = 0, n_moved = 0;
+    int* nmove = nullptr;
+    int* mig = nullptr;           // k_migrate flags: overflow, moved
+    int* h_mig = nullptr;         // pinned mirror
     long long* bsum = nullptr;
     int scan_blocks = 0;
     TileItem* items = nullptr;  // tile chunks of the last sort (tile-staged kernels)
@@ -652,6 +670,7 @@ int upload_boxdev(nyxpm_ctx* c) {
         D.tile_off = B.tile_off;
         D.part_off = B.part_off;
         D.np = B.np;
+        D.span = B.span;
         D.inv_vol = B.inv_vol;
         D.mass_offset = B.mass_offset;
         D.m_const = B.m_const;
@@ -670,7 +689,7 @@ int build_pseg(nyxpm_ctx* c) {
     std::vector<BoxSeg> ps;
     int blk = 0;
     for (size_t b = 0; b < c->boxes.size(); ++b) {
-        BoxSeg s{(int)b, blk, std::max(1, nblocks(c->boxes[b].np)), 0};
+        BoxSeg s{(int)b, blk, std::max(1, nblocks(c->boxes[b].span)), 0};
         blk += s.nblk;
         ps.push_back(s);
     }
@@ -696,6 +715,11 @@ int alloc_fields(nyxpm_ctx* c) {
     if ((rc = dalloc(c, &c->counts, c->ntiles_total))) return rc;
     if ((rc = dalloc(c, &c->cursor, c->ntiles_total))) return rc;
     if ((rc = dalloc(c, &c->tstart, c->ntiles_total + 1))) return rc;
+    if ((rc = dalloc(c, &c->tbase, c->ntiles_total))) return rc;
+    if ((rc = dalloc(c, &c->tcap, c->ntiles_total))) return rc;
+    if ((rc = dalloc(c, &c->nmove, 1))) return rc;
+    if ((rc = dalloc(c, &c->mig, 2))) return rc;
+    HIPCK(c, hipHostMalloc((void**)&c->h_mig, 2 * sizeof(int), hipHostMallocDefault));
     c->scan_blocks = nblocks(c->ntiles_total, kBlock * kScanItems);
     if ((rc = dalloc(c, &c->bsum, c->scan_blocks + 1))) return rc;
     if ((rc = dalloc(c, &c->r64, c->mg_total))) return rc;
@@ -797,49 +821,105 @@ int phase_keys(nyxpm_ctx* c) {
     return NYXPM_OK;
 }
 
+int make_items(nyxpm_ctx* c) {
+    Timed t(c, "items");
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(kBlock), 0, c->stream, c->nitems, 1LL, 0);
+    hipLaunchKernelGGL(k_make_items, dim3(nblocks(c->ntiles_total)), dim3(kBlock), 0, c->stream, c->tbase,
+                       c->cursor, c->ntiles_total, c->items, c->nitems, c->kick_chunk);
+    LAUNCHCK(c);
+    c->items_valid = true;
+    return NYXPM_OK;
+}
+
+// Incremental re-sort after a kick-drift: the listed particles that changed tile
+// move into their new tile's slack.  Returns in *done whether the layout is
+// whole again (false: a tile ran out of slack, or the emptied slots passed
+// their limit; the caller then sorts in full).  Reads two flags back.
+int phase_migrate(nyxpm_ctx* c, bool* done) {
+    PartPtrs& A = c->P[c->cur];
+    const long long* moves = reinterpret_cast<const long long*>(c->P[1 - c->cur].y);
+    {
+        Timed t(c, "migrate");
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(kBlock), 0, c->stream, c->mig, 2LL, 0);
+        hipLaunchKernelGGL(k_migrate, dim3(1024), dim3(kBlock), 0, c->stream, moves, c->nmove, c->dboxes, c->tbase,
+                           c->tcap, c->cursor, A, c->mig, c->err_flag);
+    }
+    LAUNCHCK(c);
+    HIPCK(c, hipMemcpyAsync(c->h_mig, c->mig, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    c->holes += c->h_mig[1];
+    c->n_moved += c->h_mig[1];
+    *done = c->h_mig[0] == 0 && c->holes * 1024 <= c->npart_total * (long long)c->hole_num;
+    if (*done) ++c->n_inc_sorts;
+    if (*done) return make_items(c);
+    return NYXPM_OK;
+}
+
+// Brings the particles into tile order for the deposit and the tile-staged
+// kicks: incrementally after a kick-drift that listed its tile changes, else by
+// a full counting sort into the other buffer (tiles with slack when slack_num > 0).
 int phase_sort(nyxpm_ctx* c) {
+    int rc;
+    if (c->moves_pending) {
+        c->moves_pending = false;
+        bool done = false;
+        if ((rc = phase_migrate(c, &done))) return rc;
+        if (done) {
+            c->sorted_valid = true;
+            c->keys_valid = false;
+            return NYXPM_OK;
+        }
+        c->keys_valid = false;
+    }
+    if (!c->keys_valid && (rc = phase_keys(c))) return rc;
     PartPtrs& A = c->P[c->cur];
     PartPtrs& B = c->P[1 - c->cur];
+    const bool slack = c->slack_num > 0;
     {
         Timed t(c, "scan");
-        hipLaunchKernelGGL(k_scan_reduce, dim3(c->scan_blocks), dim3(kBlock), 0, c->stream, c->counts,
+        const int* sizes = c->counts;
+        if (slack) {
+            hipLaunchKernelGGL(k_tile_caps, dim3(nblocks(c->ntiles_total)), dim3(kBlock), 0, c->stream, c->counts,
+                               c->ntiles_total, c->slack_num, c->slack_pad, c->tcap);
+            sizes = c->tcap;
+        }
+        hipLaunchKernelGGL(k_scan_reduce, dim3(c->scan_blocks), dim3(kBlock), 0, c->stream, sizes,
                            c->ntiles_total, c->bsum);
         hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, c->stream, c->bsum, c->scan_blocks);
-        hipLaunchKernelGGL(k_scan_down, dim3(c->scan_blocks), dim3(kBlock), 0, c->stream, c->counts,
+        hipLaunchKernelGGL(k_scan_down, dim3(c->scan_blocks), dim3(kBlock), 0, c->stream, sizes,
                            c->ntiles_total, c->bsum, c->tstart, c->cursor);
+        hipLaunchKernelGGL(k_tile_base, dim3(nblocks(c->ntiles_total)), dim3(kBlock), 0, c->stream, c->tseg,
+                           c->tseg_n, c->dboxes, c->tstart, c->ntiles_total, c->tbase);
     }
     {
         Timed t(c, "scatter");
+        // slots the scatter leaves free start empty
+        if (slack)
+            hipLaunchKernelGGL(k_fill_dbl, dim3(nblocks(c->part_cap)), dim3(kBlock), 0, c->stream, B.x,
+                               c->part_cap, std::nan(""));
         hipLaunchKernelGGL(k_scatter, dim3(c->pseg_blocks), dim3(kBlock), 0, c->stream, c->pseg, c->pseg_n,
-                           c->dboxes, c->key, c->tstart, c->cursor, A, B);
-    }
-    {
-        Timed t(c, "items");
-        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(kBlock), 0, c->stream, c->nitems, 1LL, 0);
-        hipLaunchKernelGGL(k_make_items, dim3(nblocks(c->ntiles_total)), dim3(kBlock), 0, c->stream, c->tstart,
-                           c->ntiles_total, c->items, c->nitems, c->kick_chunk);
+                           c->dboxes, c->key, c->tbase, c->cursor, A, B);
     }
     LAUNCHCK(c);
     c->cur = 1 - c->cur;
+    c->holes = 0;
+    ++c->n_full_sorts;
+    if ((rc = make_items(c))) return rc;
     c->sorted_valid = true;
     c->keys_valid = false;
-    c->items_valid = true;
     return NYXPM_OK;
 }
 
 // want_rho: also materialise the density (AssignDensity); the step needs only f.
 int phase_deposit(nyxpm_ctx* c, bool want_rho) {
-    if (!c->sorted_valid) {
-        int rc;
-        if (!c->keys_valid && (rc = phase_keys(c))) return rc;
-        if ((rc = phase_sort(c))) return rc;
-    }
+    int rc;
+    if (!c->sorted_valid && (rc = phase_sort(c))) return rc;
     PartPtrs& A = c->P[c->cur];
     const size_t lds = 2 * sizeof(unsigned long long) * (c->tile[0] + 1) * (c->tile[1] + 1) * (c->tile[2] + 1);
     {
         Timed t(c, "deposit");
         hipLaunchKernelGGL(k_deposit, dim3(c->tseg_blocks), dim3(kBlock), lds, c->stream, c->tseg, c->tseg_n,
-                           c->dboxes, c->tstart, A.x, A.y, A.z, A.m, c->own, c->halo, c->err_flag);
+                           c->dboxes, c->tbase, c->cursor, A.x, A.y, A.z, A.m, c->own, c->halo, c->err_flag);
     }
     {
         Timed t(c, "fold");
@@ -1120,6 +1200,9 @@ int solve_results(nyxpm_ctx* c, int* iters, double* final_res, bool allow_active
 // kick (a_prev, a_cur) of one step and the kick-drift (a_prev2, a_cur2) of the next.
 template <int MODE>
 int phase_kick(nyxpm_ctx* c, double dt, double a_prev, double a_cur, double a_prev2 = 1.0, double a_cur2 = 1.0) {
+    int rc;
+    // a kick-drift's list of tile changes is used up before the particles move again
+    if (c->moves_pending && (rc = phase_sort(c))) return rc;
     PartPtrs& A = c->P[c->cur];
     constexpr bool MOVE = MODE != kKick;
     // staged phi tile with a halo of 2, plus the three components of g on the
@@ -1128,7 +1211,13 @@ int phase_kick(nyxpm_ctx* c, double dt, double a_prev, double a_cur, double a_pr
                                          3 * (c->tile[0] + 1) * (c->tile[1] + 1) * (c->tile[2] + 1));
     if (c->items_valid && !c->untiled_kick && lds <= 64 * 1024) {
         Timed t(c, MODE == kKickPair ? "kick_pair" : MOVE ? "kick_drift" : "kick");
-        if (MOVE)
+        // incremental re-sort: the kick-drift lists the particles that change tile
+        // (the list lives in the idle buffer's y array, like the deferred list in x)
+        const int inc = MOVE && c->slack_num > 0 && !c->inc_suspended ? 1 : 0;
+        long long* moves = reinterpret_cast<long long*>(c->P[1 - c->cur].y);
+        if (inc)
+            hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(kBlock), 0, c->stream, c->nmove, 1LL, 0);
+        else if (MOVE)
             hipLaunchKernelGGL(k_fill_int, dim3(nblocks(c->ntiles_total)), dim3(kBlock), 0, c->stream, c->counts,
                                c->ntiles_total, 0);
         // the deferred-particle list lives in the sort target's x array (as many
@@ -1143,14 +1232,16 @@ int phase_kick(nyxpm_ctx* c, double dt, double a_prev, double a_cur, double a_pr
         auto kern = cfl > 0 ? k_kick_tiled<MODE, MODE != kKickDrift> : k_kick_tiled<MODE, false>;
         hipLaunchKernelGGL(kern, dim3((unsigned)c->items_cap), dim3(kBlock), lds, c->stream, c->tseg, c->tseg_n,
                            c->dboxes, c->items, c->nitems, c->phi, A.x, A.y, A.z, A.vx, A.vy, A.vz, dt, a_prev,
-                           a_cur, a_prev2, a_cur2, c->key, c->counts, defer, c->ndefer, cfl, c->dtmin, c->err_flag);
+                           a_cur, a_prev2, a_cur2, c->key, c->counts, defer, c->ndefer, cfl, c->dtmin, inc, moves,
+                           c->nmove, c->err_flag);
         hipLaunchKernelGGL(k_kick_deferred<MODE>, dim3(256), dim3(kBlock), 0, c->stream, defer, c->ndefer,
                            c->dboxes, c->phi, A.x, A.y, A.z, A.vx, A.vy, A.vz, dt, a_prev, a_cur, a_prev2, a_cur2,
-                           c->key, c->counts, cfl, c->dtmin, c->err_flag);
+                           c->key, c->counts, cfl, c->dtmin, inc, moves, c->nmove, c->err_flag);
         LAUNCHCK(c);
         c->dt_valid = cfl > 0;
         c->dt_a = a_cur;
-        c->keys_valid = MOVE;
+        c->keys_valid = MOVE && !inc;
+        c->moves_pending = inc != 0;
         c->sorted_valid = false;
         c->density_valid = false;
         return NYXPM_OK;
@@ -1309,8 +1400,16 @@ int nyxpm_create_batch(nyxpm_ctx** out, int nbox, const nyxpm_geom* geoms, const
         }
         if (opts->sweep_zchunk > 0) c->sweep_zchunk = opts->sweep_zchunk;
         if (opts->kick_chunk > 0) c->kick_chunk = std::max(kBlock, opts->kick_chunk);
+        if (opts->resort_slack < 0) {
+            c->slack_num = 0;
+            c->slack_pad = 0;
+        } else if (opts->resort_slack > 0) {
+            c->slack_num = std::max(1, (int)std::lround(std::min(opts->resort_slack, 4.0) * 1024.0));
+        }
     }
     *out = c;
+    if (opts && opts->resort_slack > 4.0)
+        FAIL(c, NYXPM_ERR_ARG, "resort_slack %g above 4 (slots per particle)", opts->resort_slack);
     for (int d = 0; d < 3; ++d)
         if (c->tile[d] < 1 || c->tile[d] > 32) FAIL(c, NYXPM_ERR_ARG, "tile[%d]=%d outside [1,32]", d, c->tile[d]);
     if ((c->tile[0] + 1) * (c->tile[1] + 1) * (c->tile[2] + 1) * 16 > 64 * 1024)
@@ -1338,6 +1437,8 @@ int nyxpm_destroy(nyxpm_ctx* c) {
     if (c->graph) hipGraphDestroy(c->graph);
     free_particles(c);
     dfree(c->counts); dfree(c->cursor); dfree(c->tstart); dfree(c->bsum);
+    dfree(c->tbase); dfree(c->tcap); dfree(c->nmove); dfree(c->mig);
+    if (c->h_mig) hipHostFree(c->h_mig);
     c->rho = nullptr;
     dfree(c->own); dfree(c->halo); dfree(c->f); dfree(c->phi); dfree(c->partial);
 
@@ -1393,6 +1494,22 @@ int64_t nyxpm_num_particles(const nyxpm_ctx* c, int box) {
 
 int64_t nyxpm_device_bytes(const nyxpm_ctx* c) { return c ? c->device_bytes : 0; }
 
+int nyxpm_sort_stats(const nyxpm_ctx* c, int64_t* out4) {
+    if (!c || !out4) return NYXPM_ERR_ARG;
+    out4[0] = c->n_full_sorts;
+    out4[1] = c->n_inc_sorts;
+    out4[2] = c->n_moved;
+    out4[3] = c->holes;
+    return NYXPM_OK;
+}
+
+// Particle slots of a box of np particles and ntiles tiles: np plus the largest
+// total slack k_tile_caps can hand out.
+static long long slot_count(const nyxpm_ctx* c, long long np, long long ntiles) {
+    if (c->slack_num == 0) return np;
+    return np + ((np * c->slack_num + 1023) >> 10) + (long long)c->slack_pad * ntiles;
+}
+
 static int upload_impl(nyxpm_ctx* c, int box, int64_t np, const double* x, const double* y, const double* z,
                        const double* vx, const double* vy, const double* vz, const double* m, double m_const,
                        hipMemcpyKind kind) {
@@ -1404,27 +1521,31 @@ static int upload_impl(nyxpm_ctx* c, int box, int64_t np, const double* x, const
     if (!m && !(m_const >= 0.0)) FAIL(c, NYXPM_ERR_ARG, "m_const must be >= 0");
     HIPCK(c, hipStreamSynchronize(c->stream));
     const int nbox = (int)c->boxes.size();
-    // new layout
-    std::vector<long long> new_off(nbox), old_off(nbox), old_np(nbox);
-    long long tot = 0;
+    // new layout: each box keeps np plus the tile slack as slots
+    // (boxes without particles get their empty slack slots)
+    std::vector<long long> new_off(nbox), old_off(nbox), old_span(nbox), new_span(nbox);
+    long long tot = 0, slots = 0;
     for (int b = 0; b < nbox; ++b) {
-        old_off[b] = c->boxes[b].part_off;
-        old_np[b] = c->boxes[b].np;
-        new_off[b] = tot;
-        tot += (b == box) ? np : c->boxes[b].np;
+        const Box& X = c->boxes[b];
+        old_off[b] = X.part_off;
+        old_span[b] = X.np > 0 ? X.span : 0;
+        new_off[b] = slots;
+        new_span[b] = b == box ? slot_count(c, np, X.ntiles) : X.np > 0 ? X.span : slot_count(c, 0, X.ntiles);
+        tot += (b == box) ? np : X.np;
+        slots += new_span[b];
     }
     const bool need_mass = c->any_mass || (m != nullptr);
     PartPtrs N[2] = {};
     int* nkey = nullptr;
-    long long cap = std::max(tot, 1LL);
+    long long cap = std::max(slots, 1LL);
     for (int s = 0; s < 2; ++s)
         if ((rc = alloc_particle_set(c, N[s], cap, need_mass))) return rc;
     if ((rc = dalloc(c, &nkey, cap))) return rc;
-    // keep the other boxes' particles
+    // keep the other boxes' slots as they are (empty slots included)
     PartPtrs& O = c->P[c->cur];
     for (int b = 0; b < nbox; ++b) {
-        if (b == box || old_np[b] == 0) continue;
-        const size_t nb8 = old_np[b] * sizeof(double);
+        if (b == box || old_span[b] == 0) continue;
+        const size_t nb8 = old_span[b] * sizeof(double);
         const long long so = old_off[b], d = new_off[b];
         HIPCK(c, hipMemcpyAsync(N[0].x + d, O.x + so, nb8, hipMemcpyDeviceToDevice, c->stream));
         HIPCK(c, hipMemcpyAsync(N[0].y + d, O.y + so, nb8, hipMemcpyDeviceToDevice, c->stream));
@@ -1432,12 +1553,12 @@ static int upload_impl(nyxpm_ctx* c, int box, int64_t np, const double* x, const
         HIPCK(c, hipMemcpyAsync(N[0].vx + d, O.vx + so, nb8, hipMemcpyDeviceToDevice, c->stream));
         HIPCK(c, hipMemcpyAsync(N[0].vy + d, O.vy + so, nb8, hipMemcpyDeviceToDevice, c->stream));
         HIPCK(c, hipMemcpyAsync(N[0].vz + d, O.vz + so, nb8, hipMemcpyDeviceToDevice, c->stream));
-        HIPCK(c, hipMemcpyAsync(N[0].id + d, O.id + so, old_np[b] * sizeof(int), hipMemcpyDeviceToDevice, c->stream));
+        HIPCK(c, hipMemcpyAsync(N[0].id + d, O.id + so, old_span[b] * sizeof(int), hipMemcpyDeviceToDevice, c->stream));
         if (need_mass) {
             if (O.m) {
                 HIPCK(c, hipMemcpyAsync(N[0].m + d, O.m + so, nb8, hipMemcpyDeviceToDevice, c->stream));
             } else {
-                std::vector<double> mm(old_np[b], c->boxes[b].m_const);
+                std::vector<double> mm(old_span[b], c->boxes[b].m_const);
                 HIPCK(c, hipMemcpyAsync(N[0].m + d, mm.data(), nb8, hipMemcpyHostToDevice, c->stream));
                 HIPCK(c, hipStreamSynchronize(c->stream));
             }
@@ -1464,8 +1585,17 @@ static int upload_impl(nyxpm_ctx* c, int box, int64_t np, const double* x, const
         hipLaunchKernelGGL(k_iota, dim3(nblocks(np)), dim3(kBlock), 0, c->stream, d, (long long)np, N[0].id);
         LAUNCHCK(c);
     }
+    for (int b = 0; b < nbox; ++b) {
+        // slots past the particles start empty
+        const long long used = b == box ? np : old_span[b];
+        if (new_span[b] > used) {
+            hipLaunchKernelGGL(k_fill_dbl, dim3(nblocks(new_span[b] - used)), dim3(kBlock), 0, c->stream,
+                               N[0].x + new_off[b] + used, new_span[b] - used, std::nan(""));
+            LAUNCHCK(c);
+        }
+    }
     HIPCK(c, hipStreamSynchronize(c->stream));
-    const long long icap = c->ntiles_total + tot / c->kick_chunk + 1;
+    const long long icap = c->ntiles_total + slots / c->kick_chunk + 1;
     if (icap > 0x7fffffffLL) FAIL(c, NYXPM_ERR_ARG, "too many tile chunks");
     TileItem* nitm = nullptr;
     if ((rc = dalloc(c, &nitm, icap))) return rc;
@@ -1482,9 +1612,11 @@ static int upload_impl(nyxpm_ctx* c, int box, int64_t np, const double* x, const
     c->any_mass = need_mass;
     for (int b = 0; b < nbox; ++b) {
         c->boxes[b].part_off = new_off[b];
+        c->boxes[b].span = new_span[b];
         if (b == box) c->boxes[b].np = np;
     }
     c->npart_total = tot;
+    c->moves_pending = false;
     // mass bookkeeping for the box (sumParticleMass and the fixed-point unit)
     Box& B = c->boxes[box];
     double mref = 0.0, total = 0.0;
@@ -1606,8 +1738,8 @@ int nyxpm_particles_download(nyxpm_ctx* c, int box, double* x, double* y, double
     double* tmp = c->P[1 - c->cur].x;
     for (int q = 0; q < 6; ++q) {
         if (!dst[q]) continue;
-        hipLaunchKernelGGL(k_unsort, dim3(nblocks(B.np)), dim3(kBlock), 0, c->stream, B.part_off, B.np, A.id,
-                           src[q], tmp);
+        hipLaunchKernelGGL(k_unsort, dim3(nblocks(B.span)), dim3(kBlock), 0, c->stream, B.part_off, B.span, A.x,
+                           A.id, src[q], tmp);
         LAUNCHCK(c);
         HIPCK(c, hipMemcpyAsync(dst[q], tmp, B.np * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCK(c, hipStreamSynchronize(c->stream));
@@ -1636,10 +1768,7 @@ int nyxpm_assign_density_and_vels(nyxpm_ctx* c, int box, double* out) {
     int rc;
     if ((rc = require_particles(c))) return rc;
     if ((rc = check_box(c, box))) return rc;
-    if (!c->sorted_valid) {
-        if (!c->keys_valid && (rc = phase_keys(c))) return rc;
-        if ((rc = phase_sort(c))) return rc;
-    }
+    if (!c->sorted_valid && (rc = phase_sort(c))) return rc;
     PartPtrs& A = c->P[c->cur];
     const long long N = c->ncell_total;
     const long long H = (long long)c->ntiles_total * c->halo_stride;
@@ -1664,18 +1793,19 @@ int nyxpm_assign_density_and_vels(nyxpm_ctx* c, int box, double* out) {
     }
     const size_t lds = 2 * sizeof(unsigned long long) * (c->tile[0] + 1) * (c->tile[1] + 1) * (c->tile[2] + 1);
     hipLaunchKernelGGL(k_deposit, dim3(c->tseg_blocks), dim3(kBlock), lds, c->stream, c->tseg, c->tseg_n, c->dboxes,
-                       c->tstart, A.x, A.y, A.z, A.m, c->own, c->halo, c->err_flag);
+                       c->tbase, c->cursor, A.x, A.y, A.z, A.m, c->own, c->halo, c->err_flag);
     hipLaunchKernelGGL(k_fold_raw, dim3(c->t_res.blocks), dim3(kBlock), 0, c->stream, c->segs(c->t_res),
                        c->t_res.count, c->dboxes, c->own, c->halo, sums);
     hipLaunchKernelGGL(k_fill_dbl, dim3(1), dim3(kBlock), 0, c->stream, reinterpret_cast<double*>(maxbits),
                        (long long)(3 * nbox), 0.0);
     if (c->pseg_blocks > 0)
         hipLaunchKernelGGL(k_max_mom, dim3(c->pseg_blocks), dim3(kBlock), 0, c->stream, c->pseg, c->pseg_n,
-                           c->dboxes, A.m, A.vx, A.vy, A.vz, maxbits);
+                           c->dboxes, A.x, A.m, A.vx, A.vy, A.vz, maxbits);
     const double* vel[3] = {A.vx, A.vy, A.vz};
     for (int d = 0; d < 3; ++d) {
         hipLaunchKernelGGL(k_deposit_mom, dim3(c->tseg_blocks), dim3(kBlock), lds, c->stream, c->tseg, c->tseg_n,
-                           c->dboxes, c->tstart, A.x, A.y, A.z, A.m, vel[d], maxbits, d, mown, mhalo, c->err_flag);
+                           c->dboxes, c->tbase, c->cursor, A.x, A.y, A.z, A.m, vel[d], maxbits, d, mown, mhalo,
+                           c->err_flag);
         hipLaunchKernelGGL(k_fold_raw, dim3(c->t_res.blocks), dim3(kBlock), 0, c->stream, c->segs(c->t_res),
                            c->t_res.count, c->dboxes, mown, mhalo, sums + (d + 1) * N);
     }
@@ -1797,7 +1927,7 @@ int nyxpm_gather_accel(nyxpm_ctx* c, int box, double* g_out) {
     const bool dev = is_device_ptr(g_out);
     double* out = g_out;
     if (!dev && (rc = dalloc(c, &out, 3 * B.np))) return rc;
-    hipLaunchKernelGGL(k_gather, dim3(nblocks(B.np)), dim3(kBlock), 0, c->stream, c->dboxes, box, c->phi, A.x, A.y,
+    hipLaunchKernelGGL(k_gather, dim3(nblocks(B.span)), dim3(kBlock), 0, c->stream, c->dboxes, box, c->phi, A.x, A.y,
                        A.z, A.id, out, c->err_flag);
     hipError_t e = hipGetLastError();
     if (!dev && e == hipSuccess)
@@ -1887,7 +2017,10 @@ int nyxpm_redistribute(nyxpm_ctx* c) {
     hipLaunchKernelGGL(k_wrap, dim3(c->pseg_blocks), dim3(kBlock), 0, c->stream, c->pseg, c->pseg_n, c->dboxes, A.x,
                        A.y, A.z, c->err_flag);
     LAUNCHCK(c);
+    // a shift can change a particle's tile (rounding at the upper boundary): the
+    // next deposit sorts in full
     c->keys_valid = c->sorted_valid = c->density_valid = false;
+    c->moves_pending = false;
     c->dt_valid = false;
     return nyxpm_sync(c);
 }
@@ -1918,6 +2051,16 @@ int nyxpm_pm_steps(nyxpm_ctx* c, int nsteps, double dt, double a_old, double a_n
     // run as one particle pass (bit-identical to the separate kernels).
     const bool fused = !nyx_faithful && !c->separate_kicks;
     const double a_half = 0.5 * (a_old + a_new);
+    // a captured step cannot read back k_migrate's flags: full sorts throughout
+    struct Suspend {
+        nyxpm_ctx* c;
+        bool old;
+        ~Suspend() { c->inc_suspended = old; }
+    } suspend{c, c->inc_suspended};
+    if (use_graph) {
+        if (c->moves_pending && (rc = phase_sort(c))) return rc;
+        c->inc_suspended = true;
+    }
     if (fused && nsteps > 0 && (rc = phase_kick<kKickDrift>(c, dt, a_old, a_half))) return rc;
     auto one_step = [&](int s) -> int {
         // every step of this call runs from a_old to a_new

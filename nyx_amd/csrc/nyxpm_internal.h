@@ -4,7 +4,9 @@
 // Layout in HBM
 //  * particles: structure of arrays (x, y, z, vx, vy, vz FP64, optional mass
 //    FP64, int32 id), boxes stored one after another, each box's particles kept
-//    in tile order after every deposit (double-buffered for the sort).
+//    grouped by deposit tile (double-buffered for the full sort).  Each tile owns
+//    a slot range with slack; a slot whose x is NaN holds no particle, so kernels
+//    that walk a box's slots skip those.
 //  * cell fields: x-fastest, no ghost cells; periodic neighbours are found by
 //    index wrapping.  All boxes share one pool per field; a box's level-l array
 //    starts at a per-(box, level) offset.
@@ -31,8 +33,10 @@ struct BoxDev {
     int fx_exp;            // E: deposit fixed-point unit is 2^(E-31) (hi) and 2^(E-63) (lo)
     long long cell_off;    // level-0 offset in the cell pools
     long long tile_off;    // first global tile id
-    long long part_off;    // first particle of this box
+    long long part_off;    // first particle slot of this box
     long long np;          // particles in this box
+    long long span;        // particle slots of this box (np plus the tile slack; a
+                           // slot with x = NaN holds no particle)
     double plo[3];
     double dxi[3];
     double rhi[3];         // largest coordinate kept by the periodic shift

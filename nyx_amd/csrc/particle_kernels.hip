@@ -63,6 +63,22 @@ __device__ __forceinline__ int agg_atomic_add(int* counters, int key, bool valid
     return result;
 }
 
+// Appends v to list (length counter n) for the active lanes with pred set: one
+// atomic per wave, safe in divergent code (only active lanes take part).
+__device__ __forceinline__ void wave_append(long long* list, int* n, long long v, bool pred) {
+    const unsigned long long mask = __ballot(pred);
+    if (mask == 0ull) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(n, (int)__popcll(mask));
+    base = __shfl(base, leader);
+    if (pred) list[base + (int)__popcll(mask & ((1ull << lane) - 1ull))] = v;
+}
+
+// x of an empty particle slot.
+__device__ __forceinline__ double empty_slot() { return __longlong_as_double(0x7ff8000000000000LL); }
+
 // i / T for 0 <= i < 2^17 and T <= 32 with magic = ceil(2^31 / T) (exact over that
 // range; the bound is checked when a context is created).
 __device__ __forceinline__ int tile_div(int i, unsigned magic) {
@@ -89,14 +105,38 @@ __global__ void __launch_bounds__(kBlock) k_keys(const BoxSeg* segs, int nseg, c
     const BoxSeg sg = segs[find_seg(segs, nseg, blockIdx.x)];
     const BoxDev& B = boxes[sg.box];
     const long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x;
-    const bool valid = q < B.np;
+    bool valid = false;
     int t = 0;
-    if (valid) {
+    if (q < B.span) {
         const long long p = B.part_off + q;
-        t = tile_of(B, x[p], y[p], z[p], err);
-        key[p] = t;
+        const double px = x[p];
+        valid = !isnan(px);  // NaN: an empty slot, keyed -1
+        if (valid) t = tile_of(B, px, y[p], z[p], err);
+        key[p] = valid ? t : -1;
     }
     agg_atomic_add(counts, t, valid);
+}
+
+// Slot capacity of each tile for a sort with slack: cnt + floor(cnt * num / 1024)
+// + pad (integer arithmetic, so a box's capacities never exceed its slots).
+__global__ void __launch_bounds__(kBlock) k_tile_caps(const int* __restrict__ counts, long long n, int num,
+                                                      int pad, int* __restrict__ cap) {
+    const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (t < n) {
+        const long long c = counts[t];
+        cap[t] = (int)(c + ((c * num) >> 10) + pad);
+    }
+}
+
+// First slot of each tile: the scan of the capacities restarted at each box's
+// first slot (boxes keep fixed slot ranges).
+__global__ void __launch_bounds__(kBlock) k_tile_base(const BoxSeg* tsegs, int ntseg, const BoxDev* boxes,
+                                                      const long long* __restrict__ start, long long n,
+                                                      long long* __restrict__ tbase) {
+    const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n) return;
+    const BoxDev& B = boxes[tsegs[find_seg(tsegs, ntseg, (int)t)].box];
+    tbase[t] = start[t] - start[B.tile_off] + B.part_off;
 }
 
 constexpr int kScanItems = 16;  // entries per thread in the scan passes
@@ -182,20 +222,27 @@ struct PartPtrs {
     int* id;
 };
 
-// Move every particle to its tile's slot (order inside a tile is arbitrary).
+// Move every particle to its tile's slot range (order inside a tile is
+// arbitrary); cursor[t] ends as the tile's particle count.
 __global__ void __launch_bounds__(kBlock) k_scatter(const BoxSeg* segs, int nseg, const BoxDev* boxes,
                                                     const int* __restrict__ key,
-                                                    const long long* __restrict__ start, int* cursor,
+                                                    const long long* __restrict__ tbase, int* cursor,
                                                     PartPtrs src, PartPtrs dst) {
     const BoxSeg sg = segs[find_seg(segs, nseg, blockIdx.x)];
     const BoxDev& B = boxes[sg.box];
     const long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x;
-    const bool valid = q < B.np;
     const long long p = B.part_off + q;
-    const int t = valid ? key[p] : 0;
+    // occupied slots only (a kick-drift keys the slots it visits, not the free ones)
+    bool valid = false;
+    int t = 0;
+    if (q < B.span) {
+        const int k = key[p];
+        valid = k >= 0 && !isnan(src.x[p]);
+        t = valid ? k : 0;
+    }
     const int r = agg_atomic_add(cursor, t, valid);
     if (valid) {
-        const long long d = start[t] + r;
+        const long long d = tbase[t] + r;
         dst.x[d] = src.x[p];
         dst.y[d] = src.y[p];
         dst.z[d] = src.z[p];
@@ -226,7 +273,8 @@ __device__ __forceinline__ int halo_index(int lx, int ly, int lz, int bx, int by
 // weights, product left to right) of the tile's particles go to LDS as fixed
 // point; interior cells are written to `own`, the low halo layer to `halo`.
 __global__ void __launch_bounds__(kBlock) k_deposit(const BoxSeg* segs, int nseg, const BoxDev* boxes,
-                                                    const long long* __restrict__ start,
+                                                    const long long* __restrict__ tbase,
+                                                    const int* __restrict__ thw,
                                                     const double* __restrict__ x, const double* __restrict__ y,
                                                     const double* __restrict__ z, const double* __restrict__ m,
                                                     double* __restrict__ own, double* __restrict__ halo, int* err) {
@@ -243,12 +291,13 @@ __global__ void __launch_bounds__(kBlock) k_deposit(const BoxSeg* segs, int nseg
     __syncthreads();
 
     const long long gt = B.tile_off + lt;
-    const long long p0 = start[gt], p1 = start[gt + 1];
+    const long long p0 = tbase[gt], p1 = p0 + thw[gt];
     const int E = B.fx_exp;
     // one particle's CIC contributions as two fixed-point words: w * 2^(31-E) =
     // hi + lo * 2^-32 (hi < 2^32 since w < 2^(E+1); lo rounded, a rounding up to
     // 2^32 carried into hi), both converted with the native f64 -> u32 instruction
     auto dep = [&](double px, double py, double pz, double mp) {
+        if (isnan(px)) return;  // empty slot
         double xint, yint, zint;
         const int i = cic_index(px, B.plo[0], B.dxi[0], &xint);
         const int j = cic_index(py, B.plo[1], B.dxi[1], &yint);
@@ -313,7 +362,8 @@ __global__ void __launch_bounds__(kBlock) k_deposit(const BoxSeg* segs, int nseg
 // independent like the mass deposit's; own cells to `own`, the low halo layer to
 // `halo` (k_deposit's layout).
 __global__ void __launch_bounds__(kBlock) k_deposit_mom(const BoxSeg* segs, int nseg, const BoxDev* boxes,
-                                                        const long long* __restrict__ start,
+                                                        const long long* __restrict__ tbase,
+                                                        const int* __restrict__ thw,
                                                         const double* __restrict__ x, const double* __restrict__ y,
                                                         const double* __restrict__ z, const double* __restrict__ m,
                                                         const double* __restrict__ v,
@@ -332,12 +382,14 @@ __global__ void __launch_bounds__(kBlock) k_deposit_mom(const BoxSeg* segs, int 
     for (int c = threadIdx.x; c < 2 * LC; c += kBlock) acc[c] = 0ull;
     __syncthreads();
     const long long gt = B.tile_off + lt;
-    const long long p0 = start[gt], p1 = start[gt + 1];
+    const long long p0 = tbase[gt], p1 = p0 + thw[gt];
     const double vmax = __longlong_as_double((long long)maxbits[3 * sg.box + comp]);
     const int E = vmax > 0.0 ? ilogb(vmax) : 0;
     for (long long p = p0 + threadIdx.x; p < p1; p += kBlock) {
+        const double px = x[p];
+        if (isnan(px)) continue;  // empty slot
         double xint, yint, zint;
-        const int i = cic_index(x[p], B.plo[0], B.dxi[0], &xint);
+        const int i = cic_index(px, B.plo[0], B.dxi[0], &xint);
         const int j = cic_index(y[p], B.plo[1], B.dxi[1], &yint);
         const int k = cic_index(z[p], B.plo[2], B.dxi[2], &zint);
         const int iw = wrap_checked(i, B.n[0], err) - o0;
@@ -383,14 +435,15 @@ __global__ void __launch_bounds__(kBlock) k_deposit_mom(const BoxSeg* segs, int 
 
 // max |m v_d| per box and component (bit pattern, atomicMax) for k_deposit_mom.
 __global__ void __launch_bounds__(kBlock) k_max_mom(const BoxSeg* segs, int nseg, const BoxDev* boxes,
-                                                    const double* __restrict__ m, const double* __restrict__ vx,
+                                                    const double* __restrict__ x, const double* __restrict__ m, const double* __restrict__ vx,
                                                     const double* __restrict__ vy, const double* __restrict__ vz,
                                                     unsigned long long* maxbits) {
     const BoxSeg sg = segs[find_seg(segs, nseg, blockIdx.x)];
     const BoxDev& B = boxes[sg.box];
     const long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x;
-    if (q >= B.np) return;
+    if (q >= B.span) return;
     const long long p = B.part_off + q;
+    if (isnan(x[p])) return;
     const double mp = B.has_mass ? m[p] : B.m_const;
     atomicMax(maxbits + 3 * sg.box + 0, dbits(fabs(mp * vx[p])));
     atomicMax(maxbits + 3 * sg.box + 1, dbits(fabs(mp * vy[p])));
@@ -651,14 +704,19 @@ __global__ void __launch_bounds__(kBlock) k_kick(const BoxSeg* segs, int nseg, c
     const BoxSeg sg = segs[find_seg(segs, nseg, blockIdx.x)];
     const BoxDev& B = boxes[sg.box];
     const long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x;
-    const bool valid = q < B.np;
+    const long long p = B.part_off + q;
+    double px = 0.0;
+    if (q < B.span) {
+        px = x[p];
+        if (DO_MOVE && isnan(px)) key[p] = -1;  // empty slot
+    }
+    const bool valid = q < B.span && !isnan(px);
     int t = 0;
     if (valid) {
-        const long long p = B.part_off + q;
         const double half_dt = 0.5 * dt;
         const double a_cur_inv = 1.0 / a_cur;
         const double dt_a_cur_inv = dt * a_cur_inv;
-        double px = x[p], py = y[p], pz = z[p];
+        double py = y[p], pz = z[p];
         double val[3];
         gather_grad(B, phi + B.cell_off, px, py, pz, val, err);
         double ux = a_prev * vx[p] + half_dt * val[0];
@@ -735,8 +793,9 @@ __device__ __forceinline__ void cic_grad_lines(LD ld, double xint, double yint, 
     val[2] = v2;
 }
 
-// Work items for the tile-staged particle kernels: each tile's particle range
-// cut into chunks of at most kChunk particles (empty tiles give none).  Slots are
+// Work items for the tile-staged particle kernels: each tile's occupied slot
+// range [tbase, tbase + thw) cut into chunks of at most kChunk slots (empty
+// tiles give none).  Slots are
 // reserved with one atomic per wave, so the item order varies between runs; every
 // item is processed independently, so results do not.
 constexpr int kChunk = 16 * kBlock;  // default particles per work item (fewer items: less phi staging)
@@ -747,14 +806,15 @@ struct TileItem {
     int tile;       // global tile id
 };
 
-__global__ void __launch_bounds__(kBlock) k_make_items(const long long* __restrict__ start, long long ntiles,
+__global__ void __launch_bounds__(kBlock) k_make_items(const long long* __restrict__ tbase,
+                                                       const int* __restrict__ thw, long long ntiles,
                                                        TileItem* __restrict__ items, int* nitems, int chunk) {
     const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
     int nc = 0;
     long long p0 = 0, cnt = 0;
     if (t < ntiles) {
-        p0 = start[t];
-        cnt = start[t + 1] - p0;
+        p0 = tbase[t];
+        cnt = thw[t];
         nc = (int)((cnt + chunk - 1) / chunk);
     }
     // wave-inclusive prefix of nc, one atomic per wave
@@ -853,9 +913,10 @@ __global__ void __launch_bounds__(kBlock) k_est_dt(const BoxSeg* segs, int nseg,
     const BoxDev& B = boxes[sg.box];
     const long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x;
     double d = 1e50;
-    if (q < B.np) {
+    if (q < B.span) {
         const long long p = B.part_off + q;
-        d = particle_dt_global(B, phi + B.cell_off, x[p], y[p], z[p], vx[p], vy[p], vz[p], a, cfl);
+        const double px = x[p];
+        if (!isnan(px)) d = particle_dt_global(B, phi + B.cell_off, px, y[p], z[p], vx[p], vy[p], vz[p], a, cfl);
     }
     block_min_to(d, dtmin + sg.box);
 }
@@ -890,8 +951,11 @@ __device__ __forceinline__ long long defer_entry(long long p, int box) {
 // cell falls outside the item's tile (binning older than a shift).  Deferral
 // keeps those rare paths out of this kernel's registers.
 //
-// Tile counts for the next sort: particles that stay in the item's tile are
-// summed per block (one atomic), the others counted individually.
+// Binning for the next deposit (kick-drifts): with inc, the particles whose new
+// tile differs from the item's are listed for k_migrate; otherwise every
+// particle gets its key, and the tile counts are summed per block for the
+// particles that stay in the item's tile (one atomic), counted individually
+// for the others.
 //
 // EST (closing kicks only): also NyxParticleContainer::estTimestep of the kicked
 // particles (cfl, a = a_cur) into dtmin, from the staged g; a compile-time
@@ -906,7 +970,8 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
                                                        double a_cur, double a_prev2, double a_cur2,
                                                        int* __restrict__ key, int* counts,
                                                        long long* __restrict__ defer, int* ndefer, double cfl,
-                                                       unsigned long long* dtmin, int* err) {
+                                                       unsigned long long* dtmin, int inc,
+                                                       long long* __restrict__ moves, int* nmove, int* err) {
     extern __shared__ double sphi[];
     __shared__ int s_same[kBlock / kWave];
     // items are in (roughly) tile order; giving each XCD a contiguous range of them
@@ -1016,6 +1081,7 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
     double dtmin_t = 1e50;
     // one particle: position (px, py, pz), velocity (qx, qy, qz), index q in the item
     auto one = [&](int q, double px, double py, double pz, double qx, double qy, double qz) {
+        if (isnan(px)) return;  // empty slot
         double xint, yint, zint;
         const int i = cic_index(px, B.plo[0], dxi0, &xint);
         const int j = cic_index(py, B.plo[1], dxi1, &yint);
@@ -1100,11 +1166,16 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
                 Y[q] = py;
                 Z[q] = pz;
                 const int t = tile_of(B, px, py, pz, err);
-                KEY[q] = t;
-                if (t == it.tile)
-                    ++same;
-                else
-                    atomicAdd(&counts[t], 1);
+                if (inc) {
+                    // incremental re-sort: only the particles that left the tile
+                    wave_append(moves, nmove, defer_entry(it.p0 + q, box), t != it.tile);
+                } else {
+                    KEY[q] = t;
+                    if (t == it.tile)
+                        ++same;
+                    else
+                        atomicAdd(&counts[t], 1);
+                }
             }
         } else {
             // binned in another tile: the whole update in k_kick_deferred
@@ -1139,10 +1210,22 @@ __global__ void __launch_bounds__(kBlock) k_kick_deferred(const long long* __res
                                                           double* __restrict__ vy, double* __restrict__ vz,
                                                           double dt, double a_prev, double a_cur, double a_prev2,
                                                           double a_cur2, int* __restrict__ key, int* counts,
-                                                          double cfl, unsigned long long* dtmin, int* err) {
+                                                          double cfl, unsigned long long* dtmin, int inc,
+                                                          long long* __restrict__ moves, int* nmove, int* err) {
     const int n = *ndefer;
     const bool est = MODE != kKickDrift && cfl > 0;
     const double half_dt = 0.5 * dt;
+    // binning of a drifted particle (k_kick_tiled's rule; k_migrate finds out
+    // whether a listed particle changed tile)
+    auto bin = [&](long long p, int box, const BoxDev& B, double px, double py, double pz) {
+        if (inc) {
+            wave_append(moves, nmove, defer_entry(p, box), true);
+        } else {
+            const int t = tile_of(B, px, py, pz, err);
+            key[p] = t;
+            atomicAdd(&counts[t], 1);
+        }
+    };
     for (int q = blockIdx.x * kBlock + threadIdx.x; q < n; q += gridDim.x * kBlock) {
         const long long e = defer[q];
         const long long p = e & ((1LL << 40) - 1);
@@ -1172,9 +1255,7 @@ __global__ void __launch_bounds__(kBlock) k_kick_deferred(const long long* __res
                 pz = pz + dt_a_cur_inv * uz;
                 vx[p] = ux; vy[p] = uy; vz[p] = uz;
                 x[p] = px; y[p] = py; z[p] = pz;
-                const int t = tile_of(B, px, py, pz, err);
-                key[p] = t;
-                atomicAdd(&counts[t], 1);
+                bin(p, box, B, px, py, pz);
                 continue;
             }
             const double sxp = periodic_shift(px, B.plo[0], B.rhi[0], B.len[0], err);
@@ -1213,10 +1294,57 @@ __global__ void __launch_bounds__(kBlock) k_kick_deferred(const long long* __res
         pz = pz + dt_a_cur2_inv * uz;
         vx[p] = ux; vy[p] = uy; vz[p] = uz;
         x[p] = px; y[p] = py; z[p] = pz;
-        const int t = tile_of(B, px, py, pz, err);
-        key[p] = t;
-        atomicAdd(&counts[t], 1);
+        bin(p, box, B, px, py, pz);
     }
+}
+
+// Incremental re-sort.  Each listed particle whose position now lies in another
+// tile than the one whose slot range holds it is copied into the free slack of
+// its new tile (slot thw[t]++), and its old slot is emptied (x = NaN).  A tile
+// with no free slot sets flags[0]: the particle stays where it is and the
+// caller sorts in full.  flags[1] counts the particles moved (= slots emptied).
+// Destinations lie at or above every tile's high-water mark at launch, so no
+// destination is a listed source.
+__global__ void __launch_bounds__(kBlock) k_migrate(const long long* __restrict__ moves, const int* nmove,
+                                                    const BoxDev* boxes, const long long* __restrict__ tbase,
+                                                    const int* __restrict__ tcap, int* thw, PartPtrs P, int* flags,
+                                                    int* err) {
+    const int n = *nmove;
+    int moved = 0;
+    for (int q = blockIdx.x * kBlock + threadIdx.x; q < n; q += gridDim.x * kBlock) {
+        const long long e = moves[q];
+        const long long p = e & ((1LL << 40) - 1);
+        const BoxDev& B = boxes[(int)((e >> 40) & 0xFFFFF)];
+        const double px = P.x[p];
+        if (isnan(px)) continue;
+        const double py = P.y[p], pz = P.z[p];
+        const int t = tile_of(B, px, py, pz, err);
+        // the tile holding slot p: the last of the box whose range starts at or before p
+        int lo = (int)B.tile_off, hi = (int)B.tile_off + B.ntile[0] * B.ntile[1] * B.ntile[2] - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (tbase[mid] <= p) lo = mid; else hi = mid - 1;
+        }
+        if (lo == t) continue;
+        const int slot = atomicAdd(&thw[t], 1);
+        if (slot >= tcap[t]) {
+            atomicOr(&flags[0], 1);
+            continue;
+        }
+        const long long d = tbase[t] + slot;
+        P.x[d] = px;
+        P.y[d] = py;
+        P.z[d] = pz;
+        P.vx[d] = P.vx[p];
+        P.vy[d] = P.vy[p];
+        P.vz[d] = P.vz[p];
+        if (P.m) P.m[d] = P.m[p];
+        P.id[d] = P.id[p];
+        P.x[p] = empty_slot();
+        ++moved;
+    }
+    for (int o = kWave / 2; o > 0; o >>= 1) moved += __shfl_xor(moved, o);
+    if ((threadIdx.x & (kWave - 1)) == 0 && moved) atomicAdd(&flags[1], moved);
 }
 
 // Periodic shift alone (nyxpm_redistribute).
@@ -1226,8 +1354,9 @@ __global__ void __launch_bounds__(kBlock) k_wrap(const BoxSeg* segs, int nseg, c
     const BoxSeg sg = segs[find_seg(segs, nseg, blockIdx.x)];
     const BoxDev& B = boxes[sg.box];
     const long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x;
-    if (q >= B.np) return;
+    if (q >= B.span) return;
     const long long p = B.part_off + q;
+    if (isnan(x[p])) return;
     x[p] = periodic_shift(x[p], B.plo[0], B.rhi[0], B.len[0], err);
     y[p] = periodic_shift(y[p], B.plo[1], B.rhi[1], B.len[1], err);
     z[p] = periodic_shift(z[p], B.plo[2], B.rhi[2], B.len[2], err);
@@ -1260,11 +1389,13 @@ __global__ void __launch_bounds__(kBlock) k_grav_vector(const BoxDev* boxes, int
     g[2 * nc + q] = 0.5 * (fzl + fzh);
 }
 
-// out[id] = in[p] for one box (undo the tile order for download).
-__global__ void __launch_bounds__(kBlock) k_unsort(long long p0, long long np, const int* __restrict__ id,
-                                                   const double* __restrict__ in, double* __restrict__ out) {
+// out[id] = in[p] for the occupied slots of one box (undo the tile order for
+// download).
+__global__ void __launch_bounds__(kBlock) k_unsort(long long p0, long long span, const double* __restrict__ x,
+                                                   const int* __restrict__ id, const double* __restrict__ in,
+                                                   double* __restrict__ out) {
     const long long q = (long long)blockIdx.x * kBlock + threadIdx.x;
-    if (q >= np) return;
+    if (q >= span || isnan(x[p0 + q])) return;
     out[id[p0 + q]] = in[p0 + q];
 }
 
@@ -1324,10 +1455,12 @@ __global__ void __launch_bounds__(kBlock) k_gather(const BoxDev* boxes, int box,
                                                    double* __restrict__ out, int* err) {
     const BoxDev& B = boxes[box];
     const long long q = (long long)blockIdx.x * kBlock + threadIdx.x;
-    if (q >= B.np) return;
+    if (q >= B.span) return;
     const long long p = B.part_off + q;
+    const double px = x[p];
+    if (isnan(px)) return;
     double val[3];
-    gather_grad(B, phi + B.cell_off, x[p], y[p], z[p], val, err);
+    gather_grad(B, phi + B.cell_off, px, y[p], z[p], val, err);
     const long long o = id[p];
     out[o] = val[0];
     out[B.np + o] = val[1];

@@ -59,7 +59,10 @@ class PMContext:
 
     def __init__(self, geoms, precision="fp64", tile=None, nu1=2, nu2=2, max_iter=200, smoother="fused",
                  kick="tiled", sweep_shape=None, sweep_zchunk=0, kick_chunk=0, pair_kicks=True,
-                 coarse="workgroup", particle_cfl=0.0):
+                 coarse="workgroup", particle_cfl=0.0, resort_slack=0.0):
+        """resort_slack: free slots per particle each tile gets at a full sort, for
+        the incremental re-sort after kick-drifts (0: library default 0.25;
+        negative: a full sort after every kick-drift)."""
         if isinstance(geoms, BoxGeom):
             geoms = [geoms]
         self.geoms = list(geoms)
@@ -82,6 +85,7 @@ class PMContext:
         o.separate_kicks = 0 if pair_kicks else 1
         o.coarse = 1 if coarse == "per_level" else 0
         o.particle_cfl = float(particle_cfl)
+        o.resort_slack = float(resort_slack)
         if tile is not None:
             for d in range(3):
                 o.tile[d] = int(tile[d])
@@ -303,6 +307,12 @@ class PMContext:
 
     def device_bytes(self):
         return int(self.L.nyxpm_device_bytes(self.h))
+
+    def sort_stats(self):
+        """{'full', 'incremental', 'moved', 'holes'}: re-sort counters since creation."""
+        out = np.zeros(4, dtype=np.int64)
+        self._ck(self.L.nyxpm_sort_stats(self.h, ptr(out)))
+        return dict(zip(("full", "incremental", "moved", "holes"), (int(v) for v in out)))
 
 
 # ---------------------------------------------------------------------------
