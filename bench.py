@@ -36,6 +36,20 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+TILE = (12, 8, 8)  # the library's default deposit/kick tile
+
+
+def tile_halo_cells(n, tile):
+    """Low-halo cells over all tiles of a box: sum of (bx+1)(by+1)(bz+1) - bx*by*bz
+    over the tiles (the last tile along a dimension may be partial)."""
+    sizes = []
+    for d in range(3):
+        full, rem = divmod(n[d], tile[d])
+        sizes.append([tile[d]] * full + ([rem] if rem else []))
+    return sum((bx + 1) * (by + 1) * (bz + 1) - bx * by * bz
+               for bx in sizes[0] for by in sizes[1] for bz in sizes[2])
+
+
 def kernel_bytes(name, ncell, npart, halo_cells, mixed_sz):
     """Algorithmic (compulsory) bytes of one launch of a kernel class at level 0."""
     sz = mixed_sz
@@ -54,8 +68,8 @@ def kernel_bytes(name, ncell, npart, halo_cells, mixed_sz):
         "deposit": 24 * P + 8 * N + 8 * halo_cells,        # read positions; write own + halos
         "fold": 8 * N + 8 * halo_cells + 16 * N,           # read own + halos; write rho, f
         "rhs": 16 * N,
-        "kick_drift": 96 * P + 4 * P + 8 * N,              # read/write x,v; key; read phi
-        "kick_pair": 96 * P + 4 * P + 8 * N,               # kick + next kick-drift: same traffic
+        "kick_drift": 96 * P + 8 * N,                      # read/write x,v; read phi
+        "kick_pair": 96 * P + 8 * N,                       # kick + next kick-drift: same traffic
         "kick": 72 * P + 8 * N,                            # read x,v; write v; read phi
         "scatter": 4 * P + 2 * 56 * P,                     # key; move x,v,id
         "keys": 24 * P + 4 * P,
@@ -205,8 +219,7 @@ def main():
             [f"{k}.d{d}" for k in ("sweep0", "sweep", "sweep_p", "sweep_u", "smooth", "resid_restrict", "prolong",
                                     "fill")
              for d in range(0, 12)]
-    tile = 8
-    halo_cells = (ncell // tile ** 3) * ((tile + 1) ** 3 - tile ** 3)
+    halo_cells = tile_halo_cells((n, n, n), TILE)
 
     t_mix = t_64 = None
     it_mix = it_64 = None

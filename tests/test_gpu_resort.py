@@ -10,7 +10,7 @@ pytestmark = pytest.mark.gpu
 
 def _hot(nyx, n, seed, vscale, mass=False):
     """Lattice GRF particles plus random velocities of rms vscale (box units per
-    unit time): at dt = 1e-3 and 8-cell tiles a large share changes tile per step."""
+    unit time): at dt = 1e-3 and tiles of 8-12 cells a large share changes tile per step."""
     pos, vel, m = nyx.ics.grf_lattice(n, seed, -2.0, 0.5, 0.1)
     r = np.random.default_rng(seed)
     vel = [v + vscale * r.standard_normal(len(v)) for v in vel]
