@@ -506,7 +506,9 @@ constexpr int kResTX = 64, kResTY = 4;
 template <typename T>
 __global__ void __launch_bounds__(kResTX * kResTY) k_residual0(const CellSeg* segs, int nseg, const int* active,
                                                               const double* __restrict__ f,
-                                                              const double* __restrict__ phi, T* __restrict__ r,
+                                                              const double* __restrict__ solv, double Ggravity,
+                                                              double inv_a, const double* __restrict__ phi,
+                                                              T* __restrict__ r,
                                                               unsigned long long* nr, unsigned long long* nf) {
     const int bid = xcd_block(blockIdx.x, gridDim.x);
     const int s = find_seg(segs, nseg, bid);
@@ -523,6 +525,10 @@ __global__ void __launch_bounds__(kResTX * kResTY) k_residual0(const CellSeg* se
     const double* F = f + sg.coff;
     T* Rr = r + sg.off;
     const double dhx = sg.dh[0], dhy = sg.dh[1], dhz = sg.dh[2];
+    // with solv: f holds rho - mass_offset; the right-hand side is k_rhs_final's
+    // ((f - solv) * G) * (1/a), formed here instead of in a pass of its own
+    const bool raw = solv != nullptr;
+    const double ns = raw ? -solv[sg.box] : 0.0;
     double ar = 0.0, af = 0.0;
     if (i < nx && j < ny) {
         const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
@@ -539,7 +545,11 @@ __global__ void __launch_bounds__(kResTX * kResTY) k_residual0(const CellSeg* se
             const double lx = dhx * (P[zo + oxm] - 2.0 * pc + P[zo + oxp]);
             const double ly = dhy * (P[zo + oym] - 2.0 * pc + P[zo + oyp]);
             const double lz = dhz * (pm - 2.0 * pc + pp);
-            const double fv = F[zo + col];
+            double fv = F[zo + col];
+            if (raw) {
+                const double t = fv + ns;
+                fv = (t * Ggravity) * inv_a;
+            }
             const double v = fv - (lx + ly + lz);
             Rr[zo + col] = T(v);
             ar = fmax(ar, fabs(v));

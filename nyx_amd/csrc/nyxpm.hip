@@ -151,6 +151,10 @@ struct nyxpm_ctx {
     bool keys_valid = false, sorted_valid = false, density_valid = false;
     bool phi_valid = false;
     bool f1_valid = false;  // f holds rho - mass_offset and partial holds its block sums
+    // f_raw: f holds rho - mass_offset and the residual kernels apply the
+    // solvability offset and scaling on the fly, ((f - solv) * f_G) * f_inv_a
+    bool f_raw = false;
+    double f_G = 0.0, f_inv_a = 0.0;
 
     // tiles: counts of the last binning, and the slot layout of the current
     // buffer (tile t owns slots tbase[t] .. tbase[t] + tcap[t] - 1, of which the
@@ -934,6 +938,7 @@ int phase_deposit(nyxpm_ctx* c, bool want_rho) {
     LAUNCHCK(c);
     c->density_valid = want_rho;
     c->f1_valid = true;
+    c->f_raw = false;
     return NYXPM_OK;
 }
 
@@ -948,11 +953,24 @@ int phase_rhs(nyxpm_ctx* c, double a, double four_pi_G) {
                                c->t_res.count, c->dboxes, c->rho, c->f, c->partial);
         hipLaunchKernelGGL(k_rhs_sum, dim3(c->t_res.count), dim3(kBlock), 0, c->stream, c->segs(c->t_res),
                            c->partial, c->dv_size, c->solv);
-        hipLaunchKernelGGL(k_rhs_final, dim3(c->t_l0.blocks), dim3(kBlock), 0, c->stream, c->segs(c->t_l0),
-                           c->t_l0.count, c->dboxes, c->solv, c->f, Ggravity, inv_a);
     }
     LAUNCHCK(c);
+    // the offset and scaling are applied where f is read (k_residual0), so f is
+    // not rewritten; rhs_materialize writes the final values when asked for
     c->f1_valid = false;
+    c->f_raw = true;
+    c->f_G = Ggravity;
+    c->f_inv_a = inv_a;
+    return NYXPM_OK;
+}
+
+// f = ((f - solv) * G) * (1/a) in place (the values the residual kernels use).
+int rhs_materialize(nyxpm_ctx* c) {
+    if (!c->f_raw) return NYXPM_OK;
+    hipLaunchKernelGGL(k_rhs_final, dim3(c->t_l0.blocks), dim3(kBlock), 0, c->stream, c->segs(c->t_l0),
+                       c->t_l0.count, c->dboxes, c->solv, c->f, c->f_G, c->f_inv_a);
+    LAUNCHCK(c);
+    c->f_raw = false;
     return NYXPM_OK;
 }
 
@@ -1101,8 +1119,8 @@ template <typename T>
 int residual(nyxpm_ctx* c, bool with_f) {
     Timed t(c, "residual");
     hipLaunchKernelGGL(k_residual0<T>, dim3(c->t_resz.blocks), dim3(kResTX * kResTY), 0, c->stream,
-                       c->segs(c->t_resz), c->t_resz.count, c->active, c->f, c->phi, Pools<T>::r(c), c->nr,
-                       with_f ? c->nf : nullptr);
+                       c->segs(c->t_resz), c->t_resz.count, c->active, c->f, c->f_raw ? c->solv : nullptr,
+                       c->f_G, c->f_inv_a, c->phi, Pools<T>::r(c), c->nr, with_f ? c->nf : nullptr);
     return NYXPM_OK;
 }
 
@@ -1879,6 +1897,7 @@ int nyxpm_solve_poisson(nyxpm_ctx* c, int box, const double* rhs, double* phi_io
     if (c->boxes.size() != 1) FAIL(c, NYXPM_ERR_ARG, "nyxpm_solve_poisson works on single-box contexts");
     const Box& B = c->boxes[box];
     HIPCK(c, hipMemcpyAsync(c->f + B.cell_off, rhs, B.lev[0].nc * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    c->f_raw = false;  // the caller's f is final
     HIPCK(c, hipMemcpyAsync(c->phi + B.cell_off, phi_io, B.lev[0].nc * sizeof(double), hipMemcpyHostToDevice,
                             c->stream));
     if ((rc = solve(c, rel_tol, abs_tol, fixed_iters, 0))) return rc;
@@ -1989,7 +2008,12 @@ int nyxpm_set_phi(nyxpm_ctx* c, int box, const double* in) {
     }
     return rc;
 }
-int nyxpm_get_rhs(nyxpm_ctx* c, int box, double* out) { return field_io(c, box, c ? c->f : nullptr, out, true); }
+int nyxpm_get_rhs(nyxpm_ctx* c, int box, double* out) {
+    if (!c) return NYXPM_ERR_ARG;
+    int rc;
+    if ((rc = rhs_materialize(c))) return rc;
+    return field_io(c, box, c->f, out, true);
+}
 
 int nyxpm_move_kick_drift(nyxpm_ctx* c, double dt, double a_old, double a_half) {
     if (!c) return NYXPM_ERR_ARG;

@@ -447,3 +447,25 @@ def test_density_and_vels_uniform_velocity(nyx):
     has = out[0] > 0
     for d, v in enumerate((0.7, -3.0, 1e-3)):
         assert np.allclose(out[1 + d][has], v, rtol=1e-14, atol=0)
+
+
+def test_rhs_after_solve_is_the_corrected_scaled_density(nyx, orc):
+    # the library keeps rho - mass_offset in f and applies the solvability
+    # offset and -4 pi G / a where the residual reads f; get_rhs writes the final
+    # values out (Gravity.cpp CorrectRhsUsingOffset and the scaling before solve)
+    n, plo, phi = (32, 24, 16), (0.0, -1.0, 0.5), (1.0, -0.25, 1.0)  # cubic cells, dx = 1/32
+    pos, vel, m = nyx.ics.grf_lattice(16, 5, -2.0, 0.5, 0.1)
+    pos = [plo[d] + (phi[d] - plo[d]) * pos[d] for d in range(3)]
+    with ctx1(nyx, n, plo, phi) as c:
+        c.upload(0, pos, vel, m_const=m)
+        rho = c.assign_density()
+        c.solve_for_phi(a=2.0, four_pi_G=3.0, rel_tol=1e-10)
+        f = c.get_rhs()
+        f2 = c.get_rhs()  # materialised once, then read as is
+        it, _ = c.solve_for_phi(a=2.0, four_pi_G=3.0, rel_tol=1e-10)  # f rebuilt from the density
+        f3 = c.get_rhs()
+    ref = np.ascontiguousarray(rho, dtype=np.float64).copy()
+    orc.correct_rhs(n, plo, phi, ref, orc.mass_offset(plo, phi, m * len(pos[0])))
+    orc.scale_rhs(ref, 3.0, 2.0)
+    assert relmax(f, ref) <= 1e-13
+    assert np.array_equal(f, f2) and np.array_equal(f, f3)
