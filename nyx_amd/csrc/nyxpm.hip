@@ -795,6 +795,20 @@ int alloc_particle_set(nyxpm_ctx* c, PartPtrs& P, long long n, bool mass) {
     return NYXPM_OK;
 }
 
+// The idle particle buffer, allocated on first use.
+int ensure_idle_buffer(nyxpm_ctx* c) {
+    PartPtrs& B = c->P[1 - c->cur];
+    if (B.x) return NYXPM_OK;
+    int rc = alloc_particle_set(c, B, std::max(c->part_cap, 1LL), c->any_mass);
+    if (rc) {
+        PartPtrs none = {};
+        for (double** p : {&B.x, &B.y, &B.z, &B.vx, &B.vy, &B.vz, &B.m}) dfree(*p);
+        dfree(B.id);
+        B = none;
+    }
+    return rc;
+}
+
 // ---------------------------------------------------------------------------
 // device phases
 // ---------------------------------------------------------------------------
@@ -876,6 +890,7 @@ int phase_sort(nyxpm_ctx* c) {
         c->keys_valid = false;
     }
     if (!c->keys_valid && (rc = phase_keys(c))) return rc;
+    if ((rc = ensure_idle_buffer(c))) return rc;
     PartPtrs& A = c->P[c->cur];
     PartPtrs& B = c->P[1 - c->cur];
     const bool slack = c->slack_num > 0;
@@ -1221,6 +1236,7 @@ int phase_kick(nyxpm_ctx* c, double dt, double a_prev, double a_cur, double a_pr
     int rc;
     // a kick-drift's list of tile changes is used up before the particles move again
     if (c->moves_pending && (rc = phase_sort(c))) return rc;
+    if ((rc = ensure_idle_buffer(c))) return rc;
     PartPtrs& A = c->P[c->cur];
     constexpr bool MOVE = MODE != kKick;
     // staged phi tile with a halo of 2, plus the three components of g on the
@@ -1553,11 +1569,12 @@ static int upload_impl(nyxpm_ctx* c, int box, int64_t np, const double* x, const
         slots += new_span[b];
     }
     const bool need_mass = c->any_mass || (m != nullptr);
+    // the second buffer (sort target, kick lists, download staging) is allocated
+    // when first needed, after the caller may have released its upload source
     PartPtrs N[2] = {};
     int* nkey = nullptr;
     long long cap = std::max(slots, 1LL);
-    for (int s = 0; s < 2; ++s)
-        if ((rc = alloc_particle_set(c, N[s], cap, need_mass))) return rc;
+    if ((rc = alloc_particle_set(c, N[0], cap, need_mass))) return rc;
     if ((rc = dalloc(c, &nkey, cap))) return rc;
     // keep the other boxes' slots as they are (empty slots included)
     PartPtrs& O = c->P[c->cur];
@@ -1749,6 +1766,7 @@ int nyxpm_particles_download(nyxpm_ctx* c, int box, double* x, double* y, double
     if ((rc = check_box(c, box))) return rc;
     const Box& B = c->boxes[box];
     if (B.np == 0) return NYXPM_OK;
+    if ((rc = ensure_idle_buffer(c))) return rc;
     PartPtrs& A = c->P[c->cur];
     double* src[6] = {A.x, A.y, A.z, A.vx, A.vy, A.vz};
     double* dst[6] = {x, y, z, vx, vy, vz};

@@ -462,13 +462,78 @@ __device__ __forceinline__ double block_sum_fixed(double v) {
     return t;
 }
 
-// rho = (own + halos of the up-to-7 upper neighbour tiles, fixed order) * 1/vol;
-// f = rho + (-mass_offset); per-block sum of f for the solvability correction.
-// Tile-centric: block b of a box takes a contiguous range of tiles (so the two
-// halves of a 128-byte line of own/f, and the halo arrays of the x neighbours,
-// are touched by the same block close in time), one thread per cell of the tile.
-// rho itself is written only when the caller wants the density (AssignDensity);
-// the step needs only f.
+// Cell (i, j, k) of a box after SumBoundary: its own deposit plus the low-halo
+// entries of the up-to-7 upper neighbour tiles that hold it, added in a fixed
+// order (neighbour in x, y, z, xy, xz, yz, xyz).
+__device__ __forceinline__ double fold_cell(const BoxDev& B, const double* own, const double* __restrict__ halo,
+                                            int i, int j, int k, size_t q) {
+    const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
+    const int T0 = B.tile[0], T1 = B.tile[1], T2 = B.tile[2];
+    const int nt0 = B.ntile[0], nt1 = B.ntile[1], nt2 = B.ntile[2];
+    const int tx = tile_div(i, B.tdiv[0]), ty = tile_div(j, B.tdiv[1]), tz = tile_div(k, B.tdiv[2]);
+    const int li = i - tx * T0, lj = j - ty * T1, lk = k - tz * T2;
+    const int bx = min(T0, nx - tx * T0), by = min(T1, ny - ty * T1), bz = min(T2, nz - tz * T2);
+    const bool topx = li == bx - 1, topy = lj == by - 1, topz = lk == bz - 1;
+    double v = own[q];
+    if (topx || topy || topz) {
+        const int sel[7] = {1, 2, 4, 3, 5, 6, 7};
+#pragma unroll
+        for (int s = 0; s < 7; ++s) {
+            const int S = sel[s];
+            const bool sx = S & 1, sy = S & 2, sz = S & 4;
+            if ((sx && !topx) || (sy && !topy) || (sz && !topz)) continue;
+            int ntx = tx + (sx ? 1 : 0), nty = ty + (sy ? 1 : 0), ntz = tz + (sz ? 1 : 0);
+            if (ntx == nt0) ntx = 0;
+            if (nty == nt1) nty = 0;
+            if (ntz == nt2) ntz = 0;
+            const int nbx = min(T0, nx - ntx * T0), nby = min(T1, ny - nty * T1), nbz = min(T2, nz - ntz * T2);
+            const int hx = sx ? 0 : li + 1, hy = sy ? 0 : lj + 1, hz = sz ? 0 : lk + 1;
+            const long long nt = B.tile_off + ((long long)ntz * nt1 + nty) * nt0 + ntx;
+            v += halo[nt * B.halo_stride + halo_index(hx, hy, hz, nbx, nby, nbz)];
+        }
+    }
+    return v;
+}
+
+// Row-major walk over a block's share of a box's cells: block b of the piece takes
+// cells [b*per, (b+1)*per) (per a multiple of kBlock, so whole cache lines), each
+// thread every kBlock-th of them in order; body(i, j, k, c) per cell.  The cell
+// order of each thread is fixed, so per-thread sums are reproducible.
+template <class F>
+__device__ __forceinline__ void for_block_cells(const CellSeg& sg, const BoxDev& B, F body) {
+    const int nx = B.n[0], ny = B.n[1];
+    const long long N = (long long)nx * ny * B.n[2];
+    long long per = (N + sg.nblk - 1) / sg.nblk;
+    per = (per + kBlock - 1) / kBlock * kBlock;
+    const long long c0 = (long long)(blockIdx.x - sg.blk0) * per;
+    const long long c1 = min(N, c0 + per);
+    long long c = c0 + threadIdx.x;
+    if (c >= c1) return;
+    const long long plane = (long long)nx * ny;
+    int k = (int)(c / plane);
+    const long long rem = c - (long long)k * plane;
+    int j = (int)(rem / nx);
+    int i = (int)(rem - (long long)j * nx);
+    for (; c < c1; c += kBlock) {
+        body(i, j, k, c);
+        i += kBlock;
+        if (i >= nx) {
+            const int t = i / nx;
+            i -= t * nx;
+            j += t;
+            if (j >= ny) {
+                const int u = j / ny;
+                j -= u * ny;
+                k += u;
+            }
+        }
+    }
+}
+
+// rho = (own + halos, fold_cell) * 1/vol; f = rho + (-mass_offset); per-block sum
+// of f for the solvability correction.  Cells in row-major order (coalesced reads
+// of own and writes of f, whole cache lines).  rho itself is written only when the
+// caller wants the density (AssignDensity); the step needs only f.
 template <bool WRITE_RHO>
 __global__ void __launch_bounds__(kBlock) k_fold(const CellSeg* segs, int nseg, const BoxDev* boxes,
                                                  const double* own, const double* __restrict__ halo,
@@ -477,97 +542,30 @@ __global__ void __launch_bounds__(kBlock) k_fold(const CellSeg* segs, int nseg, 
     // rho may be own itself (each cell is read, then written, by one thread)
     const CellSeg& sg = segs[find_seg(segs, nseg, blockIdx.x)];
     const BoxDev& B = boxes[sg.box];
-    const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
-    const int T0 = B.tile[0], T1 = B.tile[1], T2 = B.tile[2];
-    const int nt0 = B.ntile[0], nt1 = B.ntile[1], nt2 = B.ntile[2];
-    const int ntiles = nt0 * nt1 * nt2;
-    const int per = (ntiles + sg.nblk - 1) / sg.nblk;
-    const int t_begin = (blockIdx.x - sg.blk0) * per;
-    const int t_end = min(ntiles, t_begin + per);
     double acc = 0.0;  // this thread's cells, in a fixed order
-    for (int lt = t_begin; lt < t_end; ++lt) {
-        const int tx = lt % nt0, ty = (lt / nt0) % nt1, tz = lt / (nt0 * nt1);
-        const int bx = min(T0, nx - tx * T0), by = min(T1, ny - ty * T1), bz = min(T2, nz - tz * T2);
-        const int ncel = bx * by * bz;
-        for (int c = threadIdx.x; c < ncel; c += kBlock) {
-            const int li = c % bx, lj = (c / bx) % by, lk = c / (bx * by);
-            const bool topx = li == bx - 1, topy = lj == by - 1, topz = lk == bz - 1;
-            const size_t q = B.cell_off + cidx(tx * T0 + li, ty * T1 + lj, tz * T2 + lk, nx, ny);
-            double v = own[q];
-            if (topx || topy || topz) {
-                // subsets of {x,y,z}: x, y, z, xy, xz, yz, xyz
-                const int sel[7] = {1, 2, 4, 3, 5, 6, 7};
-#pragma unroll
-                for (int s = 0; s < 7; ++s) {
-                    const int S = sel[s];
-                    const bool sx = S & 1, sy = S & 2, sz = S & 4;
-                    if ((sx && !topx) || (sy && !topy) || (sz && !topz)) continue;
-                    int ntx = tx + (sx ? 1 : 0), nty = ty + (sy ? 1 : 0), ntz = tz + (sz ? 1 : 0);
-                    if (ntx == nt0) ntx = 0;
-                    if (nty == nt1) nty = 0;
-                    if (ntz == nt2) ntz = 0;
-                    const int nbx = min(T0, nx - ntx * T0), nby = min(T1, ny - nty * T1),
-                              nbz = min(T2, nz - ntz * T2);
-                    const int hx = sx ? 0 : li + 1, hy = sy ? 0 : lj + 1, hz = sz ? 0 : lk + 1;
-                    const long long nt = B.tile_off + ((long long)ntz * nt1 + nty) * nt0 + ntx;
-                    v += halo[nt * B.halo_stride + halo_index(hx, hy, hz, nbx, nby, nbz)];
-                }
-            }
-            const double rv = v * B.inv_vol;
-            if (WRITE_RHO) rho[q] = rv;
-            const double fv = rv + (-B.mass_offset);
-            f[q] = fv;
-            acc += fv;
-        }
-    }
+    for_block_cells(sg, B, [&](int i, int j, int k, long long c) {
+        const size_t q = B.cell_off + (size_t)c;
+        const double v = fold_cell(B, own, halo, i, j, k, q);
+        const double rv = v * B.inv_vol;
+        if (WRITE_RHO) rho[q] = rv;
+        const double fv = rv + (-B.mass_offset);
+        f[q] = fv;
+        acc += fv;
+    });
     const double bs = block_sum_fixed(acc);
     if (threadIdx.x == 0) partial[blockIdx.x] = bs;
 }
 
-// out = own + halos of the up-to-7 upper neighbour tiles (k_fold's order, no
-// scaling): the folded raw sums of a deposit.
+// out = own + halos (fold_cell, no scaling): the folded raw sums of a deposit.
 __global__ void __launch_bounds__(kBlock) k_fold_raw(const CellSeg* segs, int nseg, const BoxDev* boxes,
                                                      const double* __restrict__ own,
                                                      const double* __restrict__ halo, double* __restrict__ out) {
     const CellSeg& sg = segs[find_seg(segs, nseg, blockIdx.x)];
     const BoxDev& B = boxes[sg.box];
-    const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
-    const int T0 = B.tile[0], T1 = B.tile[1], T2 = B.tile[2];
-    const int nt0 = B.ntile[0], nt1 = B.ntile[1], nt2 = B.ntile[2];
-    const int ntiles = nt0 * nt1 * nt2;
-    const int per = (ntiles + sg.nblk - 1) / sg.nblk;
-    const int t_begin = (blockIdx.x - sg.blk0) * per;
-    const int t_end = min(ntiles, t_begin + per);
-    for (int lt = t_begin; lt < t_end; ++lt) {
-        const int tx = lt % nt0, ty = (lt / nt0) % nt1, tz = lt / (nt0 * nt1);
-        const int bx = min(T0, nx - tx * T0), by = min(T1, ny - ty * T1), bz = min(T2, nz - tz * T2);
-        const int ncel = bx * by * bz;
-        for (int c = threadIdx.x; c < ncel; c += kBlock) {
-            const int li = c % bx, lj = (c / bx) % by, lk = c / (bx * by);
-            const bool topx = li == bx - 1, topy = lj == by - 1, topz = lk == bz - 1;
-            const size_t q = B.cell_off + cidx(tx * T0 + li, ty * T1 + lj, tz * T2 + lk, nx, ny);
-            double v = own[q];
-            if (topx || topy || topz) {
-                const int sel[7] = {1, 2, 4, 3, 5, 6, 7};
-#pragma unroll
-                for (int s = 0; s < 7; ++s) {
-                    const int S = sel[s];
-                    const bool sx = S & 1, sy = S & 2, sz = S & 4;
-                    if ((sx && !topx) || (sy && !topy) || (sz && !topz)) continue;
-                    int ntx = tx + (sx ? 1 : 0), nty = ty + (sy ? 1 : 0), ntz = tz + (sz ? 1 : 0);
-                    if (ntx == nt0) ntx = 0;
-                    if (nty == nt1) nty = 0;
-                    if (ntz == nt2) ntz = 0;
-                    const int nbx = min(T0, nx - ntx * T0), nby = min(T1, ny - nty * T1),
-                              nbz = min(T2, nz - ntz * T2);
-                    const int hx = sx ? 0 : li + 1, hy = sy ? 0 : lj + 1, hz = sz ? 0 : lk + 1;
-                    const long long nt = B.tile_off + ((long long)ntz * nt1 + nty) * nt0 + ntx;
-                    v += halo[nt * B.halo_stride + halo_index(hx, hy, hz, nbx, nby, nbz)];
-                }
-            }
-            out[q] = v;
-        }
-    }
+    for_block_cells(sg, B, [&](int i, int j, int k, long long c) {
+        const size_t q = B.cell_off + (size_t)c;
+        out[q] = fold_cell(B, own, halo, i, j, k, q);
+    });
 }
 
 // AssignDensityAndVels for box `box`: rho = mass * 1/vol, v_d = mom_d / mass (0
@@ -1416,33 +1414,20 @@ __global__ void __launch_bounds__(kBlock) k_fill_dbl(double* a, long long n, dou
 
 // f = rho + (-mass_offset) with per-block sums (used when the density was set
 // from outside rather than by the fold).  Cells are visited exactly as k_fold
-// visits them (same tiles per block, same cells per thread, same order), so the
+// visits them (same cells per block and per thread, same order), so the
 // solvability sum of a density does not depend on which of the two produced f.
 __global__ void __launch_bounds__(kBlock) k_rhs1(const CellSeg* segs, int nseg, const BoxDev* boxes,
                                                  const double* __restrict__ rho, double* __restrict__ f,
                                                  double* __restrict__ partial) {
     const CellSeg& sg = segs[find_seg(segs, nseg, blockIdx.x)];
     const BoxDev& B = boxes[sg.box];
-    const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
-    const int T0 = B.tile[0], T1 = B.tile[1], T2 = B.tile[2];
-    const int nt0 = B.ntile[0], nt1 = B.ntile[1], nt2 = B.ntile[2];
-    const int ntiles = nt0 * nt1 * nt2;
-    const int per = (ntiles + sg.nblk - 1) / sg.nblk;
-    const int t_begin = (blockIdx.x - sg.blk0) * per;
-    const int t_end = min(ntiles, t_begin + per);
     double acc = 0.0;
-    for (int lt = t_begin; lt < t_end; ++lt) {
-        const int tx = lt % nt0, ty = (lt / nt0) % nt1, tz = lt / (nt0 * nt1);
-        const int bx = min(T0, nx - tx * T0), by = min(T1, ny - ty * T1), bz = min(T2, nz - tz * T2);
-        const int ncel = bx * by * bz;
-        for (int c = threadIdx.x; c < ncel; c += kBlock) {
-            const int li = c % bx, lj = (c / bx) % by, lk = c / (bx * by);
-            const size_t q = B.cell_off + cidx(tx * T0 + li, ty * T1 + lj, tz * T2 + lk, nx, ny);
-            const double fv = rho[q] + (-B.mass_offset);
-            f[q] = fv;
-            acc += fv;
-        }
-    }
+    for_block_cells(sg, B, [&](int, int, int, long long c) {
+        const size_t q = B.cell_off + (size_t)c;
+        const double fv = rho[q] + (-B.mass_offset);
+        f[q] = fv;
+        acc += fv;
+    });
     const double bs = block_sum_fixed(acc);
     if (threadIdx.x == 0) partial[blockIdx.x] = bs;
 }
