@@ -165,6 +165,8 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true", help="skip the mixed-vs-FP64 acceleration check")
     ap.add_argument("--no-tune", action="store_true", help="skip the smoother-shape autotuner")
     ap.add_argument("--kick-chunk", type=int, default=0, help="particles per kick work item (0: default)")
+    ap.add_argument("--phi-guess", default="default", choices=["default", "previous", "extrapolate"],
+                    help="initial guess of each step's solve (default = previous phi, as Nyx)")
     ap.add_argument("--workload", default="clustered512",
                     choices=["clustered512", "batch48", "grf256", "grf1024"],
                     help="clustered512: the headline line; batch48: 48 boxes (16 each of 64^3, 96^3, "
@@ -191,7 +193,8 @@ def main():
     t = time.perf_counter()
     pos, vel, m = ics.plummer_clustered(n, ics.SEED_BASE + 2, n_spheres=max(1, int(2000 * (n / 512) ** 3)))
     log(f"  {time.perf_counter() - t:.1f} s")
-    ctx = nyx_amd.PMContext(nyx_amd.BoxGeom((n, n, n)), precision="fp64", kick_chunk=a.kick_chunk)
+    ctx = nyx_amd.PMContext(nyx_amd.BoxGeom((n, n, n)), precision="fp64", kick_chunk=a.kick_chunk,
+                            phi_guess=a.phi_guess)
     ctx.upload(0, pos, vel, m_const=m)
     del pos, vel
     npart = n ** 3
@@ -229,12 +232,24 @@ def main():
     t_mix = t_64 = None
     it_mix = it_64 = None
     rs_mix = rs_64 = None
+    rel_l2_end = None
     st_mix, st_64 = {}, {}
     if "mixed" in modes:
         log(f"mixed: {a.warmup} warmup + {a.steps} timed steps")
         t_mix, it_mix, rs_mix = run_mode(ctx, "mixed", a.steps, a.warmup, dt, a.tol, a.graph)
         st_mix = stage_table(ctx, names, ncell, npart, halo_cells, 4, t_mix * 1e3)
         log(f"  {t_mix / a.steps * 1e3:.2f} ms/step, {it_mix} V-cycles")
+        if not a.no_accuracy:
+            # the state the timed mixed steps left: their phi against an FP64 solve of
+            # the same density from a zero guess
+            gmx = ctx.gather_accel()
+            ctx.set_precision("fp64")
+            ctx.assign_density(want=False)
+            ctx.solve_for_phi(1.0, 1.0, a.tol, 0.0, use_guess=False)
+            g64 = ctx.gather_accel()
+            rel_l2_end = float(np.sqrt(np.sum((gmx - g64) ** 2)) / np.sqrt(np.sum(g64 ** 2)))
+            del gmx, g64
+            log(f"  accel rel-L2 mixed vs fp64 after the timed steps = {rel_l2_end:.3e}")
     if "fp64" in modes:
         log(f"fp64: {a.warmup} warmup + {a.steps} timed steps")
         t_64, it_64, rs_64 = run_mode(ctx, "fp64", a.steps, a.warmup, dt, a.tol, a.graph)
@@ -304,6 +319,8 @@ def main():
         "mixed_over_fp64": round(v_mix / v_64, 3),
         "vcycles_mixed": it_mix,
         "accel_rel_l2": rel_l2,
+        "accel_rel_l2_after_steps": rel_l2_end,
+        "phi_guess": a.phi_guess,
         "roofline": roof,
         "smoother_roofline": {"mixed": smoother_roof(st_mix, 4), "fp64": smoother_roof(st_64, 8)},
         "stages_mixed": st_mix,

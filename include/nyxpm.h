@@ -107,6 +107,11 @@ typedef struct nyxpm_opts {
                           used up or the emptied slots pass np/16.  0 = default
                           (0.25), < 0 = off (full sort after every kick-drift),
                           at most 4.  Same results either way. */
+    int phi_guess;     /* initial guess of the solve inside each nyxpm_pm_steps step:
+                          0 = default = 1 = phi_n, as Nyx; 2 = 2 phi_n - phi_{n-1}
+                          (one extra pass over phi; saves a V-cycle where the
+                          previous guess sits just above a cycle boundary).  The
+                          stopping rule is the same.  Not used with nyx_faithful. */
 } nyxpm_opts;
 
 /* ---- lifecycle ---------------------------------------------------------- */

@@ -155,6 +155,13 @@ struct nyxpm_ctx {
     // solvability offset and scaling on the fly, ((f - solv) * f_G) * f_inv_a
     bool f_raw = false;
     double f_G = 0.0, f_inv_a = 0.0;
+    // initial guess of the solves inside nyxpm_pm_steps: 0 (default) and 1 =
+    // previous phi, as Nyx; 2 = 2 phi_n - phi_{n-1}.
+    // phi_prev holds phi_{n-1} when the device flag guess_ok is 1 (a device flag,
+    // so that captured steps extrapolate exactly as eager ones do).
+    int phi_guess = 0;
+    double* phi_prev = nullptr;
+    int* guess_ok = nullptr;
 
     // tiles: counts of the last binning, and the slot layout of the current
     // buffer (tile t owns slots tbase[t] .. tbase[t] + tcap[t] - 1, of which the
@@ -723,6 +730,8 @@ int alloc_fields(nyxpm_ctx* c) {
     if ((rc = dalloc(c, &c->tcap, c->ntiles_total))) return rc;
     if ((rc = dalloc(c, &c->nmove, 1))) return rc;
     if ((rc = dalloc(c, &c->mig, 2))) return rc;
+    if ((rc = dalloc(c, &c->guess_ok, 1))) return rc;
+    HIPCK(c, hipMemset(c->guess_ok, 0, sizeof(int)));
     HIPCK(c, hipHostMalloc((void**)&c->h_mig, 2 * sizeof(int), hipHostMallocDefault));
     c->scan_blocks = nblocks(c->ntiles_total, kBlock * kScanItems);
     if ((rc = dalloc(c, &c->bsum, c->scan_blocks + 1))) return rc;
@@ -976,6 +985,38 @@ int phase_rhs(nyxpm_ctx* c, double a, double four_pi_G) {
     c->f_raw = true;
     c->f_G = Ggravity;
     c->f_inv_a = inv_a;
+    return NYXPM_OK;
+}
+
+// Initial guess of a step's solve (nyxpm_pm_steps): phi_n as Nyx uses it, or on
+// request the linear extrapolation 2 phi_n - phi_{n-1} in time.  The stopping rule
+// is the same either way.
+void invalidate_guess(nyxpm_ctx* c) {
+    if (c->guess_ok) hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(kBlock), 0, c->stream, c->guess_ok, 1LL, 0);
+}
+
+bool guess_wanted(const nyxpm_ctx* c) { return c->phi_guess == 2; }
+
+// phi_{n-1}'s buffer; allocated before any graph capture (no hipMalloc inside one).
+int ensure_guess_buffer(nyxpm_ctx* c) {
+    if (c->phi_prev || !guess_wanted(c)) return NYXPM_OK;
+    return dalloc(c, &c->phi_prev, c->ncell_total);
+}
+
+int prepare_guess(nyxpm_ctx* c) {
+    if (!guess_wanted(c)) {
+        invalidate_guess(c);
+        return NYXPM_OK;
+    }
+    int rc;
+    if ((rc = ensure_guess_buffer(c))) return rc;
+    {
+        Timed t(c, "guess");
+        hipLaunchKernelGGL(k_phi_guess, dim3(nblocks(c->ncell_total)), dim3(kBlock), 0, c->stream, c->phi,
+                           c->phi_prev, c->ncell_total, c->guess_ok);
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(kBlock), 0, c->stream, c->guess_ok, 1LL, 1);
+    }
+    LAUNCHCK(c);
     return NYXPM_OK;
 }
 
@@ -1337,6 +1378,10 @@ int step_body(nyxpm_ctx* c, double dt, double a_old, double a_new, double four_p
     if ((rc = phase_sort(c))) return rc;
     if ((rc = phase_deposit(c, false))) return rc;
     if ((rc = phase_rhs(c, a_new, four_pi_G))) return rc;
+    if (nyx_faithful)
+        invalidate_guess(c);  // Nyx's guess: the old phi just solved
+    else if ((rc = prepare_guess(c)))
+        return rc;
     if ((rc = solve(c, rel, abs_tol, 0, graph_cycles))) return rc;
     if ((rc = phase_kick<kKick>(c, dt, a_half, a_new))) return rc;
     return NYXPM_OK;
@@ -1354,6 +1399,7 @@ int step_tail(nyxpm_ctx* c, double dt, double a_old, double a_new, double four_p
     if ((rc = phase_sort(c))) return rc;
     if ((rc = phase_deposit(c, false))) return rc;
     if ((rc = phase_rhs(c, a_new, four_pi_G))) return rc;
+    if ((rc = prepare_guess(c))) return rc;
     if ((rc = solve(c, rel, abs_tol, 0, graph_cycles))) return rc;
     if (next_a_new > 0.0)
         return phase_kick<kKickPair>(c, dt, a_half, a_new, next_a_old, 0.5 * (next_a_old + next_a_new));
@@ -1434,6 +1480,7 @@ int nyxpm_create_batch(nyxpm_ctx** out, int nbox, const nyxpm_geom* geoms, const
         }
         if (opts->sweep_zchunk > 0) c->sweep_zchunk = opts->sweep_zchunk;
         if (opts->kick_chunk > 0) c->kick_chunk = std::max(kBlock, opts->kick_chunk);
+        if (opts->phi_guess >= 0 && opts->phi_guess <= 2) c->phi_guess = opts->phi_guess;
         if (opts->resort_slack < 0) {
             c->slack_num = 0;
             c->slack_pad = 0;
@@ -1471,7 +1518,7 @@ int nyxpm_destroy(nyxpm_ctx* c) {
     if (c->graph) hipGraphDestroy(c->graph);
     free_particles(c);
     dfree(c->counts); dfree(c->cursor); dfree(c->tstart); dfree(c->bsum);
-    dfree(c->tbase); dfree(c->tcap); dfree(c->nmove); dfree(c->mig);
+    dfree(c->tbase); dfree(c->tcap); dfree(c->nmove); dfree(c->mig); dfree(c->phi_prev); dfree(c->guess_ok);
     if (c->h_mig) hipHostFree(c->h_mig);
     c->rho = nullptr;
     dfree(c->own); dfree(c->halo); dfree(c->f); dfree(c->phi); dfree(c->partial);
@@ -1652,6 +1699,7 @@ static int upload_impl(nyxpm_ctx* c, int box, int64_t np, const double* x, const
     }
     c->npart_total = tot;
     c->moves_pending = false;
+    invalidate_guess(c);
     // mass bookkeeping for the box (sumParticleMass and the fixed-point unit)
     Box& B = c->boxes[box];
     double mref = 0.0, total = 0.0;
@@ -1904,6 +1952,7 @@ int nyxpm_solve_for_phi(nyxpm_ctx* c, double a, double four_pi_G, double rel_tol
     if ((rc = solve(c, rel_tol, abs_tol, 0, 0))) return rc;
     if ((rc = solve_results(c, iters, final_resnorm, false))) return rc;
     c->phi_valid = true;
+    invalidate_guess(c);
     return err_check(c);
 }
 
@@ -1928,6 +1977,7 @@ int nyxpm_solve_poisson(nyxpm_ctx* c, int box, const double* rhs, double* phi_io
                             c->stream));
     HIPCK(c, hipStreamSynchronize(c->stream));
     c->phi_valid = true;
+    invalidate_guess(c);
     return rc;
 }
 
@@ -2022,6 +2072,7 @@ int nyxpm_set_phi(nyxpm_ctx* c, int box, const double* in) {
     int rc = field_io(c, box, c ? c->phi : nullptr, const_cast<double*>(in), false);
     if (!rc) {
         c->phi_valid = true;
+        invalidate_guess(c);
         c->dt_valid = false;
     }
     return rc;
@@ -2078,6 +2129,7 @@ int nyxpm_pm_init(nyxpm_ctx* c, double a, double four_pi_G, double rel_tol, doub
     if ((rc = solve(c, rel_tol, abs_tol, 0, 0))) return rc;
     if ((rc = solve_results(c, nullptr, nullptr, false))) return rc;
     c->phi_valid = true;
+    invalidate_guess(c);
     return err_check(c);
 }
 
@@ -2101,6 +2153,7 @@ int nyxpm_pm_steps(nyxpm_ctx* c, int nsteps, double dt, double a_old, double a_n
     } suspend{c, c->inc_suspended};
     if (use_graph) {
         if (c->moves_pending && (rc = phase_sort(c))) return rc;
+        if ((rc = ensure_guess_buffer(c))) return rc;
         c->inc_suspended = true;
     }
     if (fused && nsteps > 0 && (rc = phase_kick<kKickDrift>(c, dt, a_old, a_half))) return rc;

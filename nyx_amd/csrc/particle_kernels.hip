@@ -1407,6 +1407,17 @@ __global__ void __launch_bounds__(kBlock) k_fill_int(int* a, long long n, int v)
     if (q < n) a[q] = v;
 }
 
+// Initial guess of a step's solve: *ok ? phi = 2 phi - prev : phi unchanged;
+// prev = the old phi either way.
+__global__ void __launch_bounds__(kBlock) k_phi_guess(double* __restrict__ phi, double* __restrict__ prev,
+                                                      long long n, const int* ok) {
+    const long long q = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (q >= n) return;
+    const double p = phi[q];
+    if (*ok) phi[q] = 2.0 * p - prev[q];
+    prev[q] = p;
+}
+
 __global__ void __launch_bounds__(kBlock) k_fill_dbl(double* a, long long n, double v) {
     const long long q = (long long)blockIdx.x * kBlock + threadIdx.x;
     if (q < n) a[q] = v;
