@@ -293,6 +293,134 @@ __global__ void __launch_bounds__(kBlock) k_resid_restrict(const CellSeg* segs, 
     r[sg.coff + q] = T(0.125) * acc;
 }
 
+// The same restriction marching up the coarse planes: a block owns a CX x CY tile
+// of coarse columns (2CX x 2CY fine) and a chunk of coarse planes, with a ring of
+// the 4 fine e planes 2K-1 .. 2K+2 (halo 1 in x and y) and the 2 fine r planes of
+// coarse plane K in LDS, the next planes fetched into registers during the
+// compute.  Each fine value is read once per block (plus the x-y halo), where the
+// thread-per-cell kernel fetched the e planes between coarse planes twice.  Same
+// arithmetic (lap7's, children summed k, j, i) bit for bit.  Needs nx % 2CX ==
+// ny % 2CY == nz % 2 == 0.
+template <typename T, int CX, int CY>
+__global__ void __launch_bounds__(CX * CY) k_resid_restrict_z(const CellSeg* segs, int nseg, const int* active,
+                                                              const T* __restrict__ e, T* __restrict__ r) {
+    constexpr int NT = CX * CY;
+    constexpr int FX = 2 * CX + 2, FY = 2 * CY + 2;  // fine e plane with halo 1
+    constexpr int RX = 2 * CX, RY = 2 * CY;          // fine r plane
+    constexpr int NEP = (FX * FY + NT - 1) / NT, NRP = (RX * RY) / NT;
+    static_assert((RX * RY) % NT == 0, "r plane split");
+    __shared__ T se[4][FY][FX];
+    __shared__ T sr[2][RY][RX];
+    const int bid = xcd_block(blockIdx.x, gridDim.x);
+    const int s = find_seg(segs, nseg, bid);
+    const CellSeg& sg = segs[s];
+    if (!active[sg.box]) return;
+    const int nx = sg.n[0], ny = sg.n[1], nz = sg.n[2];
+    const int cx = nx >> 1, cy = ny >> 1, cz = nz >> 1;
+    const int ntx = nx / RX, nty = ny / RY;
+    const int b = bid - sg.blk0;
+    const int tx = b % ntx, ty = (b / ntx) % nty, tz = b / (ntx * nty);
+    const int x0 = tx * RX, y0 = ty * RY;
+    const int K0 = tz * sg.zchunk, K1 = min(cz, K0 + sg.zchunk);
+    const T dhx = T(sg.dh[0]), dhy = T(sg.dh[1]), dhz = T(sg.dh[2]);
+    const T* E = e + sg.off;
+    const T* R = r + sg.off;
+    const int tid = threadIdx.x;
+    const size_t plane = (size_t)nx * ny;
+    size_t eoff[NEP], roff[NRP];
+#pragma unroll
+    for (int u = 0; u < NEP; ++u) {
+        const int idx = min(tid + u * NT, FX * FY - 1);
+        const int ly = idx / FX, lx = idx - ly * FX;
+        int gx = x0 + lx - 1, gy = y0 + ly - 1;
+        gx = gx < 0 ? gx + nx : (gx >= nx ? gx - nx : gx);
+        gy = gy < 0 ? gy + ny : (gy >= ny ? gy - ny : gy);
+        eoff[u] = (size_t)gy * nx + gx;
+    }
+#pragma unroll
+    for (int u = 0; u < NRP; ++u) {
+        const int idx = tid + u * NT;
+        const int ly = idx / RX, lx = idx - ly * RX;
+        roff[u] = (size_t)(y0 + ly) * nx + (x0 + lx);
+    }
+    auto wrapz = [&](int k) { return k < 0 ? k + nz : (k >= nz ? k - nz : k); };
+    auto fetch_e = [&](int kz, T* pe) {
+        const size_t zo = (size_t)wrapz(kz) * plane;
+#pragma unroll
+        for (int u = 0; u < NEP; ++u)
+            if (tid + u * NT < FX * FY) pe[u] = E[zo + eoff[u]];
+    };
+    auto put_e = [&](int kz, const T* pe) {
+        T* dst = &se[kz & 3][0][0];
+#pragma unroll
+        for (int u = 0; u < NEP; ++u)
+            if (tid + u * NT < FX * FY) dst[tid + u * NT] = pe[u];
+    };
+    auto fetch_r = [&](int kz, T* pr) {
+        const size_t zo = (size_t)kz * plane;
+#pragma unroll
+        for (int u = 0; u < NRP; ++u) pr[u] = R[zo + roff[u]];
+    };
+    auto put_r = [&](int kz, const T* pr) {
+        T* dst = &sr[kz & 1][0][0];
+#pragma unroll
+        for (int u = 0; u < NRP; ++u) dst[tid + u * NT] = pr[u];
+    };
+    T pa[NEP], pb[NEP], qa[NRP], qb[NRP];
+    fetch_e(2 * K0 - 1, pa);
+    fetch_e(2 * K0, pb);
+    put_e(2 * K0 - 1, pa);
+    put_e(2 * K0, pb);
+    fetch_e(2 * K0 + 1, pa);
+    fetch_e(2 * K0 + 2, pb);
+    fetch_r(2 * K0, qa);
+    fetch_r(2 * K0 + 1, qb);
+    put_e(2 * K0 + 1, pa);
+    put_e(2 * K0 + 2, pb);
+    put_r(2 * K0, qa);
+    put_r(2 * K0 + 1, qb);
+    __syncthreads();
+    const int lxc = tid % CX, lyc = tid / CX;
+    T* Rc = r + sg.coff + ((size_t)(y0 >> 1) + lyc) * cx + (size_t)(x0 >> 1) + lxc;
+    for (int K = K0; K < K1; ++K) {
+        const bool more = K + 1 < K1;
+        if (more) {
+            fetch_e(2 * K + 3, pa);
+            fetch_e(2 * K + 4, pb);
+            fetch_r(2 * K + 2, qa);
+            fetch_r(2 * K + 3, qb);
+        }
+        T acc = T(0);
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk) {
+            const int k = 2 * K + kk;
+            const int sc = k & 3, sm = (k - 1) & 3, sp = (k + 1) & 3;
+#pragma unroll
+            for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+                for (int ii = 0; ii < 2; ++ii) {
+                    const int fy = 2 * lyc + jj + 1, fx = 2 * lxc + ii + 1;
+                    const T ec = se[sc][fy][fx];
+                    // lap7's arithmetic
+                    const T lx = dhx * (se[sc][fy][fx - 1] - T(2) * ec + se[sc][fy][fx + 1]);
+                    const T ly = dhy * (se[sc][fy - 1][fx] - T(2) * ec + se[sc][fy + 1][fx]);
+                    const T lz = dhz * (se[sm][fy][fx] - T(2) * ec + se[sp][fy][fx]);
+                    const T rc = sr[k & 1][fy - 1][fx - 1] - (lx + ly + lz);
+                    acc += rc;
+                }
+        }
+        Rc[(size_t)K * cy * cx] = T(0.125) * acc;
+        __syncthreads();
+        if (more) {
+            put_e(2 * K + 3, pa);
+            put_e(2 * K + 4, pb);
+            put_r(2 * K + 2, qa);
+            put_r(2 * K + 3, qb);
+            __syncthreads();
+        }
+    }
+}
+
 // e_fine += e_coarse(parent): piecewise-constant prolongation (ec: the coarse
 // level's buffer, addressed at sg.coff).
 template <typename T>

@@ -219,6 +219,9 @@ struct nyxpm_ctx {
         int first = 0, count = 0, blocks = 0;
     };
     std::vector<Table> t_full, t_half, t_coarse;  // per depth (non-bottom boxes)
+    // residual + restriction per precision and depth: boxes the z-marching kernel
+    // takes (t_rrz) and the rest (t_rrc, thread per coarse cell)
+    std::vector<Table> t_rrz[2], t_rrc[2];
     // fused-sweep column shape and launch table per precision (0: FP64, 1: FP32)
     // and depth; tx == 0 means separate half sweeps at that depth
     struct SweepCfg {
@@ -445,6 +448,11 @@ void sweep_seg_shape(nyxpm_ctx* c, int pr, int d, const Level& L, CellSeg& w) {
     w.nblk = (int)(cols * nzc);
 }
 
+// Coarse tile of k_resid_restrict_z: kRrCX x CY coarse columns, CY by precision
+// (pr 0: FP64, 1: FP32) so that the LDS ring stays near 30 KB.
+constexpr int kRrCX = 32, kRrCY64 = 4, kRrCY32 = 8;
+inline int rrz_cy(int pr) { return pr == 0 ? kRrCY64 : kRrCY32; }
+
 int build_tables(nyxpm_ctx* c) {
     std::vector<CellSeg>& S = c->seg_host;
     S.clear();
@@ -485,6 +493,10 @@ int build_tables(nyxpm_ctx* c) {
     c->t_full.assign(c->dmax, {});
     c->t_half.assign(c->dmax, {});
     c->t_coarse.assign(c->dmax, {});
+    for (int pr = 0; pr < 2; ++pr) {
+        c->t_rrz[pr].assign(c->dmax, {});
+        c->t_rrc[pr].assign(c->dmax, {});
+    }
     c->pre_buf.assign(c->dmax, 0);
     c->fin_buf.assign(c->dmax, 0);
     static const int shapes64[][2] = {{64, 16}, {64, 8}, {32, 8}, {16, 8}};
@@ -556,7 +568,7 @@ int build_tables(nyxpm_ctx* c) {
         return (d + 1 == B.dc) ? 0 : c->fin_buf[d + 1];
     };
     for (int d = 0; d + 1 < c->dmax; ++d) {
-        std::vector<CellSeg> vf, vh, vc, vs[2];
+        std::vector<CellSeg> vf, vh, vc, vs[2], vrz[2], vrc[2];
         for (int b = 0; b < nbox; ++b) {
             const Box& B = c->boxes[b];
             if (d >= B.dc) continue;
@@ -566,6 +578,28 @@ int build_tables(nyxpm_ctx* c) {
             vf.push_back(f);
             vh.push_back(mk(b, d, L.nc / 2));
             vc.push_back(mk(b, d, L.nc / 8));
+            for (int pr = 0; pr < 2; ++pr) {
+                // the z-marching kernel where the tile divides the level and the
+                // level offers at least one block per CU (chunks of 16 coarse
+                // planes, halved down to 2 while fewer than 1024 blocks)
+                const int cy = rrz_cy(pr);
+                long long cols = 0;
+                int cz = 0, zc = 16;
+                if (L.n[0] % (2 * kRrCX) == 0 && L.n[1] % (2 * cy) == 0 && L.n[2] % 2 == 0) {
+                    cols = (long long)(L.n[0] / (2 * kRrCX)) * (L.n[1] / (2 * cy));
+                    cz = L.n[2] / 2;
+                    zc = std::min(zc, cz);
+                    while (zc > 2 && cols * ((cz + zc - 1) / zc) < 1024) zc /= 2;
+                }
+                if (cols > 0 && cols * ((cz + zc - 1) / zc) >= 256) {
+                    CellSeg z = mk(b, d, 1);
+                    z.zchunk = zc;
+                    z.nblk = (int)(cols * ((cz + zc - 1) / zc));
+                    vrz[pr].push_back(z);
+                } else {
+                    vrc[pr].push_back(mk(b, d, L.nc / 8));
+                }
+            }
             for (int pr = 0; pr < 2; ++pr) {
                 if (!c->sw[pr][d].tx) continue;
                 CellSeg w = mk(b, d, 1);
@@ -577,7 +611,11 @@ int build_tables(nyxpm_ctx* c) {
         c->t_full[d] = add_table(vf);
         c->t_half[d] = add_table(vh);
         c->t_coarse[d] = add_table(vc);
-        for (int pr = 0; pr < 2; ++pr) c->sw[pr][d].t = add_table(vs[pr]);
+        for (int pr = 0; pr < 2; ++pr) {
+            c->sw[pr][d].t = add_table(vs[pr]);
+            c->t_rrz[pr][d] = add_table(vrz[pr]);
+            c->t_rrc[pr][d] = add_table(vrc[pr]);
+        }
     }
     {
         std::vector<CellSeg> vb;
@@ -1081,7 +1119,6 @@ int vcycle(nyxpm_ctx* c, bool* updated) {
     for (int d = 0; d + 1 < D; ++d) {
         auto& tf = c->t_full[d];
         auto& th = c->t_half[d];
-        auto& tc = c->t_coarse[d];
         if (tf.count == 0) continue;
         int cur = 0;
         if (c->sw[pr][d].tx) {
@@ -1109,8 +1146,15 @@ int vcycle(nyxpm_ctx* c, bool* updated) {
         }
         {
             Timed t(c, lvl_name("resid_restrict", d));
-            hipLaunchKernelGGL(k_resid_restrict<T>, dim3(tc.blocks), dim3(kBlock), 0, c->stream, c->segs(tc),
-                               tc.count, c->active, eb[cur], r);
+            const auto& tz = c->t_rrz[pr][d];
+            const auto& tr = c->t_rrc[pr][d];
+            constexpr int CY = sizeof(T) == 8 ? kRrCY64 : kRrCY32;
+            if (tz.count)
+                hipLaunchKernelGGL((k_resid_restrict_z<T, kRrCX, CY>), dim3(tz.blocks), dim3(kRrCX * CY), 0,
+                                   c->stream, c->segs(tz), tz.count, c->active, eb[cur], r);
+            if (tr.count)
+                hipLaunchKernelGGL(k_resid_restrict<T>, dim3(tr.blocks), dim3(kBlock), 0, c->stream, c->segs(tr),
+                                   tr.count, c->active, eb[cur], r);
         }
     }
     if (c->per_level_coarse) {
