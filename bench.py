@@ -64,6 +64,7 @@ def kernel_bytes(name, ncell, npart, halo_cells, mixed_sz):
         "prolong.d0": (2 * sz + sz / 8) * N,               # read/write e, read coarse e
         "fill.d0": sz * N,
         "residual": (16 + sz) * N,                         # read phi, f (FP64); write r
+        "res_sweep0": (16 + 2 * sz) * N,                   # residual + first sweep: write r and e
         "update": (16 + sz) * N,                           # read/write phi, read e
         "deposit": 24 * P + 8 * N + 8 * halo_cells,        # read positions; write own + halos
         "fold": 8 * N + 8 * halo_cells + 8 * N,            # read own + halos; write f (rho kept in own)
@@ -222,7 +223,7 @@ def main():
     modes = a.modes.split(",")
 
     names = ["residual", "norm", "update", "deposit", "fold", "rhs", "kick_drift", "kick_pair", "kick", "scatter",
-             "scan", "coarse", "migrate",
+             "scan", "coarse", "migrate", "res_sweep0", "guess",
              "items", "keys", "bottom"] + \
             [f"{k}.d{d}" for k in ("sweep0", "sweep", "sweep_p", "sweep_u", "smooth", "resid_restrict", "prolong",
                                     "fill")

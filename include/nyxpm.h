@@ -112,6 +112,10 @@ typedef struct nyxpm_opts {
                           (one extra pass over phi; saves a V-cycle where the
                           previous guess sits just above a cycle boundary).  The
                           stopping rule is the same.  Not used with nyx_faithful. */
+    int residual;      /* 0: the level-0 residual is a pass of its own (default);
+                          1: it is formed inside the first sweep of the next
+                          V-cycle where the fused sweep runs (same results; one
+                          pass fewer but half the occupancy, slower at 512^3) */
 } nyxpm_opts;
 
 /* ---- lifecycle ---------------------------------------------------------- */

@@ -90,19 +90,36 @@ __global__ void __launch_bounds__(kBlock) k_gsrb_half(const CellSeg* segs, int n
 //   kSweepProlong the input is ein + (coarse correction of the parent cell), the
 //                 piecewise-constant prolongation of k_prolong done on load;
 //   kSweepUpdate  the result is not stored: phi += result (level 0, FP64 phi at
-//                 sg.foff), the update of k_update0 done on store.
+//                 sg.foff), the update of k_update0 done on store;
+//   kSweepResidual (with kSweepZero, level 0) r is not read but formed: the
+//                 FP64 residual f - L phi of k_residual0 (same arithmetic, the
+//                 rhs offset and scaling applied as there), computed per plane on
+//                 the halo-1 region from phi planes kept in registers, stored to r
+//                 for the block's own cells, max |r| (and max |f|) folded into the
+//                 norm slots.  One pass replaces k_residual0 + the first sweep.
 // Needs nx % TX == 0, ny % TY == 0 and even nx, ny, nz.  HBM bytes per cell:
 // e read + r read + e write (3*sizeof(T)); halo re-reads are served by L2.
-constexpr int kSweepZero = 1, kSweepProlong = 2, kSweepUpdate = 4;
+constexpr int kSweepZero = 1, kSweepProlong = 2, kSweepUpdate = 4, kSweepResidual = 8;
+
+// Inputs of the residual formed inside the first sweep (kSweepResidual).
+struct SweepRes {
+    const double* f;           // level-0 rhs (at sg.foff)
+    const double* solv;        // non-null: f holds rho - mass_offset (k_residual0's raw form)
+    double G, inv_a;
+    unsigned long long* nr;    // norm slots of max |r|
+    unsigned long long* nf;    // and of max |f| (nullptr: not wanted)
+};
 
 template <typename T, int TX, int TY, int MODE>
 __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, int nseg, const int* active,
                                                           const T* __restrict__ ein, T* __restrict__ eout,
                                                           const T* __restrict__ r, const T* ec0, const T* ec1,
-                                                          double* phi) {
+                                                          double* phi, SweepRes res) {
     constexpr bool ZERO_IN = (MODE & kSweepZero) != 0;
     constexpr bool PRO = (MODE & kSweepProlong) != 0;
     constexpr bool UPD = (MODE & kSweepUpdate) != 0;
+    constexpr bool RES = (MODE & kSweepResidual) != 0;
+    static_assert(!RES || ZERO_IN, "the residual is formed only in the first sweep");
     constexpr int NT = TX * TY / 2;
     constexpr int EX = TX + 4, EY = TY + 4;   // e ring planes: halo 2
     constexpr int RX = TX + 2, RY = TY + 2;   // r planes and red-update region: halo 1
@@ -148,6 +165,10 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
         eoff[u] = (size_t)gy * nx + gx;
         if (PRO) coff_[u] = (size_t)(gy >> 1) * cnx + (gx >> 1);
     }
+    // RES: in-plane position of each r element and whether it is one of the
+    // block's own cells (stored, counted in the norms)
+    int rgx[RES ? NR : 1], rgy[RES ? NR : 1];
+    bool rown[RES ? NR : 1];
 #pragma unroll
     for (int u = 0; u < NR; ++u) {
         const int idx = min(tid + u * NT, RX * RY - 1);
@@ -156,8 +177,28 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
         gx = gx < 0 ? gx + nx : (gx >= nx ? gx - nx : gx);
         gy = gy < 0 ? gy + ny : (gy >= ny ? gy - ny : gy);
         roff[u] = (size_t)gy * nx + gx;
+        if (RES) {
+            rgx[u] = gx;
+            rgy[u] = gy;
+            rown[u] = tid + u * NT < RX * RY && lx >= 1 && lx <= TX && ly >= 1 && ly <= TY;
+        }
     }
     auto wrapz = [&](int k) { return k < 0 ? k + nz : (k >= nz ? k - nz : k); };
+    // RES: phi of each r element at planes k-1 and k of the next plane k to form
+    const double* Pf = phi + sg.foff;
+    const double* Ff = RES ? res.f + sg.foff : nullptr;
+    const bool raw = RES && res.solv != nullptr;
+    const double rns = raw ? -res.solv[sg.box] : 0.0;
+    const double rdx = sg.dh[0], rdy = sg.dh[1], rdz = sg.dh[2];
+    double wm[RES ? NR : 1], wc[RES ? NR : 1];
+    double ar = 0.0, af = 0.0;
+    if (RES) {
+#pragma unroll
+        for (int u = 0; u < NR; ++u) {
+            wm[u] = Pf[(size_t)wrapz(z0 - 2) * plane + roff[u]];
+            wc[u] = Pf[(size_t)wrapz(z0 - 1) * plane + roff[u]];
+        }
+    }
     auto fetch_e = [&](int kz) {
         const int kw = wrapz(kz);
         const bool new_coarse = !(kz & 1) || kz == z0 - 2;
@@ -188,6 +229,39 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
     };
     auto fetch_r = [&](int kz) {
         const int kw = wrapz(kz);
+        if (RES) {
+            // k_residual0's arithmetic on the planes kz-1, kz, kz+1 (called for
+            // kz = z0-1, z0, ..., z1 in order)
+            const size_t zo = (size_t)kw * plane, zp = (size_t)wrapz(kz + 1) * plane;
+            const bool own_plane = kz >= z0 && kz < z1;
+#pragma unroll
+            for (int u = 0; u < NR; ++u) {
+                const int gx = rgx[u], gy = rgy[u];
+                const int xm = gx == 0 ? nx - 1 : gx - 1, xp = gx == nx - 1 ? 0 : gx + 1;
+                const int ym = gy == 0 ? ny - 1 : gy - 1, yp = gy == ny - 1 ? 0 : gy + 1;
+                const size_t row = zo + (size_t)gy * nx;
+                const double pm = wm[u], pc = wc[u];
+                const double pp = Pf[zp + roff[u]];
+                const double lx = rdx * (Pf[row + xm] - 2.0 * pc + Pf[row + xp]);
+                const double ly = rdy * (Pf[zo + (size_t)ym * nx + gx] - 2.0 * pc + Pf[zo + (size_t)yp * nx + gx]);
+                const double lz = rdz * (pm - 2.0 * pc + pp);
+                double fv = Ff[zo + roff[u]];
+                if (raw) {
+                    const double t = fv + rns;
+                    fv = (t * res.G) * res.inv_a;
+                }
+                const double v = fv - (lx + ly + lz);
+                pr[u] = T(v);
+                if (own_plane && rown[u]) {
+                    const_cast<T*>(R)[zo + roff[u]] = T(v);
+                    ar = fmax(ar, fabs(v));
+                    af = fmax(af, fabs(fv));
+                }
+                wm[u] = pc;
+                wc[u] = pp;
+            }
+            return;
+        }
 #pragma unroll
         for (int u = 0; u < NR; ++u) {
             const int idx = tid + u * NT;
@@ -261,6 +335,29 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
             }
         }
         __syncthreads();
+    }
+    if (RES) {
+        // k_residual0's norm slots: max over the block, atomicMax on the bit pattern
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            ar = fmax(ar, __shfl_xor(ar, o));
+            af = fmax(af, __shfl_xor(af, o));
+        }
+        __shared__ double wr[NT / kWave], wf[NT / kWave];
+        if ((tid & (kWave - 1)) == 0) {
+            wr[tid / kWave] = ar;
+            wf[tid / kWave] = af;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double mr = wr[0], mf = wf[0];
+            for (int w = 1; w < NT / kWave; ++w) {
+                mr = fmax(mr, wr[w]);
+                mf = fmax(mf, wf[w]);
+            }
+            const int slot = sg.box * kNormSlots + (bid & (kNormSlots - 1));
+            atomicMax(&res.nr[slot], dbits(mr));
+            if (res.nf) atomicMax(&res.nf[slot], dbits(mf));
+        }
     }
 }
 

@@ -130,6 +130,7 @@ struct nyxpm_ctx {
     bool untiled_kick = false;  // force the thread-per-particle kick (for comparisons)
     bool separate_kicks = false;  // nyxpm_pm_steps without the kick / kick-drift pairing
     bool per_level_coarse = false;  // coarse levels by per-level launches, not k_coarse_vcycle
+    bool fuse_res = false;          // level-0 residual formed inside the first sweep
     int sweep_zchunk = 0;       // fixed z-chunk of the fused sweep (0: chosen per level)
     int force_tx[2] = {0, 0}, force_ty[2] = {8, 8};  // fixed sweep shape per precision (0: chosen)
     bool fuse_update = true;    // phi += e in the last finest-level sweep
@@ -1086,14 +1087,15 @@ struct Pools<float> {
 };
 
 template <typename T, int MODE>
-void launch_sweep(nyxpm_ctx* c, int d, const T* ein, T* eout, const T* r, const T* ec0, const T* ec1) {
+void launch_sweep(nyxpm_ctx* c, int d, const T* ein, T* eout, const T* r, const T* ec0, const T* ec1,
+                  SweepRes res = {}) {
     const auto& cfg = c->sw[sizeof(T) == 8 ? 0 : 1][d];
     const auto& t = cfg.t;
     const int key = cfg.tx * 100 + cfg.ty;
 #define NYXPM_SWEEP(TX, TY)                                                                                  \
     case TX * 100 + TY:                                                                                      \
         hipLaunchKernelGGL((k_rb_sweep<T, TX, TY, MODE>), dim3(t.blocks), dim3(TX * TY / 2), 0, c->stream, \
-                           c->segs(t), t.count, c->active, ein, eout, r, ec0, ec1, c->phi);                 \
+                           c->segs(t), t.count, c->active, ein, eout, r, ec0, ec1, c->phi, res);            \
         break;
     switch (key) {
         NYXPM_SWEEP(128, 8)
@@ -1108,9 +1110,10 @@ void launch_sweep(nyxpm_ctx* c, int d, const T* ein, T* eout, const T* r, const 
 }
 
 // One V-cycle on every active box.  Returns in *updated whether phi += e has
-// already been applied (fused into the last finest-level sweep).
+// already been applied (fused into the last finest-level sweep).  With
+// sweep0_done the first level-0 sweep was already run (by residual_sweep0).
 template <typename T>
-int vcycle(nyxpm_ctx* c, bool* updated) {
+int vcycle(nyxpm_ctx* c, bool* updated, bool sweep0_done = false) {
     T* r = Pools<T>::r(c);
     T* eb[2] = {Pools<T>::e(c), Pools<T>::e2(c)};
     const int pr = sizeof(T) == 8 ? 0 : 1;
@@ -1122,7 +1125,7 @@ int vcycle(nyxpm_ctx* c, bool* updated) {
         if (tf.count == 0) continue;
         int cur = 0;
         if (c->sw[pr][d].tx) {
-            {
+            if (!(sweep0_done && d == 0)) {
                 Timed t(c, lvl_name("sweep0", d));
                 launch_sweep<T, kSweepZero>(c, d, eb[0], eb[0], r, eb[0], eb[1]);
             }
@@ -1215,6 +1218,28 @@ int vcycle(nyxpm_ctx* c, bool* updated) {
     return NYXPM_OK;
 }
 
+// Whether the level-0 residual is formed inside the first sweep of the V-cycle:
+// every box's level 0 has a fused sweep (no box runs its whole V-cycle in the
+// coarse workgroup) and fused sweeps are not switched off.
+template <typename T>
+bool residual_in_sweep(const nyxpm_ctx* c) {
+    const int pr = sizeof(T) == 8 ? 0 : 1;
+    return c->fuse_res && c->fuse_update && !c->no_fused && c->dmax > 1 && c->sw[pr][0].tx != 0;
+}
+
+// r = f - L phi at level 0 with its norms, and the first sweep of the next
+// V-cycle from it (k_rb_sweep with kSweepResidual): k_residual0 + sweep0 in one pass.
+template <typename T>
+int residual_sweep0(nyxpm_ctx* c, bool with_f) {
+    Timed t(c, "res_sweep0");
+    T* r = Pools<T>::r(c);
+    T* e0 = Pools<T>::e(c);
+    T* e1 = Pools<T>::e2(c);
+    SweepRes res{c->f, c->f_raw ? c->solv : nullptr, c->f_G, c->f_inv_a, c->nr, with_f ? c->nf : nullptr};
+    launch_sweep<T, kSweepZero | kSweepResidual>(c, 0, e0, e0, r, e0, e1, res);
+    return NYXPM_OK;
+}
+
 template <typename T>
 int residual(nyxpm_ctx* c, bool with_f) {
     Timed t(c, "residual");
@@ -1257,15 +1282,19 @@ template <typename T>
 int solve_impl(nyxpm_ctx* c, double rel, double abs_tol, int fixed_iters, int graph_cycles) {
     const int nbox = (int)c->boxes.size();
     int rc;
+    // with the residual formed in the first sweep, every residual also starts the
+    // next V-cycle (wasted only after the last one)
+    const bool rs = residual_in_sweep<T>(c);
+    auto res = [&](bool with_f) { return rs ? residual_sweep0<T>(c, with_f) : residual<T>(c, with_f); };
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(kBlock), 0, c->stream, c->active, (long long)nbox, 1);
-    if ((rc = residual<T>(c, true))) return rc;
+    if ((rc = res(true))) return rc;
     if ((rc = check(c, true, fixed_iters, rel, abs_tol))) return rc;
     if (graph_cycles > 0) {
         for (int it = 0; it < graph_cycles; ++it) {
             bool upd;
-            if ((rc = vcycle<T>(c, &upd))) return rc;
+            if ((rc = vcycle<T>(c, &upd, rs))) return rc;
             if (!upd && (rc = update<T>(c))) return rc;
-            if ((rc = residual<T>(c, false))) return rc;
+            if ((rc = res(false))) return rc;
             if ((rc = check(c, false, fixed_iters, rel, abs_tol))) return rc;
         }
         return NYXPM_OK;
@@ -1276,9 +1305,9 @@ int solve_impl(nyxpm_ctx* c, double rel, double abs_tol, int fixed_iters, int gr
         if ((rc = read_flags(c, &any))) return rc;
         if (!any) break;
         bool upd;
-        if ((rc = vcycle<T>(c, &upd))) return rc;
+        if ((rc = vcycle<T>(c, &upd, rs))) return rc;
         if (!upd && (rc = update<T>(c))) return rc;
-        if ((rc = residual<T>(c, false))) return rc;
+        if ((rc = res(false))) return rc;
         if ((rc = check(c, false, fixed_iters, rel, abs_tol))) return rc;
     }
     return NYXPM_OK;
@@ -1525,6 +1554,7 @@ int nyxpm_create_batch(nyxpm_ctx** out, int nbox, const nyxpm_geom* geoms, const
         if (opts->sweep_zchunk > 0) c->sweep_zchunk = opts->sweep_zchunk;
         if (opts->kick_chunk > 0) c->kick_chunk = std::max(kBlock, opts->kick_chunk);
         if (opts->phi_guess >= 0 && opts->phi_guess <= 2) c->phi_guess = opts->phi_guess;
+        c->fuse_res = opts->residual == 1;
         if (opts->resort_slack < 0) {
             c->slack_num = 0;
             c->slack_pad = 0;

@@ -59,7 +59,8 @@ class PMContext:
 
     def __init__(self, geoms, precision="fp64", tile=None, nu1=2, nu2=2, max_iter=200, smoother="fused",
                  kick="tiled", sweep_shape=None, sweep_zchunk=0, kick_chunk=0, pair_kicks=True,
-                 coarse="workgroup", particle_cfl=0.0, resort_slack=0.0, phi_guess="default"):
+                 coarse="workgroup", particle_cfl=0.0, resort_slack=0.0, phi_guess="default",
+                 residual="separate"):
         """resort_slack: free slots per particle each tile gets at a full sort, for
         the incremental re-sort after kick-drifts (0: library default 0.25;
         negative: a full sort after every kick-drift).  phi_guess: initial guess of
@@ -89,6 +90,7 @@ class PMContext:
         o.particle_cfl = float(particle_cfl)
         o.resort_slack = float(resort_slack)
         o.phi_guess = {"default": 0, "previous": 1, "extrapolate": 2}[phi_guess]
+        o.residual = 1 if residual == "in_sweep" else 0
         if tile is not None:
             for d in range(3):
                 o.tile[d] = int(tile[d])

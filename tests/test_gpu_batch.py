@@ -212,3 +212,28 @@ def test_est_time_step_fused_equals_standalone(nyx, precision, use_graph):
         for d in range(3):
             assert np.array_equal(fp[b][0][d], sp[b][0][d])
             assert np.array_equal(fp[b][1][d], sp[b][1][d])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_residual_in_first_sweep_equals_separate_pass(nyx, precision, use_graph):
+    # the level-0 residual formed inside the first sweep of the next V-cycle (with
+    # its norms) gives the same bits as k_residual0 followed by that sweep
+    geoms = [nyx.BoxGeom((32, 32, 32)), nyx.BoxGeom((64, 64, 64)), nyx.BoxGeom((96, 96, 96))]
+    inputs = [_inputs(nyx, g.n_cell, 700 + i, mass=(i == 1)) for i, g in enumerate(geoms)]
+    outs = []
+    for residual in ("separate", "in_sweep"):
+        with nyx.PMContext(geoms, precision=precision, residual=residual) as c:
+            for b, inp in enumerate(inputs):
+                _upload(c, b, inp)
+            c.pm_init(1.0, 1.0, 1e-10)
+            its = c.pm_steps(5, 1e-3, rel_tol=1e-10, use_graph=use_graph)
+            outs.append(([c.download(b) for b in range(len(geoms))], [c.get_phi(b) for b in range(len(geoms))],
+                         list(its)))
+    (a, aphi, ait), (b_, bphi, bit) = outs
+    assert ait == bit
+    for b in range(len(geoms)):
+        assert np.array_equal(aphi[b], bphi[b])
+        for d in range(3):
+            assert np.array_equal(a[b][0][d], b_[b][0][d])
+            assert np.array_equal(a[b][1][d], b_[b][1][d])
