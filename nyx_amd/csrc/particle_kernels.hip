@@ -287,11 +287,25 @@ __global__ void __launch_bounds__(kBlock) k_deposit(const BoxSeg* segs, int nseg
     const int bx = min(B.tile[0], B.n[0] - o0), by = min(B.tile[1], B.n[1] - o1), bz = min(B.tile[2], B.n[2] - o2);
     const int Lx = bx + 1, Ly = by + 1, Lz = bz + 1;
     const int LC = Lx * Ly * Lz;
-    for (int c = threadIdx.x; c < 2 * LC; c += kBlock) acc[c] = 0ull;
-    __syncthreads();
-
     const long long gt = B.tile_off + lt;
     const long long p0 = tbase[gt], p1 = p0 + thw[gt];
+    // the thread's first two particles are loaded before the accumulator is
+    // cleared, so their latency overlaps the clearing
+    double xa = 0.0, ya = 0.0, za = 0.0, ma = 0.0, xb = 0.0, yb = 0.0, zb = 0.0, mb = 0.0;
+    auto load2 = [&](long long pa) {
+        const long long pb = pa + kBlock;
+        if (pa < p1) {
+            xa = x[pa]; ya = y[pa]; za = z[pa];
+            ma = B.has_mass ? m[pa] : B.m_const;
+        }
+        if (pb < p1) {
+            xb = x[pb]; yb = y[pb]; zb = z[pb];
+            mb = B.has_mass ? m[pb] : B.m_const;
+        }
+    };
+    load2(p0 + threadIdx.x);
+    for (int c = threadIdx.x; c < 2 * LC; c += kBlock) acc[c] = 0ull;
+    __syncthreads();
     const int E = B.fx_exp;
     // one particle's CIC contributions as two fixed-point words: w * 2^(31-E) =
     // hi + lo * 2^-32 (hi < 2^32 since w < 2^(E+1); lo rounded, a rounding up to
@@ -332,17 +346,9 @@ __global__ void __launch_bounds__(kBlock) k_deposit(const BoxSeg* segs, int nseg
     };
     // two particles per thread, both loaded before either is deposited
     for (long long pa = p0 + threadIdx.x; pa < p1; pa += 2 * kBlock) {
-        const long long pb = pa + kBlock;
-        const bool hb = pb < p1;
-        const double xa = x[pa], ya = y[pa], za = z[pa];
-        const double ma = B.has_mass ? m[pa] : B.m_const;
-        double xb = 0.0, yb = 0.0, zb = 0.0, mb = 0.0;
-        if (hb) {
-            xb = x[pb]; yb = y[pb]; zb = z[pb];
-            mb = B.has_mass ? m[pb] : B.m_const;
-        }
+        if (pa != p0 + threadIdx.x) load2(pa);
         dep(xa, ya, za, ma);
-        if (hb) dep(xb, yb, zb, mb);
+        if (pa + kBlock < p1) dep(xb, yb, zb, mb);
     }
     __syncthreads();
     for (int c = threadIdx.x; c < LC; c += kBlock) {
