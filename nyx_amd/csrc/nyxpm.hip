@@ -189,6 +189,9 @@ struct nyxpm_ctx {
     int scan_blocks = 0;
     TileItem* items = nullptr;  // tile chunks of the last sort (tile-staged kernels)
     int* nitems = nullptr;
+    int* heavy = nullptr;       // tiles of more than kHeavyTile slots (deposit order)
+    int* nheavy = nullptr;
+    int heavy_cap = 0;
     int* ndefer = nullptr;  // entries of the list handed from the tile-staged kick to k_kick_deferred
     // particle time step (estTimestep): per-box minimum as bits, kept by the closing
     // kicks when cfl > 0; valid for the state after the last such kick (a = dt_a)
@@ -784,6 +787,8 @@ int alloc_fields(nyxpm_ctx* c) {
     if ((rc = dalloc(c, &c->failed, nbox))) return rc;
     if ((rc = dalloc(c, &c->err_flag, 1))) return rc;
     if ((rc = dalloc(c, &c->nitems, 1))) return rc;
+    if ((rc = dalloc(c, &c->nheavy, 1))) return rc;
+    HIPCK(c, hipMemset(c->nheavy, 0, sizeof(int)));
     if ((rc = dalloc(c, &c->ndefer, 1))) return rc;
     if ((rc = dalloc(c, &c->dtmin, nbox))) return rc;
     if ((rc = dalloc(c, &c->nr, (size_t)nbox * kNormSlots))) return rc;
@@ -828,6 +833,7 @@ void free_particles(nyxpm_ctx* c) {
     }
     dfree(c->key);
     dfree(c->items);
+    dfree(c->heavy);
 }
 
 int alloc_particle_set(nyxpm_ctx* c, PartPtrs& P, long long n, bool mass) {
@@ -890,8 +896,9 @@ int phase_keys(nyxpm_ctx* c) {
 int make_items(nyxpm_ctx* c) {
     Timed t(c, "items");
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(kBlock), 0, c->stream, c->nitems, 1LL, 0);
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(kBlock), 0, c->stream, c->nheavy, 1LL, 0);
     hipLaunchKernelGGL(k_make_items, dim3(nblocks(c->ntiles_total)), dim3(kBlock), 0, c->stream, c->tbase,
-                       c->cursor, c->ntiles_total, c->items, c->nitems, c->kick_chunk);
+                       c->cursor, c->ntiles_total, c->items, c->nitems, c->kick_chunk, c->heavy, c->nheavy);
     LAUNCHCK(c);
     c->items_valid = true;
     return NYXPM_OK;
@@ -985,8 +992,9 @@ int phase_deposit(nyxpm_ctx* c, bool want_rho) {
     const size_t lds = 2 * sizeof(unsigned long long) * (c->tile[0] + 1) * (c->tile[1] + 1) * (c->tile[2] + 1);
     {
         Timed t(c, "deposit");
-        hipLaunchKernelGGL(k_deposit, dim3(c->tseg_blocks), dim3(kBlock), lds, c->stream, c->tseg, c->tseg_n,
-                           c->dboxes, c->tbase, c->cursor, A.x, A.y, A.z, A.m, c->own, c->halo, c->err_flag);
+        hipLaunchKernelGGL(k_deposit, dim3(c->heavy_cap + c->tseg_blocks), dim3(kBlock), lds, c->stream, c->tseg,
+                           c->tseg_n, c->dboxes, c->tbase, c->cursor, c->heavy, c->nheavy, c->heavy_cap, A.x, A.y,
+                           A.z, A.m, c->own, c->halo, c->err_flag);
     }
     {
         Timed t(c, "fold");
@@ -1600,7 +1608,7 @@ int nyxpm_destroy(nyxpm_ctx* c) {
     dfree(c->r64); dfree(c->e64); dfree(c->e64b); dfree(c->G64);
     dfree(c->r32); dfree(c->e32); dfree(c->e32b); dfree(c->G32);
     dfree(c->active); dfree(c->iters); dfree(c->failed); dfree(c->err_flag); dfree(c->nitems); dfree(c->nr); dfree(c->nf);
-    dfree(c->ndefer); dfree(c->dtmin);
+    dfree(c->ndefer); dfree(c->dtmin); dfree(c->nheavy);
     dfree(c->state); dfree(c->solv); dfree(c->dv_size); dfree(c->dboxes);
     dfree(c->seg_pool); dfree(c->pseg); dfree(c->tseg); dfree(c->cc_first);
     if (c->h_flags) hipHostFree(c->h_flags);
@@ -1755,12 +1763,19 @@ static int upload_impl(nyxpm_ctx* c, int box, int64_t np, const double* x, const
     if (icap > 0x7fffffffLL) FAIL(c, NYXPM_ERR_ARG, "too many tile chunks");
     TileItem* nitm = nullptr;
     if ((rc = dalloc(c, &nitm, icap))) return rc;
+    // a tile above kHeavyTile slots holds more than kHeavyTile particles or holes
+    const long long hcap = std::min<long long>(c->ntiles_total, slots / kHeavyTile + 1);
+    int* nheavy_list = nullptr;
+    if ((rc = dalloc(c, &nheavy_list, hcap))) return rc;
     free_particles(c);
     c->P[0] = N[0];
     c->P[1] = N[1];
     c->key = nkey;
     c->items = nitm;
     c->items_cap = icap;
+    dfree(c->heavy);
+    c->heavy = nheavy_list;
+    c->heavy_cap = (int)hcap;
     c->items_valid = false;
     c->dt_valid = false;
     c->cur = 0;
@@ -1950,8 +1965,9 @@ int nyxpm_assign_density_and_vels(nyxpm_ctx* c, int box, double* out) {
         return rc;
     }
     const size_t lds = 2 * sizeof(unsigned long long) * (c->tile[0] + 1) * (c->tile[1] + 1) * (c->tile[2] + 1);
-    hipLaunchKernelGGL(k_deposit, dim3(c->tseg_blocks), dim3(kBlock), lds, c->stream, c->tseg, c->tseg_n, c->dboxes,
-                       c->tbase, c->cursor, A.x, A.y, A.z, A.m, c->own, c->halo, c->err_flag);
+    hipLaunchKernelGGL(k_deposit, dim3(c->heavy_cap + c->tseg_blocks), dim3(kBlock), lds, c->stream, c->tseg,
+                       c->tseg_n, c->dboxes, c->tbase, c->cursor, c->heavy, c->nheavy, c->heavy_cap, A.x, A.y, A.z,
+                       A.m, c->own, c->halo, c->err_flag);
     hipLaunchKernelGGL(k_fold_raw, dim3(c->t_res.blocks), dim3(kBlock), 0, c->stream, c->segs(c->t_res),
                        c->t_res.count, c->dboxes, c->own, c->halo, sums);
     hipLaunchKernelGGL(k_fill_dbl, dim3(1), dim3(kBlock), 0, c->stream, reinterpret_cast<double*>(maxbits),

@@ -79,6 +79,9 @@ __device__ __forceinline__ void wave_append(long long* list, int* n, long long v
 // x of an empty particle slot.
 __device__ __forceinline__ double empty_slot() { return __longlong_as_double(0x7ff8000000000000LL); }
 
+// Slots above which a tile's deposit block is launched ahead of the others.
+constexpr int kHeavyTile = 2048;
+
 // i / T for 0 <= i < 2^17 and T <= 32 with magic = ceil(2^31 / T) (exact over that
 // range; the bound is checked when a context is created).
 __device__ __forceinline__ int tile_div(int i, unsigned magic) {
@@ -275,13 +278,25 @@ __device__ __forceinline__ int halo_index(int lx, int ly, int lz, int bx, int by
 __global__ void __launch_bounds__(kBlock) k_deposit(const BoxSeg* segs, int nseg, const BoxDev* boxes,
                                                     const long long* __restrict__ tbase,
                                                     const int* __restrict__ thw,
+                                                    const int* __restrict__ heavy, const int* nheavy,
+                                                    int heavy_cap,
                                                     const double* __restrict__ x, const double* __restrict__ y,
                                                     const double* __restrict__ z, const double* __restrict__ m,
                                                     double* __restrict__ own, double* __restrict__ halo, int* err) {
     extern __shared__ unsigned long long acc[];
-    const BoxSeg sg = segs[find_seg(segs, nseg, blockIdx.x)];
+    // the first heavy_cap blocks take the heavy tiles (listed by k_make_items), so
+    // the longest blocks start first; the rest take every other tile in order
+    int tile;
+    if ((int)blockIdx.x < heavy_cap) {
+        if ((int)blockIdx.x >= *nheavy) return;
+        tile = heavy[blockIdx.x];
+    } else {
+        tile = blockIdx.x - heavy_cap;
+        if (thw[tile] > kHeavyTile) return;
+    }
+    const BoxSeg sg = segs[find_seg(segs, nseg, tile)];
     const BoxDev& B = boxes[sg.box];
-    const int lt = blockIdx.x - sg.blk0;
+    const int lt = tile - sg.blk0;
     const int tx = lt % B.ntile[0], ty = (lt / B.ntile[0]) % B.ntile[1], tz = lt / (B.ntile[0] * B.ntile[1]);
     const int o0 = tx * B.tile[0], o1 = ty * B.tile[1], o2 = tz * B.tile[2];
     const int bx = min(B.tile[0], B.n[0] - o0), by = min(B.tile[1], B.n[1] - o1), bz = min(B.tile[2], B.n[2] - o2);
@@ -812,7 +827,8 @@ struct TileItem {
 
 __global__ void __launch_bounds__(kBlock) k_make_items(const long long* __restrict__ tbase,
                                                        const int* __restrict__ thw, long long ntiles,
-                                                       TileItem* __restrict__ items, int* nitems, int chunk) {
+                                                       TileItem* __restrict__ items, int* nitems, int chunk,
+                                                       int* __restrict__ heavy, int* nheavy) {
     const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
     int nc = 0;
     long long p0 = 0, cnt = 0;
@@ -820,6 +836,8 @@ __global__ void __launch_bounds__(kBlock) k_make_items(const long long* __restri
         p0 = tbase[t];
         cnt = thw[t];
         nc = (int)((cnt + chunk - 1) / chunk);
+        // tiles the deposit starts first (one block per tile: the longest blocks)
+        if (cnt > kHeavyTile) heavy[atomicAdd(nheavy, 1)] = (int)t;
     }
     // wave-inclusive prefix of nc, one atomic per wave
     const int lane = threadIdx.x & (kWave - 1);
