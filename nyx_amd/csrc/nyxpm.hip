@@ -222,7 +222,7 @@ struct nyxpm_ctx {
     struct Table {
         int first = 0, count = 0, blocks = 0;
     };
-    std::vector<Table> t_full, t_half, t_coarse;  // per depth (non-bottom boxes)
+    std::vector<Table> t_full, t_half;  // per depth (non-bottom boxes)
     // residual + restriction per precision and depth: boxes the z-marching kernel
     // takes (t_rrz) and the rest (t_rrc, thread per coarse cell)
     std::vector<Table> t_rrz[2], t_rrc[2];
@@ -496,7 +496,6 @@ int build_tables(nyxpm_ctx* c) {
     // not depend on the column shape, so both precisions share it).
     c->t_full.assign(c->dmax, {});
     c->t_half.assign(c->dmax, {});
-    c->t_coarse.assign(c->dmax, {});
     for (int pr = 0; pr < 2; ++pr) {
         c->t_rrz[pr].assign(c->dmax, {});
         c->t_rrc[pr].assign(c->dmax, {});
@@ -572,7 +571,7 @@ int build_tables(nyxpm_ctx* c) {
         return (d + 1 == B.dc) ? 0 : c->fin_buf[d + 1];
     };
     for (int d = 0; d + 1 < c->dmax; ++d) {
-        std::vector<CellSeg> vf, vh, vc, vs[2], vrz[2], vrc[2];
+        std::vector<CellSeg> vf, vh, vs[2], vrz[2], vrc[2];
         for (int b = 0; b < nbox; ++b) {
             const Box& B = c->boxes[b];
             if (d >= B.dc) continue;
@@ -581,7 +580,6 @@ int build_tables(nyxpm_ctx* c) {
             f.ebuf = coarse_buf(B, d);
             vf.push_back(f);
             vh.push_back(mk(b, d, L.nc / 2));
-            vc.push_back(mk(b, d, L.nc / 8));
             for (int pr = 0; pr < 2; ++pr) {
                 // the z-marching kernel where the tile divides the level and the
                 // level offers at least one block per CU (chunks of 16 coarse
@@ -614,7 +612,6 @@ int build_tables(nyxpm_ctx* c) {
         }
         c->t_full[d] = add_table(vf);
         c->t_half[d] = add_table(vh);
-        c->t_coarse[d] = add_table(vc);
         for (int pr = 0; pr < 2; ++pr) {
             c->sw[pr][d].t = add_table(vs[pr]);
             c->t_rrz[pr][d] = add_table(vrz[pr]);
