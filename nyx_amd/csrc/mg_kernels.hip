@@ -124,8 +124,15 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
     constexpr int EX = TX + 4, EY = TY + 4;   // e ring planes: halo 2
     constexpr int RX = TX + 2, RY = TY + 2;   // r planes and red-update region: halo 1
     constexpr int NE = (EX * EY + NT - 1) / NT, NR = (RX * RY + NT - 1) / NT;
-    __shared__ T se[4][EY][EX];
-    __shared__ T sr[2][RY][RX];
+    // FP32: planes are moved as pairs of x-adjacent cells (8-byte loads and LDS
+    // stores, half the instructions).  Pairs start at even x (nx is even), so the
+    // r planes are kept with halo 2 in x as well (RSH shifts the r index).
+    constexpr bool V2 = sizeof(T) == 4 && !RES;
+    constexpr int RXL = V2 ? TX + 4 : RX, RSH = V2 ? 1 : 0;
+    constexpr int EXP = EX / 2, RXP = RXL / 2;
+    constexpr int NEP = (EXP * EY + NT - 1) / NT, NRP = (RXP * RY + NT - 1) / NT;
+    __shared__ __align__(16) T se[4][EY][EX];
+    __shared__ __align__(16) T sr[2][RY][RXL];
     const int bid = xcd_block(blockIdx.x, gridDim.x);
     const int s = find_seg(segs, nseg, bid);
     const CellSeg& sg = segs[s];
@@ -152,11 +159,35 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
     // in-plane offsets (periodic wrap included) are the same for every plane and
     // are computed once; the coarse values of the prolongation change every
     // second plane (z0 is even), so they are reloaded only when kz is even.
-    T pe[NE], pr[NR], pc[PRO ? NE : 1];
-    size_t eoff[NE], coff_[PRO ? NE : 1], roff[NR];
+    T pe[V2 ? 1 : NE], pr[V2 ? 1 : NR], pc[PRO && !V2 ? NE : 1];
+    size_t eoff[V2 ? 1 : NE], coff_[PRO && !V2 ? NE : 1], roff[V2 ? 1 : NR];
+    float2 pe2[V2 ? NEP : 1], pr2[V2 ? NRP : 1];
+    float pc2[V2 && PRO ? NEP : 1];
+    size_t eoff2[V2 ? NEP : 1], coff2[V2 && PRO ? NEP : 1], roff2[V2 ? NRP : 1];
     const size_t plane = (size_t)nx * ny, cplane = (size_t)cnx * cny;
+    if (V2) {
 #pragma unroll
-    for (int u = 0; u < NE; ++u) {
+        for (int u = 0; u < NEP; ++u) {
+            const int pidx = min(tid + u * NT, EXP * EY - 1);
+            const int ly = pidx / EXP, lxp = pidx - ly * EXP;
+            int gx = x0 + 2 * lxp - 2, gy = y0 + ly - 2;
+            gx = gx < 0 ? gx + nx : (gx >= nx ? gx - nx : gx);
+            gy = gy < 0 ? gy + ny : (gy >= ny ? gy - ny : gy);
+            eoff2[u] = (size_t)gy * nx + gx;
+            if (PRO) coff2[u] = (size_t)(gy >> 1) * cnx + (gx >> 1);
+        }
+#pragma unroll
+        for (int u = 0; u < NRP; ++u) {
+            const int pidx = min(tid + u * NT, RXP * RY - 1);
+            const int ly = pidx / RXP, lxp = pidx - ly * RXP;
+            int gx = x0 + 2 * lxp - 2, gy = y0 + ly - 1;
+            gx = gx < 0 ? gx + nx : (gx >= nx ? gx - nx : gx);
+            gy = gy < 0 ? gy + ny : (gy >= ny ? gy - ny : gy);
+            roff2[u] = (size_t)gy * nx + gx;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < (V2 ? 0 : NE); ++u) {
         const int idx = min(tid + u * NT, EX * EY - 1);
         const int ly = idx / EX, lx = idx - ly * EX;
         int gx = x0 + lx - 2, gy = y0 + ly - 2;
@@ -170,7 +201,7 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
     int rgx[RES ? NR : 1], rgy[RES ? NR : 1];
     bool rown[RES ? NR : 1];
 #pragma unroll
-    for (int u = 0; u < NR; ++u) {
+    for (int u = 0; u < (V2 ? 0 : NR); ++u) {
         const int idx = min(tid + u * NT, RX * RY - 1);
         const int ly = idx / RX, lx = idx - ly * RX;
         int gx = x0 + lx - 1, gy = y0 + ly - 1;
@@ -202,6 +233,27 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
     auto fetch_e = [&](int kz) {
         const int kw = wrapz(kz);
         const bool new_coarse = !(kz & 1) || kz == z0 - 2;
+        if (V2) {
+            const T* Ep = Ein + (size_t)kw * plane;
+#pragma unroll
+            for (int u = 0; u < NEP; ++u) {
+                if (tid + u * NT < EXP * EY) {
+                    if (ZERO_IN) {
+                        pe2[u] = make_float2(0.0f, 0.0f);
+                    } else {
+                        float2 v = *reinterpret_cast<const float2*>(Ep + eoff2[u]);
+                        if (PRO) {
+                            // both cells of a pair have the same parent
+                            if (new_coarse) pc2[u] = (float)Ec[(size_t)(kw >> 1) * cplane + coff2[u]];
+                            v.x = v.x + pc2[u];
+                            v.y = v.y + pc2[u];
+                        }
+                        pe2[u] = v;
+                    }
+                }
+            }
+            return;
+        }
 #pragma unroll
         for (int u = 0; u < NE; ++u) {
             const int idx = tid + u * NT;
@@ -221,6 +273,13 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
     };
     auto put_e = [&](int kz) {
         T* dst = &se[kz & 3][0][0];
+        if (V2) {
+            float2* d2 = reinterpret_cast<float2*>(dst);
+#pragma unroll
+            for (int u = 0; u < NEP; ++u)
+                if (tid + u * NT < EXP * EY) d2[tid + u * NT] = pe2[u];
+            return;
+        }
 #pragma unroll
         for (int u = 0; u < NE; ++u) {
             const int idx = tid + u * NT;
@@ -262,6 +321,13 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
             }
             return;
         }
+        if (V2) {
+            const T* Rp = R + (size_t)kw * plane;
+#pragma unroll
+            for (int u = 0; u < NRP; ++u)
+                if (tid + u * NT < RXP * RY) pr2[u] = *reinterpret_cast<const float2*>(Rp + roff2[u]);
+            return;
+        }
 #pragma unroll
         for (int u = 0; u < NR; ++u) {
             const int idx = tid + u * NT;
@@ -270,6 +336,13 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
     };
     auto put_r = [&](int kz) {
         T* dst = &sr[kz & 1][0][0];
+        if (V2) {
+            float2* d2 = reinterpret_cast<float2*>(dst);
+#pragma unroll
+            for (int u = 0; u < NRP; ++u)
+                if (tid + u * NT < RXP * RY) d2[tid + u * NT] = pr2[u];
+            return;
+        }
 #pragma unroll
         for (int u = 0; u < NR; ++u) {
             const int idx = tid + u * NT;
@@ -300,7 +373,7 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
                 const int lx = 2 * mm + ((x0 + y0 + ly + kr) & 1);  // gx + gy + kr even
                 const int ex = lx + 1, ey = ly + 1;
                 const T ec = se[sc][ey][ex];
-                const T res = sr[rs][ly][lx] - gamma * ec
+                const T res = sr[rs][ly][lx + RSH] - gamma * ec
                             - dhx * (se[sc][ey][ex - 1] + se[sc][ey][ex + 1])
                             - dhy * (se[sc][ey - 1][ex] + se[sc][ey + 1][ex])
                             - dhz * (se[sm][ey][ex] + se[sp][ey][ex]);
@@ -318,7 +391,7 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
             const int bxl = lx + (1 - par);            // local x of the black cell
             const int ex = bxl + 2, ey = ly + 2;
             const T ec = se[sc][ey][ex];
-            const T res = sr[rs][ly + 1][bxl + 1] - gamma * ec
+            const T res = sr[rs][ly + 1][bxl + 1 + RSH] - gamma * ec
                         - dhx * (se[sc][ey][ex - 1] + se[sc][ey][ex + 1])
                         - dhy * (se[sc][ey - 1][ex] + se[sc][ey + 1][ex])
                         - dhz * (se[sm][ey][ex] + se[sp][ey][ex]);

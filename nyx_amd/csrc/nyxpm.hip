@@ -412,7 +412,7 @@ int setup_boxes(nyxpm_ctx* c, int nbox, const nyxpm_geom* geoms) {
             L.inv_gamma = 1.0 / L.gamma;
             L.nc = (long long)L.n[0] * L.n[1] * L.n[2];
             L.off = mg;
-            mg += L.nc;
+            mg += (L.nc + 63) / 64 * 64;  // every level starts 64-cell aligned (paired FP32 loads)
         }
         B.G_off = Gtot;
         Gtot += nb;
