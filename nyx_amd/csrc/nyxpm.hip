@@ -2366,7 +2366,7 @@ int nyxpm_autotune(nyxpm_ctx* c, int precision_mask, char* report, int report_le
     const int nbox = (int)c->boxes.size();
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(kBlock), 0, c->stream, c->active, (long long)nbox, 1);
     static const int shapes[][2] = {{128, 8}, {64, 16}, {64, 8}, {32, 8}, {16, 8}};
-    static const int zcs[] = {8, 16, 32, 64, 128, 256};
+    static const int zcs[] = {2, 4, 8, 16, 32, 64, 128, 256};
     hipEvent_t e0, e1;
     HIPCK(c, hipEventCreate(&e0));
     HIPCK(c, hipEventCreate(&e1));
