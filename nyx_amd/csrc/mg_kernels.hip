@@ -14,6 +14,7 @@
 //   residual + restrict   8 (e) + 8 (r) + 1 (coarse)    / 4+4+0.5
 //   prolong + add         8 (e) + 8 (e write) + 1       / 4+4+0.5
 #include "nyxpm_internal.h"
+#include <type_traits>
 
 namespace nyxpm {
 
@@ -124,10 +125,14 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
     constexpr int EX = TX + 4, EY = TY + 4;   // e ring planes: halo 2
     constexpr int RX = TX + 2, RY = TY + 2;   // r planes and red-update region: halo 1
     constexpr int NE = (EX * EY + NT - 1) / NT, NR = (RX * RY + NT - 1) / NT;
-    // FP32: planes are moved as pairs of x-adjacent cells (8-byte loads and LDS
-    // stores, half the instructions).  Pairs start at even x (nx is even), so the
-    // r planes are kept with halo 2 in x as well (RSH shifts the r index).
-    constexpr bool V2 = sizeof(T) == 4 && !RES;
+    // Planes are moved as pairs of x-adjacent cells (8-byte FP32 or 16-byte FP64
+    // loads and LDS stores, half the instructions; one coarse load per pair in the
+    // prolongation).  Pairs start at even x (nx is even; level offsets are 64-cell
+    // aligned), so the r planes are kept with halo 2 in x as well (RSH shifts the
+    // r index).  FP64 pairs only in the prolongation modes: elsewhere the FP64
+    // sweep measured faster with single-cell moves.
+    constexpr bool V2 = !RES && (sizeof(T) == 4 || PRO);
+    using T2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
     constexpr int RXL = V2 ? TX + 4 : RX, RSH = V2 ? 1 : 0;
     constexpr int EXP = EX / 2, RXP = RXL / 2;
     constexpr int NEP = (EXP * EY + NT - 1) / NT, NRP = (RXP * RY + NT - 1) / NT;
@@ -155,14 +160,21 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
     const int tid = threadIdx.x;
 
     // Planes are fetched into registers one iteration ahead of their use, so the
-    // global-memory latency overlaps the previous plane's compute.  A thread's
-    // in-plane offsets (periodic wrap included) are the same for every plane and
-    // are computed once; the coarse values of the prolongation change every
-    // second plane (z0 is even), so they are reloaded only when kz is even.
+    // global-memory latency overlaps the previous plane's compute.  Every thread
+    // loads and stores all of its elements: an element index past the region is
+    // clamped onto the region's last element, so the extra lanes load that cell
+    // and store the same value to the same LDS word as its owner.  With no
+    // branch around them the loads issue back to back and only the LDS stores
+    // wait for them.  A thread's in-plane offsets (periodic wrap included) are
+    // the same for every plane and are computed once; the coarse values of the
+    // prolongation change every second plane (z0 is even), so they are reloaded
+    // only when kz is even, and are added when the plane is stored to LDS.
     T pe[V2 ? 1 : NE], pr[V2 ? 1 : NR], pc[PRO && !V2 ? NE : 1];
     size_t eoff[V2 ? 1 : NE], coff_[PRO && !V2 ? NE : 1], roff[V2 ? 1 : NR];
-    float2 pe2[V2 ? NEP : 1], pr2[V2 ? NRP : 1];
-    float pc2[V2 && PRO ? NEP : 1];
+    // the two cells of each pair in separate scalar arrays (an array of double2
+    // under a conditional load is not split into registers)
+    T pex[V2 ? NEP : 1], pey[V2 ? NEP : 1], prx[V2 ? NRP : 1], pry[V2 ? NRP : 1];
+    T pc2[V2 && PRO ? NEP : 1];
     size_t eoff2[V2 ? NEP : 1], coff2[V2 && PRO ? NEP : 1], roff2[V2 ? NRP : 1];
     const size_t plane = (size_t)nx * ny, cplane = (size_t)cnx * cny;
     if (V2) {
@@ -237,54 +249,44 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
             const T* Ep = Ein + (size_t)kw * plane;
 #pragma unroll
             for (int u = 0; u < NEP; ++u) {
-                if (tid + u * NT < EXP * EY) {
-                    if (ZERO_IN) {
-                        pe2[u] = make_float2(0.0f, 0.0f);
-                    } else {
-                        float2 v = *reinterpret_cast<const float2*>(Ep + eoff2[u]);
-                        if (PRO) {
-                            // both cells of a pair have the same parent
-                            if (new_coarse) pc2[u] = (float)Ec[(size_t)(kw >> 1) * cplane + coff2[u]];
-                            v.x = v.x + pc2[u];
-                            v.y = v.y + pc2[u];
-                        }
-                        pe2[u] = v;
-                    }
+                if (ZERO_IN) {
+                    pex[u] = T(0);
+                    pey[u] = T(0);
+                } else {
+                    const T2 v = *reinterpret_cast<const T2*>(Ep + eoff2[u]);
+                    // both cells of a pair have the same parent (added in put_e)
+                    if (PRO && new_coarse) pc2[u] = Ec[(size_t)(kw >> 1) * cplane + coff2[u]];
+                    pex[u] = v.x;
+                    pey[u] = v.y;
                 }
             }
             return;
         }
 #pragma unroll
         for (int u = 0; u < NE; ++u) {
-            const int idx = tid + u * NT;
-            if (idx < EX * EY) {
-                if (ZERO_IN) {
-                    pe[u] = T(0);
-                } else {
-                    T v = Ein[(size_t)kw * plane + eoff[u]];
-                    if (PRO) {
-                        if (new_coarse) pc[u] = Ec[(size_t)(kw >> 1) * cplane + coff_[u]];
-                        v = v + pc[u];
-                    }
-                    pe[u] = v;
-                }
+            if (ZERO_IN) {
+                pe[u] = T(0);
+            } else {
+                pe[u] = Ein[(size_t)kw * plane + eoff[u]];
+                if (PRO && new_coarse) pc[u] = Ec[(size_t)(kw >> 1) * cplane + coff_[u]];
             }
         }
     };
     auto put_e = [&](int kz) {
         T* dst = &se[kz & 3][0][0];
         if (V2) {
-            float2* d2 = reinterpret_cast<float2*>(dst);
+            T2* d2 = reinterpret_cast<T2*>(dst);
 #pragma unroll
-            for (int u = 0; u < NEP; ++u)
-                if (tid + u * NT < EXP * EY) d2[tid + u * NT] = pe2[u];
+            for (int u = 0; u < NEP; ++u) {
+                T2 v;
+                v.x = PRO ? pex[u] + pc2[u] : pex[u];
+                v.y = PRO ? pey[u] + pc2[u] : pey[u];
+                d2[min(tid + u * NT, EXP * EY - 1)] = v;
+            }
             return;
         }
 #pragma unroll
-        for (int u = 0; u < NE; ++u) {
-            const int idx = tid + u * NT;
-            if (idx < EX * EY) dst[idx] = pe[u];
-        }
+        for (int u = 0; u < NE; ++u) dst[min(tid + u * NT, EX * EY - 1)] = PRO ? pe[u] + pc[u] : pe[u];
     };
     auto fetch_r = [&](int kz) {
         const int kw = wrapz(kz);
@@ -325,29 +327,31 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
             const T* Rp = R + (size_t)kw * plane;
 #pragma unroll
             for (int u = 0; u < NRP; ++u)
-                if (tid + u * NT < RXP * RY) pr2[u] = *reinterpret_cast<const float2*>(Rp + roff2[u]);
+                {
+                    const T2 v = *reinterpret_cast<const T2*>(Rp + roff2[u]);
+                    prx[u] = v.x;
+                    pry[u] = v.y;
+                }
             return;
         }
 #pragma unroll
-        for (int u = 0; u < NR; ++u) {
-            const int idx = tid + u * NT;
-            if (idx < RX * RY) pr[u] = R[(size_t)kw * plane + roff[u]];
-        }
+        for (int u = 0; u < NR; ++u) pr[u] = R[(size_t)kw * plane + roff[u]];
     };
     auto put_r = [&](int kz) {
         T* dst = &sr[kz & 1][0][0];
         if (V2) {
-            float2* d2 = reinterpret_cast<float2*>(dst);
+            T2* d2 = reinterpret_cast<T2*>(dst);
 #pragma unroll
-            for (int u = 0; u < NRP; ++u)
-                if (tid + u * NT < RXP * RY) d2[tid + u * NT] = pr2[u];
+            for (int u = 0; u < NRP; ++u) {
+                T2 v;
+                v.x = prx[u];
+                v.y = pry[u];
+                d2[min(tid + u * NT, RXP * RY - 1)] = v;
+            }
             return;
         }
 #pragma unroll
-        for (int u = 0; u < NR; ++u) {
-            const int idx = tid + u * NT;
-            if (idx < RX * RY) dst[idx] = pr[u];
-        }
+        for (int u = 0; u < NR; ++u) dst[min(tid + u * NT, RX * RY - 1)] = pr[u];
     };
 
     fetch_e(z0 - 2);
@@ -359,9 +363,18 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
     for (int kr = z0 - 1; kr <= z1; ++kr) {
         put_e(kr + 1);
         put_r(kr);
-        if (kr < z1) {
-            fetch_e(kr + 2);
-            fetch_r(kr + 1);
+        // No branch around the prefetch loads (the last iteration re-loads planes
+        // z1+1 and z1, already loaded), so the waits before the LDS stores and the
+        // black step's store count only what they need.
+        if (RES) {   // fetch_r forms and stores r: exactly once per plane
+            if (kr < z1) {
+                fetch_e(kr + 2);
+                fetch_r(kr + 1);
+            }
+        } else {
+            const int kf = min(kr + 2, z1 + 1);
+            fetch_e(kf);
+            fetch_r(kf - 1);
         }
         __syncthreads();
         {   // red cells of plane kr on the halo-1 region: (i+j+k) even, which is
@@ -794,13 +807,15 @@ __global__ void __launch_bounds__(kCoarseThreads) k_coarse_vcycle(const CellSeg*
     (void)nl;
 }
 
-// Level-0 residual r = f - L phi (FP64), stored as T.  Threads own (i, j)
-// columns (64 x 4 per block) and march up a chunk of zc planes keeping
-// phi(k-1), phi(k), phi(k+1) in registers; in-plane neighbours come from cache.
-// The arithmetic is that of lap7.  Each block folds max |r| (and max |f| when nf
-// != nullptr) into one of kNormSlots per-box slots with atomicMax on the bit
-// pattern (exact and order independent); k_check reduces the slots.
-constexpr int kResTX = 64, kResTY = 4;
+// Level-0 residual r = f - L phi (FP64), stored as T.  A block owns a TX x TY
+// column of cells and marches up a chunk of zc planes; each phi plane (with a
+// one-cell periodic halo in x and y) is loaded once into a ring of 4 LDS planes,
+// one plane ahead of its use, and the 7-point stencil reads the ring.  Threads
+// of a tile that runs past the box edge load wrapped halo values and store
+// nothing.  The arithmetic is that of lap7.  Each block folds max |r| (and max
+// |f| when nf != nullptr) into one of kNormSlots per-box slots with atomicMax
+// on the bit pattern (exact and order independent); k_check reduces the slots.
+constexpr int kResTX = 64, kResTY = 16;
 template <typename T>
 __global__ void __launch_bounds__(kResTX * kResTY) k_residual0(const CellSeg* segs, int nseg, const int* active,
                                                               const double* __restrict__ f,
@@ -808,16 +823,22 @@ __global__ void __launch_bounds__(kResTX * kResTY) k_residual0(const CellSeg* se
                                                               double inv_a, const double* __restrict__ phi,
                                                               T* __restrict__ r,
                                                               unsigned long long* nr, unsigned long long* nf) {
+    constexpr int TX = kResTX, TY = kResTY, NT = TX * TY;
+    constexpr int HX = TX + 2, HY = TY + 2, NH = (HX * HY + NT - 1) / NT;
+    __shared__ double sp[4][HY][HX];
     const int bid = xcd_block(blockIdx.x, gridDim.x);
     const int s = find_seg(segs, nseg, bid);
     const CellSeg& sg = segs[s];
     if (!active[sg.box]) return;
     const int nx = sg.n[0], ny = sg.n[1], nz = sg.n[2];
-    const int nbx = (nx + kResTX - 1) / kResTX, nby = (ny + kResTY - 1) / kResTY;
+    const int nbx = (nx + TX - 1) / TX, nby = (ny + TY - 1) / TY;
     const int b = bid - sg.blk0;
     const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
-    const int i = bx * kResTX + (threadIdx.x & (kResTX - 1));
-    const int j = by * kResTY + (threadIdx.x / kResTX);
+    const int x0 = bx * TX, y0 = by * TY;
+    const int tid = threadIdx.x;
+    const int lx = tid % TX, ly = tid / TX;
+    const int i = x0 + lx, j = y0 + ly;
+    const bool own = i < nx && j < ny;
     const int z0 = bz * sg.zchunk, z1 = min(nz, z0 + sg.zchunk);
     const double* P = phi + sg.coff;
     const double* F = f + sg.coff;
@@ -827,48 +848,81 @@ __global__ void __launch_bounds__(kResTX * kResTY) k_residual0(const CellSeg* se
     // ((f - solv) * G) * (1/a), formed here instead of in a pass of its own
     const bool raw = solv != nullptr;
     const double ns = raw ? -solv[sg.box] : 0.0;
+    const size_t plane = (size_t)nx * ny;
+    const size_t col = own ? (size_t)j * nx + i : 0;
+    // in-plane offsets of this thread's elements of the halo region (periodic)
+    size_t hoff[NH];
+#pragma unroll
+    for (int u = 0; u < NH; ++u) {
+        const int idx = min(tid + u * NT, HX * HY - 1);
+        const int hy = idx / HX, hx = idx - hy * HX;
+        int gx = (x0 + hx - 1) % nx, gy = (y0 + hy - 1) % ny;
+        gx = gx < 0 ? gx + nx : gx;
+        gy = gy < 0 ? gy + ny : gy;
+        hoff[u] = (size_t)gy * nx + gx;
+    }
+    auto wrapz = [&](int k) { return k < 0 ? k + nz : (k >= nz ? k - nz : k); };
+    double hv[NH];
+    auto fetch = [&](int k) {
+        const double* Pk = P + (size_t)wrapz(k) * plane;
+#pragma unroll
+        for (int u = 0; u < NH; ++u)
+            if (tid + u * NT < HX * HY) hv[u] = Pk[hoff[u]];
+    };
+    auto put = [&](int k) {
+        double* dst = &sp[k & 3][0][0];
+#pragma unroll
+        for (int u = 0; u < NH; ++u)
+            if (tid + u * NT < HX * HY) dst[tid + u * NT] = hv[u];
+    };
     double ar = 0.0, af = 0.0;
-    if (i < nx && j < ny) {
-        const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
-        const int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
-        const size_t plane = (size_t)nx * ny;
-        const size_t col = (size_t)j * nx + i;
-        const size_t oxm = (size_t)j * nx + im, oxp = (size_t)j * nx + ip;
-        const size_t oym = (size_t)jm * nx + i, oyp = (size_t)jp * nx + i;
-        double pm = P[(size_t)(z0 == 0 ? nz - 1 : z0 - 1) * plane + col];
-        double pc = P[(size_t)z0 * plane + col];
-        for (int k = z0; k < z1; ++k) {
-            const size_t zo = (size_t)k * plane;
-            const double pp = P[(size_t)(k == nz - 1 ? 0 : k + 1) * plane + col];
-            const double lx = dhx * (P[zo + oxm] - 2.0 * pc + P[zo + oxp]);
-            const double ly = dhy * (P[zo + oym] - 2.0 * pc + P[zo + oyp]);
-            const double lz = dhz * (pm - 2.0 * pc + pp);
-            double fv = F[zo + col];
+    fetch(z0 - 1);
+    put(z0 - 1);
+    fetch(z0);
+    put(z0);
+    fetch(z0 + 1);
+    double fn = own ? F[(size_t)z0 * plane + col] : 0.0;
+    // iteration k: plane k+1 goes into the ring, the loads of plane k+2 (and f
+    // of plane k+1) are issued, plane k is formed.  Plane k+1 takes the slot of
+    // plane k-3, last read in iteration k-2, one barrier earlier.
+    for (int k = z0; k < z1; ++k) {
+        put(k + 1);
+        const double fv0 = fn;
+        if (k + 1 < z1) {
+            fetch(k + 2);
+            if (own) fn = F[(size_t)(k + 1) * plane + col];
+        }
+        __syncthreads();
+        if (own) {
+            const int ex = lx + 1, ey = ly + 1;
+            const double pc = sp[k & 3][ey][ex];
+            const double lxv = dhx * (sp[k & 3][ey][ex - 1] - 2.0 * pc + sp[k & 3][ey][ex + 1]);
+            const double lyv = dhy * (sp[k & 3][ey - 1][ex] - 2.0 * pc + sp[k & 3][ey + 1][ex]);
+            const double lzv = dhz * (sp[(k - 1) & 3][ey][ex] - 2.0 * pc + sp[(k + 1) & 3][ey][ex]);
+            double fv = fv0;
             if (raw) {
                 const double t = fv + ns;
                 fv = (t * Ggravity) * inv_a;
             }
-            const double v = fv - (lx + ly + lz);
-            Rr[zo + col] = T(v);
+            const double v = fv - (lxv + lyv + lzv);
+            Rr[(size_t)k * plane + col] = T(v);
             ar = fmax(ar, fabs(v));
             af = fmax(af, fabs(fv));
-            pm = pc;
-            pc = pp;
         }
     }
     for (int o = kWave / 2; o > 0; o >>= 1) {
         ar = fmax(ar, __shfl_xor(ar, o));
         af = fmax(af, __shfl_xor(af, o));
     }
-    __shared__ double wr[kResTX * kResTY / kWave], wf[kResTX * kResTY / kWave];
-    if ((threadIdx.x & (kWave - 1)) == 0) {
-        wr[threadIdx.x / kWave] = ar;
-        wf[threadIdx.x / kWave] = af;
+    __shared__ double wr[NT / kWave], wf[NT / kWave];
+    if ((tid & (kWave - 1)) == 0) {
+        wr[tid / kWave] = ar;
+        wf[tid / kWave] = af;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
         double mr = wr[0], mf = wf[0];
-        for (int w = 1; w < kResTX * kResTY / kWave; ++w) {
+        for (int w = 1; w < NT / kWave; ++w) {
             mr = fmax(mr, wr[w]);
             mf = fmax(mf, wf[w]);
         }

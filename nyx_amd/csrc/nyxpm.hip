@@ -439,7 +439,7 @@ void sweep_seg_shape(nyxpm_ctx* c, int pr, int d, const Level& L, CellSeg& w) {
     const int tx = c->sw[pr][d].tx, ty = c->sw[pr][d].ty;
     const long long cols = (long long)(L.n[0] / tx) * (L.n[1] / ty);
     const int sz = pr == 0 ? 8 : 4;
-    const long long lds = (4LL * (ty + 4) * (tx + 4) + 2LL * (ty + 2) * (tx + 2)) * sz;
+    const long long lds = (4LL * (ty + 4) * (tx + 4) + 2LL * (ty + 2) * (tx + 4)) * sz;
     const long long per_cu = std::max(1LL, std::min(2048LL / (tx * ty / 2), (160LL * 1024) / lds));
     const long long resident = 256LL * per_cu;
     long long nzc = (4 * resident + cols - 1) / std::max(1LL, cols);
@@ -667,12 +667,12 @@ int build_tables(nyxpm_ctx* c) {
             s.coff = c->boxes[b].cell_off;
             s.nblk = std::min(s.nblk, kReduceBlocks);
             vr.push_back(s);
-            // z-marching residual: 64 x 4 columns, chunks of up to 32 planes
+            // z-marching residual: kResTX x kResTY columns, chunks of up to 64 planes
             const Level& L = c->boxes[b].lev[0];
             CellSeg z = mk(b, 0, 1);
             z.coff = c->boxes[b].cell_off;
             const int nbx = (L.n[0] + kResTX - 1) / kResTX, nby = (L.n[1] + kResTY - 1) / kResTY;
-            z.zchunk = std::min(32, L.n[2]);
+            z.zchunk = std::min(64, L.n[2]);
             z.nblk = nbx * nby * ((L.n[2] + z.zchunk - 1) / z.zchunk);
             vz.push_back(z);
         }
