@@ -159,22 +159,27 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
     double* Ph = phi + sg.foff;
     const int tid = threadIdx.x;
 
-    // Planes are fetched into registers one iteration ahead of their use, so the
-    // global-memory latency overlaps the previous plane's compute.  Every thread
+    // Planes are fetched into registers DEPTH iterations ahead of their use
+    // (DEPTH register sets, used by alternate iterations), so the global-memory
+    // latency overlaps DEPTH planes' compute.  Every thread
     // loads and stores all of its elements: an element index past the region is
     // clamped onto the region's last element, so the extra lanes load that cell
     // and store the same value to the same LDS word as its owner.  With no
     // branch around them the loads issue back to back and only the LDS stores
     // wait for them.  A thread's in-plane offsets (periodic wrap included) are
     // the same for every plane and are computed once; the coarse values of the
-    // prolongation change every second plane (z0 is even), so they are reloaded
-    // only when kz is even, and are added when the plane is stored to LDS.
-    T pe[V2 ? 1 : NE], pr[V2 ? 1 : NR], pc[PRO && !V2 ? NE : 1];
+    // prolongation change at even planes: an odd plane takes those of the plane
+    // fetched before it (the other register set; fetches go in plane order), and
+    // they are added when the plane is stored to LDS.
+    // (two sets only where measured faster: the plain and zero-input sweeps; the
+    // prolongation and update modes hold more registers per set)
+    constexpr int DEPTH = RES || PRO || UPD ? 1 : 2;
+    T pe[DEPTH][V2 ? 1 : NE], pr[DEPTH][V2 ? 1 : NR], pc[DEPTH][PRO && !V2 ? NE : 1];
     size_t eoff[V2 ? 1 : NE], coff_[PRO && !V2 ? NE : 1], roff[V2 ? 1 : NR];
     // the two cells of each pair in separate scalar arrays (an array of double2
     // under a conditional load is not split into registers)
-    T pex[V2 ? NEP : 1], pey[V2 ? NEP : 1], prx[V2 ? NRP : 1], pry[V2 ? NRP : 1];
-    T pc2[V2 && PRO ? NEP : 1];
+    T pex[DEPTH][V2 ? NEP : 1], pey[DEPTH][V2 ? NEP : 1], prx[DEPTH][V2 ? NRP : 1], pry[DEPTH][V2 ? NRP : 1];
+    T pc2[DEPTH][V2 && PRO ? NEP : 1];
     size_t eoff2[V2 ? NEP : 1], coff2[V2 && PRO ? NEP : 1], roff2[V2 ? NRP : 1];
     const size_t plane = (size_t)nx * ny, cplane = (size_t)cnx * cny;
     if (V2) {
@@ -242,22 +247,23 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
             wc[u] = Pf[(size_t)wrapz(z0 - 1) * plane + roff[u]];
         }
     }
-    auto fetch_e = [&](int kz) {
+    auto fetch_e = [&](int kz, int S) {
         const int kw = wrapz(kz);
         const bool new_coarse = !(kz & 1) || kz == z0 - 2;
+        const int O = DEPTH - 1 - S;  // the set of the plane fetched before
         if (V2) {
             const T* Ep = Ein + (size_t)kw * plane;
 #pragma unroll
             for (int u = 0; u < NEP; ++u) {
                 if (ZERO_IN) {
-                    pex[u] = T(0);
-                    pey[u] = T(0);
+                    pex[S][u] = T(0);
+                    pey[S][u] = T(0);
                 } else {
                     const T2 v = *reinterpret_cast<const T2*>(Ep + eoff2[u]);
                     // both cells of a pair have the same parent (added in put_e)
-                    if (PRO && new_coarse) pc2[u] = Ec[(size_t)(kw >> 1) * cplane + coff2[u]];
-                    pex[u] = v.x;
-                    pey[u] = v.y;
+                    if (PRO) pc2[S][u] = new_coarse ? Ec[(size_t)(kw >> 1) * cplane + coff2[u]] : pc2[O][u];
+                    pex[S][u] = v.x;
+                    pey[S][u] = v.y;
                 }
             }
             return;
@@ -265,30 +271,30 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
 #pragma unroll
         for (int u = 0; u < NE; ++u) {
             if (ZERO_IN) {
-                pe[u] = T(0);
+                pe[S][u] = T(0);
             } else {
-                pe[u] = Ein[(size_t)kw * plane + eoff[u]];
-                if (PRO && new_coarse) pc[u] = Ec[(size_t)(kw >> 1) * cplane + coff_[u]];
+                pe[S][u] = Ein[(size_t)kw * plane + eoff[u]];
+                if (PRO) pc[S][u] = new_coarse ? Ec[(size_t)(kw >> 1) * cplane + coff_[u]] : pc[O][u];
             }
         }
     };
-    auto put_e = [&](int kz) {
+    auto put_e = [&](int kz, int S) {
         T* dst = &se[kz & 3][0][0];
         if (V2) {
             T2* d2 = reinterpret_cast<T2*>(dst);
 #pragma unroll
             for (int u = 0; u < NEP; ++u) {
                 T2 v;
-                v.x = PRO ? pex[u] + pc2[u] : pex[u];
-                v.y = PRO ? pey[u] + pc2[u] : pey[u];
+                v.x = PRO ? pex[S][u] + pc2[S][u] : pex[S][u];
+                v.y = PRO ? pey[S][u] + pc2[S][u] : pey[S][u];
                 d2[min(tid + u * NT, EXP * EY - 1)] = v;
             }
             return;
         }
 #pragma unroll
-        for (int u = 0; u < NE; ++u) dst[min(tid + u * NT, EX * EY - 1)] = PRO ? pe[u] + pc[u] : pe[u];
+        for (int u = 0; u < NE; ++u) dst[min(tid + u * NT, EX * EY - 1)] = PRO ? pe[S][u] + pc[S][u] : pe[S][u];
     };
-    auto fetch_r = [&](int kz) {
+    auto fetch_r = [&](int kz, int S) {
         const int kw = wrapz(kz);
         if (RES) {
             // k_residual0's arithmetic on the planes kz-1, kz, kz+1 (called for
@@ -312,7 +318,7 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
                     fv = (t * res.G) * res.inv_a;
                 }
                 const double v = fv - (lx + ly + lz);
-                pr[u] = T(v);
+                pr[S][u] = T(v);
                 if (own_plane && rown[u]) {
                     const_cast<T*>(R)[zo + roff[u]] = T(v);
                     ar = fmax(ar, fabs(v));
@@ -329,52 +335,49 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
             for (int u = 0; u < NRP; ++u)
                 {
                     const T2 v = *reinterpret_cast<const T2*>(Rp + roff2[u]);
-                    prx[u] = v.x;
-                    pry[u] = v.y;
+                    prx[S][u] = v.x;
+                    pry[S][u] = v.y;
                 }
             return;
         }
 #pragma unroll
-        for (int u = 0; u < NR; ++u) pr[u] = R[(size_t)kw * plane + roff[u]];
+        for (int u = 0; u < NR; ++u) pr[S][u] = R[(size_t)kw * plane + roff[u]];
     };
-    auto put_r = [&](int kz) {
+    auto put_r = [&](int kz, int S) {
         T* dst = &sr[kz & 1][0][0];
         if (V2) {
             T2* d2 = reinterpret_cast<T2*>(dst);
 #pragma unroll
             for (int u = 0; u < NRP; ++u) {
                 T2 v;
-                v.x = prx[u];
-                v.y = pry[u];
+                v.x = prx[S][u];
+                v.y = pry[S][u];
                 d2[min(tid + u * NT, RXP * RY - 1)] = v;
             }
             return;
         }
 #pragma unroll
-        for (int u = 0; u < NR; ++u) dst[min(tid + u * NT, RX * RY - 1)] = pr[u];
+        for (int u = 0; u < NR; ++u) dst[min(tid + u * NT, RX * RY - 1)] = pr[S][u];
     };
 
-    fetch_e(z0 - 2);
-    put_e(z0 - 2);
-    fetch_e(z0 - 1);
-    put_e(z0 - 1);
-    fetch_e(z0);
-    fetch_r(z0 - 1);
-    for (int kr = z0 - 1; kr <= z1; ++kr) {
-        put_e(kr + 1);
-        put_r(kr);
-        // No branch around the prefetch loads (the last iteration re-loads planes
+    // iteration kr: stores e plane kr+1 and r plane kr (fetched DEPTH iterations
+    // earlier into set S), refills set S, updates the red cells of plane kr and
+    // the black cells of plane kr-1
+    auto step = [&](int kr, int S) {
+        put_e(kr + 1, S);
+        put_r(kr, S);
+        // No branch around the prefetch loads (the last iterations re-load planes
         // z1+1 and z1, already loaded), so the waits before the LDS stores and the
         // black step's store count only what they need.
         if (RES) {   // fetch_r forms and stores r: exactly once per plane
             if (kr < z1) {
-                fetch_e(kr + 2);
-                fetch_r(kr + 1);
+                fetch_e(kr + 2, S);
+                fetch_r(kr + 1, S);
             }
         } else {
-            const int kf = min(kr + 2, z1 + 1);
-            fetch_e(kf);
-            fetch_r(kf - 1);
+            const int kf = min(kr + 1 + DEPTH, z1 + 1);
+            fetch_e(kf, S);
+            fetch_r(kf - 1, S);
         }
         __syncthreads();
         {   // red cells of plane kr on the halo-1 region: (i+j+k) even, which is
@@ -421,6 +424,24 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
             }
         }
         __syncthreads();
+    };
+    fetch_e(z0 - 2, 0);
+    put_e(z0 - 2, 0);
+    fetch_e(z0 - 1, DEPTH - 1);
+    put_e(z0 - 1, DEPTH - 1);
+    fetch_e(z0, 0);
+    fetch_r(z0 - 1, 0);
+    int kr = z0 - 1;
+    if (DEPTH == 2) {
+        fetch_e(z0 + 1, DEPTH - 1);
+        fetch_r(z0, DEPTH - 1);
+        for (; kr + 1 <= z1; kr += 2) {
+            step(kr, 0);
+            step(kr + 1, DEPTH - 1);
+        }
+        if (kr <= z1) step(kr, 0);
+    } else {
+        for (; kr <= z1; ++kr) step(kr, 0);
     }
     if (RES) {
         // k_residual0's norm slots: max over the block, atomicMax on the bit pattern
