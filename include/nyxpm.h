@@ -260,6 +260,11 @@ int nyxpm_autotune(nyxpm_ctx *ctx, int precision_mask, char *report, int report_
  * per depth d, "sweep0.dD", "sweep.dD", "sweep_p.dD", "sweep_u.dD",
  * "smooth.dD", "resid_restrict.dD", "prolong.dD", "fill.dD". */
 int nyxpm_timing_enable(nyxpm_ctx *ctx, int on);
+/* Restricts the timing to the stages named in `names` (comma-separated, e.g.
+ * "kick_pair"); NULL or "" times every stage.  Each timed launch adds two event
+ * markers to the stream, so a measurement of whole steps times only the stages
+ * it reports.  No counterpart in the reference (measurement only). */
+int nyxpm_timing_select(nyxpm_ctx *ctx, const char *names);
 int nyxpm_timing_get(nyxpm_ctx *ctx, const char *name, int64_t *count, double *mean_ms,
                      double *total_ms);
 int nyxpm_timing_reset(nyxpm_ctx *ctx);

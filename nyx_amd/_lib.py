@@ -88,6 +88,7 @@ SIGNATURES = [
     ("nyxpm_pm_steps_comoving", _i, [_vp, _i, _d, _vp, _d, _d, _d, _i, _d, _d, _d, _i, _vp]),
     ("nyxpm_autotune", _i, [_vp, _i, _vp, _i]),
     ("nyxpm_timing_enable", _i, [_vp, _i]),
+    ("nyxpm_timing_select", _i, [_vp, _s]),
     ("nyxpm_timing_get", _i, [_vp, _s, _vp, _vp, _vp]),
     ("nyxpm_timing_reset", _i, [_vp]),
     ("nyxpm_device_bytes", _i64, [_vp]),

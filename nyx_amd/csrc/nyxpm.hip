@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -251,6 +252,7 @@ struct nyxpm_ctx {
 
     // timing
     bool timing = false;
+    std::set<std::string> timing_only;   // empty: every stage is timed
     std::map<std::string, std::vector<EventPair>> pending;
     std::map<std::string, std::pair<long long, double>> timed;
     std::vector<hipEvent_t> event_pool;
@@ -324,6 +326,7 @@ struct Timed {
         return e;
     }
     Timed(nyxpm_ctx* c_, const char* n_) : c(c_), name(n_) {
+        if (c->timing && !c->timing_only.empty() && !c->timing_only.count(name)) return;
         if (c->timing) {
             a = get();
             b = get();
@@ -331,7 +334,7 @@ struct Timed {
         }
     }
     ~Timed() {
-        if (c->timing) {
+        if (a) {   // bracketed when constructed
             hipEventRecord(b, c->stream);
             c->pending[name].push_back({a, b});
         }
@@ -2446,6 +2449,20 @@ int nyxpm_autotune(nyxpm_ctx* c, int precision_mask, char* report, int report_le
 int nyxpm_timing_enable(nyxpm_ctx* c, int on) {
     if (!c) return NYXPM_ERR_ARG;
     c->timing = on != 0;
+    return NYXPM_OK;
+}
+
+int nyxpm_timing_select(nyxpm_ctx* c, const char* names) {
+    if (!c) return NYXPM_ERR_ARG;
+    c->timing_only.clear();
+    if (!names) return NYXPM_OK;
+    std::string s(names);
+    size_t p = 0;
+    while (p <= s.size()) {
+        const size_t q = std::min(s.find(',', p), s.size());
+        if (q > p) c->timing_only.insert(s.substr(p, q - p));
+        p = q + 1;
+    }
     return NYXPM_OK;
 }
 

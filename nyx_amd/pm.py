@@ -299,6 +299,13 @@ class PMContext:
     def timing_enable(self, on=True):
         self._ck(self.L.nyxpm_timing_enable(self.h, int(bool(on))))
 
+    def timing_select(self, names=None):
+        """Time only the named stages (an iterable or comma-separated string of
+        stage names); None times every stage."""
+        if names is not None and not isinstance(names, str):
+            names = ",".join(names)
+        self._ck(self.L.nyxpm_timing_select(self.h, names.encode() if names else None))
+
     def timing_reset(self):
         self._ck(self.L.nyxpm_timing_reset(self.h))
 

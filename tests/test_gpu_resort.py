@@ -110,3 +110,29 @@ def test_calls_between_kick_drift_and_deposit(nyx):
     assert np.array_equal(np.asarray(a[2]), np.asarray(b[2]))
     assert np.array_equal(a[3], b[3])
     assert np.array_equal(a[4], b[4])
+
+
+def test_timing_select_times_only_the_named_stages(nyx):
+    # stage timing restricted to one stage: its launches are counted, the other
+    # stages' are not, and the results are those of an untimed run
+    inp = _hot(nyx, 32, 41, 0.5)
+    outs = []
+    for sel in (None, "deposit", ""):
+        with nyx.PMContext(nyx.BoxGeom((32, 32, 32))) as c:
+            c.upload(0, inp[0], inp[1], m_const=inp[3])
+            c.pm_init(1.0, 1.0, 1e-10)
+            c.timing_select(sel if sel != "" else None)
+            c.timing_enable(sel is not None)
+            c.pm_steps(3, 1e-3, rel_tol=1e-10)
+            c.sync()
+            c.timing_enable(False)
+            counts = {k: c.timing(k)[0] for k in ("deposit", "kick_pair", "residual")}
+            outs.append((c.download(0), counts))
+    (a, ca), (b, cb), (d, cd) = outs
+    assert ca == {"deposit": 0, "kick_pair": 0, "residual": 0}
+    assert cb["deposit"] >= 3 and cb["kick_pair"] == 0 and cb["residual"] == 0
+    assert cd["deposit"] >= 3 and cd["kick_pair"] >= 2 and cd["residual"] >= 3
+    for q in (b, d):
+        for d_ in range(3):
+            assert np.array_equal(a[0][d_], q[0][d_])
+            assert np.array_equal(a[1][d_], q[1][d_])
