@@ -130,10 +130,12 @@ def test_fused_sweeps_equal_half_sweeps(nyx, orc, n, nu):
     assert np.array_equal(out[0], ref)
 
 
-@pytest.mark.parametrize("shape,zc", [((128, 8), 0), ((64, 16), 0), ((64, 8), 5), ((32, 8), 0), ((16, 8), 3)])
+@pytest.mark.parametrize("shape,zc", [((128, 8), 0), ((64, 16), 0), ((64, 8), 5), ((32, 8), 0), ((16, 8), 3),
+                                      ((64, 8), 2), ((16, 8), 4), ((32, 8), 1)])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_sweep_shapes_equal_half_sweeps(nyx, shape, zc, precision):
-    # every fused-sweep column shape and uneven z-chunks, FP64 and FP32 V-cycles
+    # every fused-sweep column shape, uneven z-chunks and the short chunks of the
+    # small levels (1, 2, 4 planes), FP64 and FP32 V-cycles
     n = (256, 64, 32)
     r = np.random.default_rng(12)
     f = r.standard_normal((n[2], n[1], n[0]))
