@@ -106,7 +106,7 @@ __global__ void __launch_bounds__(kBlock) k_keys(const BoxSeg* segs, int nseg, c
                                                  const double* __restrict__ z, int* __restrict__ key,
                                                  int* counts, int* err) {
     const BoxSeg sg = segs[find_seg(segs, nseg, blockIdx.x)];
-    const BoxDev& B = boxes[sg.box];
+    const BoxDev B = boxes[__builtin_amdgcn_readfirstlane(sg.box)];
     const long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x;
     bool valid = false;
     int t = 0;
@@ -232,7 +232,7 @@ __global__ void __launch_bounds__(kBlock) k_scatter(const BoxSeg* segs, int nseg
                                                     const long long* __restrict__ tbase, int* cursor,
                                                     PartPtrs src, PartPtrs dst) {
     const BoxSeg sg = segs[find_seg(segs, nseg, blockIdx.x)];
-    const BoxDev& B = boxes[sg.box];
+    const BoxDev B = boxes[__builtin_amdgcn_readfirstlane(sg.box)];
     const long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x;
     const long long p = B.part_off + q;
     // occupied slots only (a kick-drift keys the slots it visits, not the free ones)
@@ -295,7 +295,10 @@ __global__ void __launch_bounds__(kBlock) k_deposit(const BoxSeg* segs, int nseg
         if (thw[tile] > kHeavyTile) return;
     }
     const BoxSeg sg = segs[find_seg(segs, nseg, tile)];
-    const BoxDev& B = boxes[sg.box];
+    // the box's fields as block-uniform values in registers: read through a
+    // reference they would be re-read from memory after every possible store
+    // or atomic (the error flag) of the particle loop
+    const BoxDev B = boxes[__builtin_amdgcn_readfirstlane(sg.box)];
     const int lt = tile - sg.blk0;
     const int tx = lt % B.ntile[0], ty = (lt / B.ntile[0]) % B.ntile[1], tz = lt / (B.ntile[0] * B.ntile[1]);
     const int o0 = tx * B.tile[0], o1 = ty * B.tile[1], o2 = tz * B.tile[2];
@@ -393,7 +396,7 @@ __global__ void __launch_bounds__(kBlock) k_deposit_mom(const BoxSeg* segs, int 
                                                         int* err) {
     extern __shared__ unsigned long long acc[];
     const BoxSeg sg = segs[find_seg(segs, nseg, blockIdx.x)];
-    const BoxDev& B = boxes[sg.box];
+    const BoxDev B = boxes[__builtin_amdgcn_readfirstlane(sg.box)];
     const int lt = blockIdx.x - sg.blk0;
     const int tx = lt % B.ntile[0], ty = (lt / B.ntile[0]) % B.ntile[1], tz = lt / (B.ntile[0] * B.ntile[1]);
     const int o0 = tx * B.tile[0], o1 = ty * B.tile[1], o2 = tz * B.tile[2];
@@ -460,7 +463,7 @@ __global__ void __launch_bounds__(kBlock) k_max_mom(const BoxSeg* segs, int nseg
                                                     const double* __restrict__ vy, const double* __restrict__ vz,
                                                     unsigned long long* maxbits) {
     const BoxSeg sg = segs[find_seg(segs, nseg, blockIdx.x)];
-    const BoxDev& B = boxes[sg.box];
+    const BoxDev B = boxes[__builtin_amdgcn_readfirstlane(sg.box)];
     const long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x;
     if (q >= B.span) return;
     const long long p = B.part_off + q;
@@ -562,7 +565,7 @@ __global__ void __launch_bounds__(kBlock) k_fold(const CellSeg* segs, int nseg, 
                                                  double* __restrict__ partial) {
     // rho may be own itself (each cell is read, then written, by one thread)
     const CellSeg& sg = segs[find_seg(segs, nseg, blockIdx.x)];
-    const BoxDev& B = boxes[sg.box];
+    const BoxDev B = boxes[__builtin_amdgcn_readfirstlane(sg.box)];
     double acc = 0.0;  // this thread's cells, in a fixed order
     for_block_cells(sg, B, [&](int i, int j, int k, long long c) {
         const size_t q = B.cell_off + (size_t)c;
@@ -582,7 +585,7 @@ __global__ void __launch_bounds__(kBlock) k_fold_raw(const CellSeg* segs, int ns
                                                      const double* __restrict__ own,
                                                      const double* __restrict__ halo, double* __restrict__ out) {
     const CellSeg& sg = segs[find_seg(segs, nseg, blockIdx.x)];
-    const BoxDev& B = boxes[sg.box];
+    const BoxDev B = boxes[__builtin_amdgcn_readfirstlane(sg.box)];
     for_block_cells(sg, B, [&](int i, int j, int k, long long c) {
         const size_t q = B.cell_off + (size_t)c;
         out[q] = fold_cell(B, own, halo, i, j, k, q);
@@ -595,7 +598,7 @@ __global__ void __launch_bounds__(kBlock) k_vels_finish(const BoxDev* boxes, int
                                                         const double* __restrict__ mass,
                                                         const double* __restrict__ mom,
                                                         long long mom_stride, double* __restrict__ out) {
-    const BoxDev& B = boxes[box];
+    const BoxDev B = boxes[box];
     const long long nc = (long long)B.n[0] * B.n[1] * B.n[2];
     const long long q = (long long)blockIdx.x * kBlock + threadIdx.x;
     if (q >= nc) return;
@@ -627,7 +630,7 @@ __global__ void __launch_bounds__(kBlock) k_rhs_final(const CellSeg* segs, int n
                                                       const double* __restrict__ solv, double* __restrict__ f,
                                                       double Ggravity, double inv_a) {
     const CellSeg& sg = segs[find_seg(segs, nseg, blockIdx.x)];
-    const BoxDev& B = boxes[sg.box];
+    const BoxDev B = boxes[__builtin_amdgcn_readfirstlane(sg.box)];
     const long long n = (long long)B.n[0] * B.n[1] * B.n[2];
     const long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x;
     if (q >= n) return;
@@ -721,7 +724,7 @@ __global__ void __launch_bounds__(kBlock) k_kick(const BoxSeg* segs, int nseg, c
                                                  double* __restrict__ vz, double dt, double a_prev, double a_cur,
                                                  int* __restrict__ key, int* counts, int* err) {
     const BoxSeg sg = segs[find_seg(segs, nseg, blockIdx.x)];
-    const BoxDev& B = boxes[sg.box];
+    const BoxDev B = boxes[__builtin_amdgcn_readfirstlane(sg.box)];
     const long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x;
     const long long p = B.part_off + q;
     double px = 0.0;
@@ -932,7 +935,7 @@ __global__ void __launch_bounds__(kBlock) k_est_dt(const BoxSeg* segs, int nseg,
                                                    const double* __restrict__ vz, double a, double cfl,
                                                    unsigned long long* dtmin) {
     const BoxSeg sg = segs[find_seg(segs, nseg, blockIdx.x)];
-    const BoxDev& B = boxes[sg.box];
+    const BoxDev B = boxes[__builtin_amdgcn_readfirstlane(sg.box)];
     const long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x;
     double d = 1e50;
     if (q < B.span) {
@@ -1002,8 +1005,11 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
     if ((int)blockIdx.x >= n_items) return;
     const TileItem it = items[xcd_block(blockIdx.x, n_items)];
     const int sgi = find_seg(tsegs, ntseg, it.tile);
-    const int box = tsegs[sgi].box;
-    const BoxDev& B = boxes[box];
+    const int box = __builtin_amdgcn_readfirstlane(tsegs[sgi].box);
+    // the box's fields as block-uniform values in registers: read through a
+    // reference they would be re-read from memory (and waited for, behind the
+    // particle stores) after every store or atomic of the particle loop
+    const BoxDev B = boxes[box];
     const int lt = it.tile - tsegs[sgi].blk0;
     const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
     const int tx = lt % B.ntile[0], ty = (lt / B.ntile[0]) % B.ntile[1], tz = lt / (B.ntile[0] * B.ntile[1]);
@@ -1374,7 +1380,7 @@ __global__ void __launch_bounds__(kBlock) k_wrap(const BoxSeg* segs, int nseg, c
                                                  double* __restrict__ x, double* __restrict__ y,
                                                  double* __restrict__ z, int* err) {
     const BoxSeg sg = segs[find_seg(segs, nseg, blockIdx.x)];
-    const BoxDev& B = boxes[sg.box];
+    const BoxDev B = boxes[__builtin_amdgcn_readfirstlane(sg.box)];
     const long long q = (long long)(blockIdx.x - sg.blk0) * kBlock + threadIdx.x;
     if (q >= B.span) return;
     const long long p = B.part_off + q;
@@ -1387,7 +1393,7 @@ __global__ void __launch_bounds__(kBlock) k_wrap(const BoxSeg* segs, int nseg, c
 // Cell-centred gravity field of one box (get_new_grav_vector), 3 components.
 __global__ void __launch_bounds__(kBlock) k_grav_vector(const BoxDev* boxes, int box, const double* __restrict__ phi,
                                                         double* __restrict__ g) {
-    const BoxDev& B = boxes[box];
+    const BoxDev B = boxes[box];
     const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
     const long long nc = (long long)nx * ny * nz;
     const long long q = (long long)blockIdx.x * kBlock + threadIdx.x;
@@ -1455,7 +1461,7 @@ __global__ void __launch_bounds__(kBlock) k_rhs1(const CellSeg* segs, int nseg, 
                                                  const double* __restrict__ rho, double* __restrict__ f,
                                                  double* __restrict__ partial) {
     const CellSeg& sg = segs[find_seg(segs, nseg, blockIdx.x)];
-    const BoxDev& B = boxes[sg.box];
+    const BoxDev B = boxes[__builtin_amdgcn_readfirstlane(sg.box)];
     double acc = 0.0;
     for_block_cells(sg, B, [&](int, int, int, long long c) {
         const size_t q = B.cell_off + (size_t)c;
@@ -1473,7 +1479,7 @@ __global__ void __launch_bounds__(kBlock) k_gather(const BoxDev* boxes, int box,
                                                    const double* __restrict__ x, const double* __restrict__ y,
                                                    const double* __restrict__ z, const int* __restrict__ id,
                                                    double* __restrict__ out, int* err) {
-    const BoxDev& B = boxes[box];
+    const BoxDev B = boxes[box];
     const long long q = (long long)blockIdx.x * kBlock + threadIdx.x;
     if (q >= B.span) return;
     const long long p = B.part_off + q;
