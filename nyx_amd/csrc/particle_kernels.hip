@@ -1037,7 +1037,13 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
     };
     if ((int)threadIdx.x < it.np) load1(threadIdx.x);
     {
-        // rows of SX cells: a wave takes rpw rows per pass (SX <= 35 < 64)
+        // rows of SX cells: a wave takes rpw rows per pass (SX <= 35 < 64).  The
+        // passes go in batches of kStageBatch: every load of a batch is issued
+        // before any of them is stored, so their latencies overlap (one load,
+        // wait, store per pass left 8 serial latencies per lane and tile).  Rows
+        // past the last are clamped onto it (the extra lanes store the owner's
+        // value to the same LDS word), so no branch separates the loads.
+        constexpr int kStageBatch = 8;
         const int rpw = kWave / SX;
         const int lrow = lane / SX, lx = lane - lrow * SX;
         if (lrow < rpw) {
@@ -1047,14 +1053,24 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
             int ly = r0 % SY, lz = r0 / SY;
             const int dly = step % SY, dlz = step / SY;
             while (lz < SZ) {
-                const int gy = wrap1(o1 - 2 + ly, ny), gz = wrap1(o2 - 2 + lz, nz);
-                sphi[(lz * SY + ly) * SX + lx] = P[((size_t)gz * ny + gy) * nx + gx];
-                ly += dly;
-                lz += dlz;
-                if (ly >= SY) {
-                    ly -= SY;
-                    ++lz;
+                double v[kStageBatch];
+                int o[kStageBatch];
+#pragma unroll
+                for (int u = 0; u < kStageBatch; ++u) {
+                    const bool in = lz < SZ;
+                    const int cy = in ? ly : SY - 1, cz = in ? lz : SZ - 1;
+                    const int gy = wrap1(o1 - 2 + cy, ny), gz = wrap1(o2 - 2 + cz, nz);
+                    v[u] = P[((size_t)gz * ny + gy) * nx + gx];
+                    o[u] = (cz * SY + cy) * SX + lx;
+                    ly += dly;
+                    lz += dlz;
+                    if (ly >= SY) {
+                        ly -= SY;
+                        ++lz;
+                    }
                 }
+#pragma unroll
+                for (int u = 0; u < kStageBatch; ++u) sphi[o[u]] = v[u];
             }
         }
     }
