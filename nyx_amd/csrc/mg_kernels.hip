@@ -279,6 +279,10 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
         }
     };
     auto put_e = [&](int kz, int S) {
+        // zero input: the ring's input values are never read (the red step uses
+        // literal zeros, the black step reads only red values written this
+        // sweep), so the zero planes are not stored
+        if (ZERO_IN) return;
         T* dst = &se[kz & 3][0][0];
         if (V2) {
             T2* d2 = reinterpret_cast<T2*>(dst);
@@ -388,11 +392,14 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
                 const int ly = idx / HX, mm = idx - ly * HX;
                 const int lx = 2 * mm + ((x0 + y0 + ly + kr) & 1);  // gx + gy + kr even
                 const int ex = lx + 1, ey = ly + 1;
-                const T ec = se[sc][ey][ex];
+                // with zero input every operand but r is an input zero (the
+                // black cells of planes kr-1, kr, kr+1 are not yet updated):
+                // the same operations on literal zeros, the same bits
+                const T ec = ZERO_IN ? T(0) : se[sc][ey][ex];
                 const T res = sr[rs][ly][lx + RSH] - gamma * ec
-                            - dhx * (se[sc][ey][ex - 1] + se[sc][ey][ex + 1])
-                            - dhy * (se[sc][ey - 1][ex] + se[sc][ey + 1][ex])
-                            - dhz * (se[sm][ey][ex] + se[sp][ey][ex]);
+                            - dhx * (ZERO_IN ? T(0) + T(0) : se[sc][ey][ex - 1] + se[sc][ey][ex + 1])
+                            - dhy * (ZERO_IN ? T(0) + T(0) : se[sc][ey - 1][ex] + se[sc][ey + 1][ex])
+                            - dhz * (ZERO_IN ? T(0) + T(0) : se[sm][ey][ex] + se[sp][ey][ex]);
                 se[sc][ey][ex] = ec + res * inv_gamma;
             }
         }
@@ -406,7 +413,7 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
             const int par = (x0 + lx + gy + kb) & 1;  // 0: cell lx is red, lx+1 black
             const int bxl = lx + (1 - par);            // local x of the black cell
             const int ex = bxl + 2, ey = ly + 2;
-            const T ec = se[sc][ey][ex];
+            const T ec = ZERO_IN ? T(0) : se[sc][ey][ex];   // the black cell's input
             const T res = sr[rs][ly + 1][bxl + 1 + RSH] - gamma * ec
                         - dhx * (se[sc][ey][ex - 1] + se[sc][ey][ex + 1])
                         - dhy * (se[sc][ey - 1][ex] + se[sc][ey + 1][ex])
