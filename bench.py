@@ -180,28 +180,86 @@ def stage_table(ctx, names, ncell, npart, halo_cells, sz, elapsed_ms):
     return out
 
 
-def cpu_baseline(n_cpu, threads, steps):
+def host_cpu():
+    """CPU model and core counts of this host (/proc/cpuinfo, as lscpu reads it)."""
+    model, cores, logical = None, set(), 0
+    try:
+        phys = core = None
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "processor":
+                    logical += 1
+                elif k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                    cores.add((phys, core))
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except Exception:
+        usable = os.cpu_count() or 1
+    return {"model": model, "physical_cores": len(cores) or None, "logical_cpus": logical or None,
+            "usable_cpus": usable}
+
+
+def cpu_baseline(n, pos, vel, phi, m, threads, dt, tol):
+    """The oracle (the C/OpenMP FP64 restatement of the Nyx CPU path) on this
+    host: one canonical KDK step of the workload from the same state as the GPU's
+    timed steps (particles and phi), then one step of Nyx's two-solve form (old
+    phi re-solved from a zero guess, Nyx_advance_particles.cpp:148-170)."""
     import oracle
-    from nyx_amd import ics
     oracle.set_num_threads(threads)
-    pos, vel, m = ics.plummer_clustered(n_cpu, ics.SEED_BASE + 2, n_spheres=max(1, int(2000 * (n_cpu / 512) ** 3)))
-    n = (n_cpu,) * 3
+    nn = (n, n, n)
     pos = [np.ascontiguousarray(a) for a in pos]
     vel = [np.ascontiguousarray(a) for a in vel]
-    phi, _, _ = oracle.solve_for_phi(n, (0, 0, 0), (1, 1, 1), pos, m_const=m, rel_tol=1e-10)
+    phi = np.ascontiguousarray(phi)
     t0 = time.perf_counter()
-    stages = np.zeros(3)
-    for _ in range(steps):
-        _, st = oracle.pm_step(n, (0, 0, 0), (1, 1, 1), pos, vel, phi, m_const=m, dt=1e-3, rel_tol=1e-10)
-        stages += st[4:7]
-    t = time.perf_counter() - t0
-    return {"value": round(n_cpu ** 3 * steps / t, 1), "unit": "particle-steps/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{steps} canonical KDK steps (deposit, MG to rel 1e-10 with {int(st[0])} V-cycles in the "
-                      f"last, gradient, kick-drift, kick) of the same clustered recipe at {n_cpu}^3 "
-                      f"cells/particles ({max(1, int(2000 * (n_cpu / 512) ** 3))} Plummer spheres), FP64, "
-                      f"{t:.2f} s; stage s: deposit {stages[0]:.2f} solve {stages[1]:.2f} "
-                      f"grav+kick {stages[2]:.2f}"}
+    _, st = oracle.pm_step(nn, (0, 0, 0), (1, 1, 1), pos, vel, phi, m_const=m, dt=dt, rel_tol=tol)
+    t1 = time.perf_counter()
+    _, st2 = oracle.pm_step(nn, (0, 0, 0), (1, 1, 1), pos, vel, phi, m_const=m, dt=dt, rel_tol=tol,
+                            nyx_faithful=True)
+    t2 = time.perf_counter()
+    cpu = host_cpu()
+    return {"value": round(n ** 3 / (t1 - t0), 1), "unit": "particle-steps/s", "cores": threads,
+            "kind": "port", "cpu_model": cpu["model"], "host_physical_cores": cpu["physical_cores"],
+            "host_usable_cpus": cpu["usable_cpus"],
+            "nyx_faithful_value": round(n ** 3 / (t2 - t1), 1),
+            "sample": f"one canonical KDK step of the same {n}^3 clustered workload from the state the GPU's "
+                      f"timed steps start from (FP64 deposit, MG to rel {tol:g} with {int(st[0])} V-cycles, "
+                      f"gradient, kick-drift, kick) in {t1 - t0:.2f} s (deposit {st[4]:.2f} s, solve "
+                      f"{st[5]:.2f} s, gravity + kicks {st[6]:.2f} s); then one step of Nyx's two-solve form "
+                      f"(old phi from a zero guess, {int(st2[7])} + {int(st2[0])} V-cycles) in {t2 - t1:.2f} s; "
+                      f"{threads} OpenMP threads"}
+
+
+class Snapshot:
+    """Particles (upload order) and phi of a context, kept in device memory, so
+    that each precision is timed from the same state."""
+
+    def __init__(self, ctx, n):
+        import torch
+        N = n ** 3
+        self.N = N
+        self.parts = [torch.empty(N, dtype=torch.float64, device="cuda") for _ in range(6)]
+        self.phi = torch.empty(N, dtype=torch.float64, device="cuda")
+        ctx.download_device([t.data_ptr() for t in self.parts])
+        ctx.get_phi_device(self.phi.data_ptr())
+        ctx.sync()
+
+    def restore(self, ctx, m):
+        ctx.upload_device(0, self.N, [t.data_ptr() for t in self.parts], m_const=m)
+        ctx.set_phi_device(self.phi.data_ptr())
+        ctx.sync()
+
+    def host(self):
+        a = [t.cpu().numpy() for t in self.parts]
+        return a[:3], a[3:], self.phi.cpu().numpy()
 
 
 def main():
@@ -210,8 +268,6 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=512)
-    ap.add_argument("--cpu-n", type=int, default=384)
-    ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-10)
@@ -219,9 +275,14 @@ def main():
     ap.add_argument("--modes", default="mixed,fp64", help="precisions to time (comma list)")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the mixed-vs-FP64 acceleration check")
     ap.add_argument("--no-tune", action="store_true", help="skip the smoother-shape autotuner")
+    ap.add_argument("--no-stages", action="store_true", help="skip the per-stage tables")
     ap.add_argument("--kick-chunk", type=int, default=0, help="particles per kick work item (0: default)")
     ap.add_argument("--phi-guess", default="default", choices=["default", "previous", "extrapolate"],
                     help="initial guess of each step's solve (default = previous phi, as Nyx)")
+    ap.add_argument("--trace-summary", default=None, metavar="KERNEL_TRACE_CSV",
+                    help="instead of running: summarise a rocprofv3 kernel trace of a bench run over that "
+                         "run's timed mixed steps (needs --from-json with the run's JSON line)")
+    ap.add_argument("--from-json", default=None)
     ap.add_argument("--workload", default="clustered512",
                     choices=["clustered512", "batch48", "grf256", "grf1024"],
                     help="clustered512: the headline line; batch48: 48 boxes (16 each of 64^3, 96^3, "
@@ -229,6 +290,8 @@ def main():
                          "256^3 GRF, 10 steps; grf1024: 1024^3 GRF, 50 mixed steps with per-step "
                          "accel rel-L2 against FP64 and the step-50 power spectrum check (--n to shrink)")
     a = ap.parse_args()
+    if a.trace_summary:
+        return trace_summary(a.trace_summary, a.from_json)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md, "
               "'How the path would shard'); run with --gpus 1", file=sys.stderr)
@@ -240,20 +303,23 @@ def main():
     if a.workload == "grf256":
         return main_grf256(a)
 
+    import torch
     import nyx_amd
     from nyx_amd import ics
     n = a.n
     dt = 1e-3
-    log(f"generating {n}^3 clustered particles")
+    npart = ncell = n ** 3
+    nsph = max(1, int(2000 * (n / 512) ** 3))
+    log(f"generating {n}^3 clustered particles on the device")
     t = time.perf_counter()
-    pos, vel, m = ics.plummer_clustered(n, ics.SEED_BASE + 2, n_spheres=max(1, int(2000 * (n / 512) ** 3)))
+    pos, vel, m = ics.plummer_clustered_device(n, ics.SEED_BASE + 2, n_spheres=nsph)
     log(f"  {time.perf_counter() - t:.1f} s")
     ctx = nyx_amd.PMContext(nyx_amd.BoxGeom((n, n, n)), precision="fp64", kick_chunk=a.kick_chunk,
                             phi_guess=a.phi_guess)
-    ctx.upload(0, pos, vel, m_const=m)
+    ctx.upload_device(0, npart, [t_.data_ptr() for t_ in pos + vel], m_const=m)
+    ctx.sync()
     del pos, vel
-    npart = n ** 3
-    ncell = n ** 3
+    torch.cuda.empty_cache()
     log("device bytes", ctx.device_bytes())
     tuned = None
     if not a.no_tune:
@@ -266,15 +332,26 @@ def main():
     ctx.set_precision("fp64")
     ctx.pm_init(1.0, 1.0, a.tol)
     rel_l2 = None
+    g64 = gmx = None
     if not a.no_accuracy:
-        g64 = ctx.gather_accel()
+        g64 = torch.empty(3 * npart, dtype=torch.float64, device="cuda")
+        gmx = torch.empty(3 * npart, dtype=torch.float64, device="cuda")
+        ctx.gather_accel_device(g64.data_ptr())
         ctx.set_precision("mixed")
         it_m, _ = ctx.solve_for_phi(1.0, 1.0, a.tol, 0.0, use_guess=False)
-        gmx = ctx.gather_accel()
-        rel_l2 = float(np.sqrt(np.sum((gmx - g64) ** 2)) / np.sqrt(np.sum(g64 ** 2)))
-        del g64, gmx
+        ctx.gather_accel_device(gmx.data_ptr())
+        rel_l2 = (torch.linalg.vector_norm(gmx - g64) / torch.linalg.vector_norm(g64)).item()
         log(f"accel rel-L2 mixed vs fp64 = {rel_l2:.3e}  (mixed cold-start cycles {int(it_m[0])})")
+        ctx.set_precision("fp64")
+        ctx.solve_for_phi(1.0, 1.0, a.tol, 0.0, use_guess=False)  # back to the FP64 phi
     modes = a.modes.split(",")
+    # warm-up steps, then the state every timed run starts from
+    warm_prec = "mixed" if "mixed" in modes else "fp64"
+    ctx.set_precision(warm_prec)
+    if a.warmup:
+        ctx.pm_steps(a.warmup, dt, rel_tol=a.tol, use_graph=a.graph)
+    snap = Snapshot(ctx, n)
+    log(f"{a.warmup} {warm_prec} warm-up steps; state saved")
 
     names = ["residual", "norm", "update", "deposit", "fold", "rhs", "kick_drift", "kick_pair", "kick", "scatter",
              "scan", "coarse", "migrate", "res_sweep0", "guess",
@@ -284,110 +361,197 @@ def main():
              for d in range(0, 12)]
     halo_cells = tile_halo_cells((n, n, n), TILE)
 
-    t_mix = t_64 = None
-    it_mix = it_64 = None
-    rs_mix = rs_64 = None
-    rel_l2_end = None
-    st_mix, st_64 = {}, {}
-    dom_mix = dom_64 = None
-    if "mixed" in modes:
-        log(f"mixed: {a.warmup} warmup + {a.steps} timed steps")
-        t_mix, it_mix, rs_mix, dom_mix = run_timed(ctx, "mixed", a.steps, a.warmup, dt, a.tol, a.graph)
-        log(f"  {t_mix / a.steps * 1e3:.2f} ms/step, {it_mix} V-cycles")
-        if not a.no_accuracy:
-            # the state the timed mixed steps left: their phi against an FP64 solve of
-            # the same density from a zero guess
-            gmx = ctx.gather_accel()
+    def settle(prec):
+        # same state, then one untimed step of this precision (its first sort is
+        # a full one after the restore)
+        snap.restore(ctx, m)
+        ctx.set_precision(prec)
+        ctx.pm_steps(1, dt, rel_tol=a.tol, use_graph=a.graph)
+        ctx.sync()
+
+    res = {}
+    windows = {}
+    for prec in ("mixed", "fp64"):
+        if prec not in modes:
+            continue
+        settle(prec)
+        log(f"{prec}: {a.steps} timed steps from the saved state")
+        w0 = time.monotonic_ns()
+        t_, it_, rs_, dom_ = run_timed(ctx, prec, a.steps, 0, dt, a.tol, a.graph)
+        windows[prec] = [w0, time.monotonic_ns()]
+        res[prec] = {"t": t_, "it": it_, "rs": rs_, "dom": dom_}
+        log(f"  {t_ / a.steps * 1e3:.2f} ms/step, {it_} V-cycles")
+        if prec == "mixed" and not a.no_accuracy:
+            # the phi the timed mixed steps left against an FP64 solve of the same
+            # density from a zero guess
+            ctx.gather_accel_device(gmx.data_ptr())
             ctx.set_precision("fp64")
             ctx.assign_density(want=False)
             ctx.solve_for_phi(1.0, 1.0, a.tol, 0.0, use_guess=False)
-            g64 = ctx.gather_accel()
-            rel_l2_end = float(np.sqrt(np.sum((gmx - g64) ** 2)) / np.sqrt(np.sum(g64 ** 2)))
-            del gmx, g64
-            log(f"  accel rel-L2 mixed vs fp64 after the timed steps = {rel_l2_end:.3e}")
-    if "fp64" in modes:
-        log(f"fp64: {a.warmup} warmup + {a.steps} timed steps")
-        t_64, it_64, rs_64, dom_64 = run_timed(ctx, "fp64", a.steps, a.warmup, dt, a.tol, a.graph)
-        log(f"  {t_64 / a.steps * 1e3:.2f} ms/step, {it_64} V-cycles")
-    # the stage tables: further steps of each mode, every stage bracketed by events
-    if "mixed" in modes:
-        st_mix, dom_mix, ti = run_instrumented(ctx, "mixed", a.steps, dt, a.tol, a.graph, names, ncell, npart,
-                                               halo_cells, 4, dom_mix)
-        log(f"  mixed stage table: {a.steps} instrumented steps, {ti / a.steps * 1e3:.2f} ms/step")
-    if "fp64" in modes:
-        st_64, dom_64, ti = run_instrumented(ctx, "fp64", a.steps, dt, a.tol, a.graph, names, ncell, npart,
-                                             halo_cells, 8, dom_64)
-        log(f"  fp64 stage table: {a.steps} instrumented steps, {ti / a.steps * 1e3:.2f} ms/step")
+            ctx.gather_accel_device(g64.data_ptr())
+            res[prec]["rel_end"] = (torch.linalg.vector_norm(gmx - g64) / torch.linalg.vector_norm(g64)).item()
+            log(f"  accel rel-L2 mixed vs fp64 after the timed steps = {res[prec]['rel_end']:.3e}")
+    del g64, gmx
+    torch.cuda.empty_cache()
+    # the stage tables: the same steps again from the same state, every stage
+    # bracketed by events
+    st = {}
+    if not a.no_stages:
+        for prec in ("mixed", "fp64"):
+            if prec not in modes:
+                continue
+            settle(prec)
+            st[prec], res[prec]["dom"], ti = run_instrumented(ctx, prec, a.steps, dt, a.tol, a.graph, names, ncell,
+                                                              npart, halo_cells, 4 if prec == "mixed" else 8,
+                                                              res[prec]["dom"])
+            log(f"  {prec} stage table: {a.steps} instrumented steps, {ti / a.steps * 1e3:.2f} ms/step")
     ctx.close()
-    if t_mix is None:  # FP64 only: report it as the value
-        t_mix, it_mix, st_mix, rs_mix, dom_mix = t_64, it_64, st_64, rs_64, dom_64
-    if t_64 is None:
-        t_64, it_64, st_64, rs_64, dom_64 = t_mix, it_mix, st_mix, rs_mix, dom_mix
-
-    v_mix = npart * a.steps / t_mix
-    v_64 = npart * a.steps / t_64
-    # dominant kernel (largest share of a steady-state step, from the stage table);
-    # its mean launch duration is the one measured inside the timed steps
+    first = "mixed" if "mixed" in res else "fp64"
+    other = "fp64" if "fp64" in res else first
+    R, R64 = res[first], res[other]
+    v_mix = npart * a.steps / R["t"]
+    v_64 = npart * a.steps / R64["t"]
+    dom_mix = R["dom"]
     dname, dmean = dom_mix["stage"], dom_mix["mean_ms"]
-    b = kernel_bytes(dname, ncell, npart, halo_cells, 4)
+    b = kernel_bytes(dname, ncell, npart, halo_cells, 4 if first == "mixed" else 8)
     achieved = b / (dmean * 1e-3) / 1e9 if b and dmean > 0 else None
     roof = {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 1) if achieved else None,
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
             "traffic": measured_traffic(dname, n), "bytes_per_launch": int(b) if b else None,
             "mean_launch_ms": dmean, "launches_timed": dom_mix["launches"], "timed_in": dom_mix["timed_in"]}
 
-    def smoother_roof(st, sz):
-        e = st.get("sweep.d0")
+    def smoother_roof(stt, sz):
+        e = (stt or {}).get("sweep.d0")
         if not e:
             return None
-        b = kernel_bytes("sweep.d0", ncell, npart, halo_cells, sz)
-        ach = b / (e["mean_ms"] * 1e-3) / 1e9
+        bb = kernel_bytes("sweep.d0", ncell, npart, halo_cells, sz)
+        ach = bb / (e["mean_ms"] * 1e-3) / 1e9
         return {"kernel": "sweep.d0 (fused red+black sweep, finest level)", "achieved": round(ach, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                "bytes_per_launch": int(b), "mean_launch_ms": e["mean_ms"]}
+                "bytes_per_launch": int(bb), "mean_launch_ms": e["mean_ms"]}
 
     cpu = None
     if not a.no_cpu:
-        try:
-            aff = len(os.sched_getaffinity(0))
-        except Exception:
-            aff = os.cpu_count() or 1
-        threads = max(1, min(16, aff))
-        log(f"CPU baseline at {a.cpu_n}^3 on {threads} threads")
-        cpu = cpu_baseline(a.cpu_n, threads, a.cpu_steps)
+        info = host_cpu()
+        # the OpenMP threads this job may use (the box sets OMP_NUM_THREADS to its
+        # share of the host), else every usable CPU
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or info["usable_cpus"]
+        log(f"CPU baseline at {n}^3 on {threads} threads ({info})")
+        hp, hv, hphi = snap.host()
+        del snap
+        torch.cuda.empty_cache()
+        cpu = cpu_baseline(n, hp, hv, hphi.reshape(n, n, n), m, threads, dt, a.tol)
         log("  ", cpu["value"], "particle-steps/s")
 
     out = {
         "metric": METRIC, "value": round(v_mix, 1), "unit": "particle-steps/s", "n_gpus": 1,
-        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(t_mix / a.steps * 1e3, 3),
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(R["t"] / a.steps * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"{n}^3 cells, {n}^3 particles, 20% uniform + 80% in "
-                               f"{max(1, int(2000 * (n / 512) ** 3))} Plummer spheres, mixed precision "
-                               f"(FP32 V-cycles in FP64 refinement, FP64 particles), dt 1e-3, rel tol {a.tol:g}",
+        "config": {"workload": f"{n}^3 cells, {n}^3 particles, 20% uniform + 80% in {nsph} Plummer spheres, "
+                               f"mixed precision (FP32 V-cycles in FP64 refinement, FP64 particles), dt 1e-3, "
+                               f"rel tol {a.tol:g}",
                    "cells": ncell, "particles": npart, "graph": bool(a.graph), "autotuned": tuned},
-        "fp64": {"value": round(v_64, 1), "ms_per_step": round(t_64 / a.steps * 1e3, 3), "vcycles": it_64,
-                 "dtype": "f64", "resort": rs_64},
-        "resort": rs_mix,
+        "fp64": {"value": round(v_64, 1), "ms_per_step": round(R64["t"] / a.steps * 1e3, 3), "vcycles": R64["it"],
+                 "dtype": "f64", "resort": R64["rs"]},
+        "same_state": f"each precision restarts from the state after the {a.warmup} warm-up steps (particles "
+                      f"and phi restored), runs one untimed step, then the {a.steps} timed steps",
+        "resort": R["rs"],
         "mixed_over_fp64": round(v_mix / v_64, 3),
-        "vcycles_mixed": it_mix,
+        "vcycles_mixed": R["it"],
         "accel_rel_l2": rel_l2,
-        "accel_rel_l2_after_steps": rel_l2_end,
+        "accel_rel_l2_after_steps": R.get("rel_end"),
         "phi_guess": a.phi_guess,
         "roofline": roof,
-        "smoother_roofline": {"mixed": smoother_roof(st_mix, 4), "fp64": smoother_roof(st_64, 8)},
-        "stages_note": "stage tables: further steps of each mode (as many as timed), run after both "
-                       "timed runs with HIP events around every stage; the timed steps bracket only "
-                       "the paired kick and the level-0 residual (the dominant-kernel candidates)",
-        "stages_mixed": st_mix,
-        "stages_fp64": st_64,
+        "smoother_roofline": {"mixed": smoother_roof(st.get("mixed"), 4), "fp64": smoother_roof(st.get("fp64"), 8)},
+        "timed_windows_monotonic_ns": windows,
+        "stages_note": "stage tables: the same steps again from the same state with HIP events around every "
+                       "stage; the timed steps bracket only the paired kick and the level-0 residual (the "
+                       "dominant-kernel candidates)",
+        "stages_mixed": st.get("mixed", {}),
+        "stages_fp64": st.get("fp64", {}),
         "cpu_baseline": cpu,
     }
-    line = json.dumps(out)
-    if a.json_out:
-        with open(a.json_out, "w") as fh:
-            fh.write(line + "\n")
-    print(line, flush=True)
+    emit(out, a)
+
+
+def _stage_of_kernel(name):
+    """Stage name (as the library's timing classes) of a kernel in a rocprofv3
+    trace, from its demangled name; None for kernels outside the step."""
+    import re
+    s = name
+    if "k_rb_sweep" in s:
+        mt = re.search(r"k_rb_sweep<(\w+), *(\d+), *(\d+), *(\d+)>", s)
+        mode = int(mt.group(4)) if mt else -1
+        return {1: "sweep0", 2: "sweep_p", 4: "sweep_u", 6: "sweep_p", 0: "sweep"}.get(mode, "sweep?"), \
+            (mt.group(1) if mt else "?")
+    for k, v in (("k_residual0", "residual"), ("k_kick_tiled", None), ("k_deposit_mom", "deposit_mom"),
+                 ("k_deposit", "deposit"), ("k_fold", "fold"), ("k_resid_restrict_z", "resid_restrict"),
+                 ("k_resid_restrict", "resid_restrict"), ("k_coarse_vcycle", "coarse"), ("k_migrate", "migrate"),
+                 ("k_scatter", "scatter"), ("k_keys", "keys"), ("k_make_items", "items"),
+                 ("k_rhs_sum", "rhs"), ("k_check", "check"), ("k_update0", "update"),
+                 ("k_kick_deferred", "kick_deferred"), ("k_fill", "fill"), ("k_scan", "scan"),
+                 ("k_tile", "scan"), ("k_prolong", "prolong"), ("k_gsrb_half", "smooth"), ("k_bottom", "bottom"),
+                 ("k_graph_note", "check"), ("k_phi_guess", "guess")):
+        if k in s:
+            if k == "k_kick_tiled":
+                mt = re.search(r"k_kick_tiled<(\d+)", s)
+                return {"0": "kick", "1": "kick_drift", "2": "kick_pair"}.get(mt.group(1) if mt else "", "kick?"), ""
+            prec = "float" if "<float" in s else ("double" if "<double" in s else "")
+            return v, prec
+    return None, ""
+
+
+def trace_summary(csv_path, json_path):
+    """Per-kernel launches, mean duration and achieved GB/s over the timed mixed
+    steps of a bench run, from a rocprofv3 kernel trace (CSV) of that run and the
+    run's JSON line (which holds the timed windows)."""
+    import csv
+    with open(json_path) as fh:
+        run = json.loads(fh.read().strip().splitlines()[-1])
+    n = round(run["config"]["cells"] ** (1.0 / 3.0))
+    steps = run["steps"]
+    halo_cells = tile_halo_cells((n, n, n), TILE)
+    out_rows = []
+    for prec in ("mixed", "fp64"):
+        w = run.get("timed_windows_monotonic_ns", {}).get(prec)
+        if not w:
+            continue
+        agg = {}
+        with open(csv_path) as fh:
+            for r in csv.DictReader(fh):
+                t0, t1 = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+                if t0 < w[0] or t1 > w[1]:
+                    continue
+                nm = r["Kernel_Name"]
+                e = agg.setdefault(nm, [0, 0.0])
+                e[0] += 1
+                e[1] += (t1 - t0) * 1e-6
+        sz = 4 if prec == "mixed" else 8
+        rows = []
+        for nm, (cnt, tot) in agg.items():
+            stage, tprec = _stage_of_kernel(nm)
+            depth0 = False
+            b = None
+            if stage:
+                key = stage
+                if stage.startswith("sweep") or stage == "resid_restrict":
+                    key = stage + ".d0"
+                b = kernel_bytes(key, n ** 3, n ** 3, halo_cells, 4 if tprec == "float" else sz)
+                depth0 = True
+            rows.append((tot, nm, stage, cnt, tot / cnt, b, depth0))
+        rows.sort(reverse=True)
+        out_rows.append(f"### {prec} timed steps ({steps} steps, window {w[1] - w[0]} ns)\n")
+        out_rows.append("| kernel | stage | launches | mean us | ms per step | GB/s (level-0 bytes) | of 8 TB/s |")
+        out_rows.append("|---|---|---|---|---|---|---|")
+        for tot, nm, stage, cnt, mean, b, _ in rows:
+            short = nm.split("(")[0][:90]
+            gbs = f"{b / (mean * 1e-3) / 1e9:.0f}" if b else ""
+            frac = f"{b / (mean * 1e-3) / 1e9 / HBM_PEAK_GBS:.2f}" if b else ""
+            out_rows.append(f"| `{short}` | {stage or ''} | {cnt} | {mean * 1e3:.1f} | {tot / steps:.3f} | "
+                            f"{gbs} | {frac} |")
+        out_rows.append("")
+    print("\n".join(out_rows))
 
 
 def emit(out, a):

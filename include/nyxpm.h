@@ -116,6 +116,10 @@ typedef struct nyxpm_opts {
                           1: it is formed inside the first sweep of the next
                           V-cycle where the fused sweep runs (same results; one
                           pass fewer but half the occupancy, slower at 512^3) */
+    int graph_cycles;  /* V-cycles recorded per solve in a hipGraph-captured step
+                          (nyxpm_pm_steps with use_graph); 0 = default: the most
+                          V-cycles a solve of the same call needed eagerly plus 2,
+                          re-captured when a replay uses up that margin */
 } nyxpm_opts;
 
 /* ---- lifecycle ---------------------------------------------------------- */
@@ -146,8 +150,8 @@ int nyxpm_particles_upload_device(nyxpm_ctx *ctx, int box, int64_t np,
  * by file size) and uploads it as nyxpm_particles_upload does (per-particle
  * mass).  np_out (may be NULL) receives NP. */
 int nyxpm_particles_load_binary(nyxpm_ctx *ctx, int box, const char *path, int64_t *np_out);
-/* Copy box `box`'s particles to host arrays, in upload (id) order.  Any pointer
- * may be NULL to skip that component. */
+/* Copy box `box`'s particles to host or device arrays (unified addressing), in
+ * upload (id) order.  Any pointer may be NULL to skip that component. */
 int nyxpm_particles_download(nyxpm_ctx *ctx, int box,
                              double *x, double *y, double *z,
                              double *vx, double *vy, double *vz);
@@ -220,8 +224,11 @@ int nyxpm_pm_init(nyxpm_ctx *ctx, double a, double four_pi_G, double rel_tol, do
 /* nsteps KDK steps with fixed dt: KD with the stored phi, deposit, solve (previous
  * phi as guess), K, wrap.  nyx_faithful != 0 re-solves the old phi from a zero
  * guess at the start of each step, as Nyx_advance_particles.cpp:148-170 does.
- * use_graph != 0 captures the step into a hipGraph once and replays it (the
- * V-cycle count is then decided on the device; max_iter cycles are recorded).
+ * use_graph != 0 runs the first step eagerly, then captures two steps into a
+ * hipGraph once and replays them (the V-cycle count is decided on the device;
+ * opts.graph_cycles cycles are recorded per solve, by default the most the
+ * eager solves needed plus 2).  A solve of a replay that is still above its
+ * target after the recorded cycles returns NYXPM_ERR_NOT_CONVERGED.
  * iters (length nbox, may be NULL) receives the V-cycles of the last step. */
 int nyxpm_pm_steps(nyxpm_ctx *ctx, int nsteps, double dt, double a_old, double a_new,
                    double four_pi_G, double rel_tol, double abs_tol, int nyx_faithful,

@@ -652,15 +652,17 @@ __global__ void __launch_bounds__(kBlock) k_prolong(const CellSeg* segs, int nse
 }
 
 // Exact bottom solve: e[c] = sum_{c'} G[(c - c') mod n] r[c'], c' in natural order.
-// One block per box; r and G staged in LDS (at most 4096 cells each).
+// One block per box; r and G staged in LDS (at most 4096 cells each).  The
+// convolution is FP64 in both precisions (G and the sum in FP64, r widened), so
+// the FP32 V-cycle of the mixed path keeps its coarsest solve in FP64.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_bottom(const CellSeg* segs, int nseg, const int* active,
                                                    T* __restrict__ e, const T* __restrict__ r,
-                                                   const T* __restrict__ G) {
+                                                   const double* __restrict__ G) {
     const CellSeg& sg = segs[blockIdx.x];
     if (!active[sg.box]) return;
     __shared__ T sr[kBottomMaxCells];
-    __shared__ T sG[kBottomMaxCells];
+    __shared__ double sG[kBottomMaxCells];
     const int nx = sg.n[0], ny = sg.n[1], nz = sg.n[2];
     const int nc = nx * ny * nz;
     for (int c = threadIdx.x; c < nc; c += blockDim.x) {
@@ -670,18 +672,18 @@ __global__ void __launch_bounds__(kBlock) k_bottom(const CellSeg* segs, int nseg
     __syncthreads();
     for (int c = threadIdx.x; c < nc; c += blockDim.x) {
         const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
-        T acc = T(0);
+        double acc = 0.0;
         for (int qz = 0; qz < nz; ++qz) {
             int dz = cz - qz; dz = dz < 0 ? dz + nz : dz;
             for (int qy = 0; qy < ny; ++qy) {
                 int dy = cy - qy; dy = dy < 0 ? dy + ny : dy;
                 for (int qx = 0; qx < nx; ++qx) {
                     int dxx = cx - qx; dxx = dxx < 0 ? dxx + nx : dxx;
-                    acc += sG[(dz * ny + dy) * nx + dxx] * sr[(qz * ny + qy) * nx + qx];
+                    acc += sG[(dz * ny + dy) * nx + dxx] * (double)sr[(qz * ny + qy) * nx + qx];
                 }
             }
         }
-        e[sg.off + c] = acc;
+        e[sg.off + c] = T(acc);
     }
 }
 
@@ -726,7 +728,7 @@ template <typename T>
 __global__ void __launch_bounds__(kCoarseThreads) k_coarse_vcycle(const CellSeg* segs, const int* first,
                                                                   const int* active, T* __restrict__ e,
                                                                   const T* __restrict__ r,
-                                                                  const T* __restrict__ G, int nu1, int nu2) {
+                                                                  const double* __restrict__ G, int nu1, int nu2) {
     extern __shared__ double coarse_smem[];
     T* lds = reinterpret_cast<T*>(coarse_smem);
     const int l0 = first[blockIdx.x], l1 = first[blockIdx.x + 1];
@@ -779,29 +781,30 @@ __global__ void __launch_bounds__(kCoarseThreads) k_coarse_vcycle(const CellSeg*
         }
         __syncthreads();
     }
-    // bottom: e = G * r (periodic convolution, natural order)
+    // bottom: e = G * r (periodic convolution, natural order), FP64 in both
+    // precisions as in k_bottom
     {
         const int lb = l1 - 1;
         const CellSeg& sg = segs[lb];
         const int nx = sg.n[0], ny = sg.n[1], nz = sg.n[2];
         const int nc = nx * ny * nz;
         const T* R = rptr(lb);
-        const T* Gb = G + sg.coff;
+        const double* Gb = G + sg.coff;
         T* E = lds + eoff(lb);
         for (int c = tid; c < nc; c += nth) {
             const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
-            T acc = T(0);
+            double acc = 0.0;
             for (int qz = 0; qz < nz; ++qz) {
                 int dz = cz - qz; dz = dz < 0 ? dz + nz : dz;
                 for (int qy = 0; qy < ny; ++qy) {
                     int dy = cy - qy; dy = dy < 0 ? dy + ny : dy;
                     for (int qx = 0; qx < nx; ++qx) {
                         int dxx = cx - qx; dxx = dxx < 0 ? dxx + nx : dxx;
-                        acc += Gb[(dz * ny + dy) * nx + dxx] * R[(qz * ny + qy) * nx + qx];
+                        acc += Gb[(dz * ny + dy) * nx + dxx] * (double)R[(qz * ny + qy) * nx + qx];
                     }
                 }
             }
-            E[c] = acc;
+            E[c] = T(acc);
         }
         __syncthreads();
     }
@@ -1018,6 +1021,16 @@ __global__ void k_check(int nbox, int init, int fixed_iters, int max_iter, doubl
             failed[b] = (more && it >= max_iter) ? 1 : 0;
         }
     }
+}
+
+// After the recorded V-cycles of a captured solve: a box still active has not
+// converged (sticky over the solves of one replay), and the most cycles a box
+// used.  The host clears gstat before each replay.
+__global__ void k_graph_note(int nbox, const int* active, const int* iters, int* gstat) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbox) return;
+    if (active[b]) gstat[b] = 1;
+    gstat[nbox + b] = max(gstat[nbox + b], iters[b]);
 }
 
 }  // namespace nyxpm

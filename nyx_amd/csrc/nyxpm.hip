@@ -119,6 +119,11 @@ void bottom_green(const int n[3], const double dh[3], double* G) {
 
 inline int nblocks(long long n, int per = kBlock) { return (int)((n + per - 1) / per); }
 
+// V-cycles a captured solve records beyond the most a solve of the same call
+// needed when run eagerly (converged boxes skip the extra cycles on the device).
+constexpr int kGraphMargin = 2;
+
+
 }  // namespace
 
 struct nyxpm_ctx {
@@ -126,7 +131,10 @@ struct nyxpm_ctx {
     int verbose = 0;
     int precision = NYXPM_FP64;
     int nu1 = 2, nu2 = 2, max_iter = 200;
-    int graph_cycles = 24;
+    // V-cycles recorded per solve in a captured step: fixed (> 0) or, with 0, the
+    // most cycles of the solves since the call began plus kGraphMargin
+    int graph_cycles = 0;
+    int iters_seen = 0;  // most V-cycles of a box in the solves of the current call
     bool no_fused = false;  // force half-sweep smoothing (for comparisons)
     bool untiled_kick = false;  // force the thread-per-particle kick (for comparisons)
     bool separate_kicks = false;  // nyxpm_pm_steps without the kick / kick-drift pairing
@@ -206,8 +214,9 @@ struct nyxpm_ctx {
     // level-0 fields
     double *own = nullptr, *halo = nullptr, *rho = nullptr, *f = nullptr, *phi = nullptr, *partial = nullptr;
     // MG pools
-    double *r64 = nullptr, *e64 = nullptr, *e64b = nullptr, *G64 = nullptr;
-    float *r32 = nullptr, *e32 = nullptr, *e32b = nullptr, *G32 = nullptr;
+    double *r64 = nullptr, *e64 = nullptr, *e64b = nullptr;
+    float *r32 = nullptr, *e32 = nullptr, *e32b = nullptr;
+    double* G64 = nullptr;  // bottom Green's functions (FP64 in both precisions)
     // per-box device state
     int *active = nullptr, *iters = nullptr, *failed = nullptr, *err_flag = nullptr;
     unsigned long long *nr = nullptr, *nf = nullptr;
@@ -262,7 +271,13 @@ struct nyxpm_ctx {
     hipGraph_t graph = nullptr;
     double g_key[6] = {0, 0, 0, 0, 0, 0};
     int g_prec = -1;
-    int g_mode = -1;  // 0 separate kicks, 1 faithful, 2 pair kicks
+    int g_mode = -1;    // 0 separate kicks, 1 faithful, 2 pair kicks
+    int g_state = -1;   // host state the capture started from (particle buffer, binning flags)
+    int g_cycles = 0;   // V-cycles recorded per solve
+    // per box, over the solves of one replay: [b] 1 if a solve ended still above
+    // its target, [nbox + b] the most V-cycles a solve used (k_graph_note)
+    int* gstat = nullptr;
+    int* h_gstat = nullptr;
 
     long long device_bytes = 0;
 
@@ -781,7 +796,6 @@ int alloc_fields(nyxpm_ctx* c) {
     if ((rc = dalloc(c, &c->e64, c->mg_total))) return rc;
     if ((rc = dalloc(c, &c->e64b, c->mg_total))) return rc;
     if ((rc = dalloc(c, &c->G64, c->G_total))) return rc;
-    if ((rc = dalloc(c, &c->G32, c->G_total))) return rc;
     if ((rc = dalloc(c, &c->active, nbox))) return rc;
     if ((rc = dalloc(c, &c->iters, nbox))) return rc;
     if ((rc = dalloc(c, &c->failed, nbox))) return rc;
@@ -797,6 +811,8 @@ int alloc_fields(nyxpm_ctx* c) {
     if ((rc = dalloc(c, &c->solv, nbox))) return rc;
     if ((rc = dalloc(c, &c->dv_size, 2 * nbox))) return rc;
     if ((rc = dalloc(c, &c->dboxes, nbox))) return rc;
+    if ((rc = dalloc(c, &c->gstat, 2 * nbox))) return rc;
+    HIPCK(c, hipHostMalloc((void**)&c->h_gstat, 2 * nbox * sizeof(int), hipHostMallocDefault));
     HIPCK(c, hipHostMalloc((void**)&c->h_flags, (3 * nbox + 1) * sizeof(int), hipHostMallocDefault));
     HIPCK(c, hipHostMalloc((void**)&c->h_state, 4 * nbox * sizeof(double), hipHostMallocDefault));
     HIPCK(c, hipMemset(c->phi, 0, c->ncell_total * sizeof(double)));
@@ -806,14 +822,11 @@ int alloc_fields(nyxpm_ctx* c) {
     HIPCK(c, hipMemset(c->counts, 0, c->ntiles_total * sizeof(int)));
     // bottom Green's functions
     std::vector<double> G(c->G_total);
-    std::vector<float> G32(c->G_total);
     for (auto& B : c->boxes) {
         const Level& L = B.lev[B.nlev - 1];
         bottom_green(L.n, L.dh, G.data() + B.G_off);
     }
-    for (long long q = 0; q < c->G_total; ++q) G32[q] = (float)G[q];
     HIPCK(c, hipMemcpy(c->G64, G.data(), G.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPCK(c, hipMemcpy(c->G32, G32.data(), G32.size() * sizeof(float), hipMemcpyHostToDevice));
     return NYXPM_OK;
 }
 
@@ -1084,14 +1097,12 @@ struct Pools<double> {
     static double* r(nyxpm_ctx* c) { return c->r64; }
     static double* e(nyxpm_ctx* c) { return c->e64; }
     static double* e2(nyxpm_ctx* c) { return c->e64b; }
-    static double* G(nyxpm_ctx* c) { return c->G64; }
 };
 template <>
 struct Pools<float> {
     static float* r(nyxpm_ctx* c) { return c->r32; }
     static float* e(nyxpm_ctx* c) { return c->e32; }
     static float* e2(nyxpm_ctx* c) { return c->e32b; }
-    static float* G(nyxpm_ctx* c) { return c->G32; }
 };
 
 template <typename T, int MODE>
@@ -1171,12 +1182,12 @@ int vcycle(nyxpm_ctx* c, bool* updated, bool sweep0_done = false) {
     if (c->per_level_coarse) {
         Timed t(c, "bottom");
         hipLaunchKernelGGL(k_bottom<T>, dim3(c->t_bottom.count), dim3(kBlock), 0, c->stream, c->segs(c->t_bottom),
-                           c->t_bottom.count, c->active, eb[0], r, Pools<T>::G(c));
+                           c->t_bottom.count, c->active, eb[0], r, c->G64);
     } else {
         Timed t(c, "coarse");
         hipLaunchKernelGGL(k_coarse_vcycle<T>, dim3((unsigned)c->boxes.size()), dim3(kCoarseThreads),
                            c->cc_lds[sizeof(T) == 8 ? 0 : 1], c->stream, c->segs(c->t_cc), c->cc_first, c->active,
-                           eb[0], r, Pools<T>::G(c), c->nu1, c->nu2);
+                           eb[0], r, c->G64, c->nu1, c->nu2);
     }
     for (int d = D - 2; d >= 0; --d) {
         auto& tf = c->t_full[d];
@@ -1305,6 +1316,11 @@ int solve_impl(nyxpm_ctx* c, double rel, double abs_tol, int fixed_iters, int gr
             if ((rc = res(false))) return rc;
             if ((rc = check(c, false, fixed_iters, rel, abs_tol))) return rc;
         }
+        // a box still active now did not converge within the recorded cycles:
+        // kept in gstat, which the next solve of the replay does not reset
+        hipLaunchKernelGGL(k_graph_note, dim3(nblocks(nbox, 64)), dim3(64), 0, c->stream, nbox, c->active,
+                           c->iters, c->gstat);
+        LAUNCHCK(c);
         return NYXPM_OK;
     }
     const int limit = fixed_iters > 0 ? fixed_iters : c->max_iter;
@@ -1341,6 +1357,7 @@ int solve_results(nyxpm_ctx* c, int* iters, double* final_res, bool allow_active
     HIPCK(c, hipStreamSynchronize(c->stream));
     int bad = -1;
     for (int b = 0; b < nbox; ++b) {
+        c->iters_seen = std::max(c->iters_seen, c->h_flags[nbox + b]);
         if (iters) iters[b] = c->h_flags[nbox + b];
         if (final_res) final_res[b] = c->h_state[4 * b + 2];
         if (c->h_flags[2 * nbox + b] || (!allow_active && c->h_flags[b])) bad = b;
@@ -1563,6 +1580,7 @@ int nyxpm_create_batch(nyxpm_ctx** out, int nbox, const nyxpm_geom* geoms, const
         if (opts->kick_chunk > 0) c->kick_chunk = std::max(kBlock, opts->kick_chunk);
         if (opts->phi_guess >= 0 && opts->phi_guess <= 2) c->phi_guess = opts->phi_guess;
         c->fuse_res = opts->residual == 1;
+        if (opts->graph_cycles > 0) c->graph_cycles = opts->graph_cycles;
         if (opts->resort_slack < 0) {
             c->slack_num = 0;
             c->slack_pad = 0;
@@ -1606,11 +1624,13 @@ int nyxpm_destroy(nyxpm_ctx* c) {
     dfree(c->own); dfree(c->halo); dfree(c->f); dfree(c->phi); dfree(c->partial);
 
     dfree(c->r64); dfree(c->e64); dfree(c->e64b); dfree(c->G64);
-    dfree(c->r32); dfree(c->e32); dfree(c->e32b); dfree(c->G32);
+    dfree(c->r32); dfree(c->e32); dfree(c->e32b);
     dfree(c->active); dfree(c->iters); dfree(c->failed); dfree(c->err_flag); dfree(c->nitems); dfree(c->nr); dfree(c->nf);
     dfree(c->ndefer); dfree(c->dtmin); dfree(c->nheavy);
     dfree(c->state); dfree(c->solv); dfree(c->dv_size); dfree(c->dboxes);
     dfree(c->seg_pool); dfree(c->pseg); dfree(c->tseg); dfree(c->cc_first);
+    dfree(c->gstat);
+    if (c->h_gstat) hipHostFree(c->h_gstat);
     if (c->h_flags) hipHostFree(c->h_flags);
     if (c->h_state) hipHostFree(c->h_state);
     for (auto& kv : c->pending)
@@ -1914,7 +1934,7 @@ int nyxpm_particles_download(nyxpm_ctx* c, int box, double* x, double* y, double
         hipLaunchKernelGGL(k_unsort, dim3(nblocks(B.span)), dim3(kBlock), 0, c->stream, B.part_off, B.span, A.x,
                            A.id, src[q], tmp);
         LAUNCHCK(c);
-        HIPCK(c, hipMemcpyAsync(dst[q], tmp, B.np * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCK(c, hipMemcpyAsync(dst[q], tmp, B.np * sizeof(double), hipMemcpyDefault, c->stream));
         HIPCK(c, hipStreamSynchronize(c->stream));
     }
     return err_check(c);
@@ -1936,13 +1956,50 @@ int nyxpm_assign_density(nyxpm_ctx* c, int box, double* rho_out) {
     return err_check(c);
 }
 
+// AssignDensityAndVels into `out` (4 fields of box `box`'s cells, device memory
+// when dst_dev, else written through the device buffer dst) with the given
+// scratch: sums (4 N: folded mass and momenta of every box), mown (N) and mhalo
+// (H) for the momentum deposit, maxbits (3 per box).  Stream-ordered; the
+// caller synchronises.
+static int density_and_vels_impl(nyxpm_ctx* c, int box, double* dst, double* sums, double* mown, double* mhalo,
+                                 unsigned long long* maxbits) {
+    PartPtrs& A = c->P[c->cur];
+    const long long N = c->ncell_total;
+    const int nbox = (int)c->boxes.size();
+    const size_t lds = 2 * sizeof(unsigned long long) * (c->tile[0] + 1) * (c->tile[1] + 1) * (c->tile[2] + 1);
+    hipLaunchKernelGGL(k_deposit, dim3(c->heavy_cap + c->tseg_blocks), dim3(kBlock), lds, c->stream, c->tseg,
+                       c->tseg_n, c->dboxes, c->tbase, c->cursor, c->heavy, c->nheavy, c->heavy_cap, A.x, A.y, A.z,
+                       A.m, c->own, c->halo, c->err_flag);
+    hipLaunchKernelGGL(k_fold_raw, dim3(c->t_res.blocks), dim3(kBlock), 0, c->stream, c->segs(c->t_res),
+                       c->t_res.count, c->dboxes, c->own, c->halo, sums);
+    // the per-box maxima start at zero on every call (the scratch keeps the last
+    // call's contents)
+    hipLaunchKernelGGL(k_fill_dbl, dim3(1), dim3(kBlock), 0, c->stream, reinterpret_cast<double*>(maxbits),
+                       (long long)(3 * nbox), 0.0);
+    if (c->pseg_blocks > 0)
+        hipLaunchKernelGGL(k_max_mom, dim3(c->pseg_blocks), dim3(kBlock), 0, c->stream, c->pseg, c->pseg_n,
+                           c->dboxes, A.x, A.m, A.vx, A.vy, A.vz, maxbits);
+    const double* vel[3] = {A.vx, A.vy, A.vz};
+    for (int d = 0; d < 3; ++d) {
+        hipLaunchKernelGGL(k_deposit_mom, dim3(c->tseg_blocks), dim3(kBlock), lds, c->stream, c->tseg, c->tseg_n,
+                           c->dboxes, c->tbase, c->cursor, A.x, A.y, A.z, A.m, vel[d], maxbits, d, mown, mhalo,
+                           c->err_flag);
+        hipLaunchKernelGGL(k_fold_raw, dim3(c->t_res.blocks), dim3(kBlock), 0, c->stream, c->segs(c->t_res),
+                           c->t_res.count, c->dboxes, mown, mhalo, sums + (d + 1) * N);
+    }
+    const long long nc = c->boxes[box].lev[0].nc;
+    hipLaunchKernelGGL(k_vels_finish, dim3(nblocks(nc)), dim3(kBlock), 0, c->stream, c->dboxes, box, sums,
+                       sums + N, N, dst);
+    LAUNCHCK(c);
+    return NYXPM_OK;
+}
+
 int nyxpm_assign_density_and_vels(nyxpm_ctx* c, int box, double* out) {
     if (!c || !out) return NYXPM_ERR_ARG;
     int rc;
     if ((rc = require_particles(c))) return rc;
     if ((rc = check_box(c, box))) return rc;
     if (!c->sorted_valid && (rc = phase_sort(c))) return rc;
-    PartPtrs& A = c->P[c->cur];
     const long long N = c->ncell_total;
     const long long H = (long long)c->ntiles_total * c->halo_stride;
     const int nbox = (int)c->boxes.size();
@@ -1964,37 +2021,17 @@ int nyxpm_assign_density_and_vels(nyxpm_ctx* c, int box, double* out) {
         release();
         return rc;
     }
-    const size_t lds = 2 * sizeof(unsigned long long) * (c->tile[0] + 1) * (c->tile[1] + 1) * (c->tile[2] + 1);
-    hipLaunchKernelGGL(k_deposit, dim3(c->heavy_cap + c->tseg_blocks), dim3(kBlock), lds, c->stream, c->tseg,
-                       c->tseg_n, c->dboxes, c->tbase, c->cursor, c->heavy, c->nheavy, c->heavy_cap, A.x, A.y, A.z,
-                       A.m, c->own, c->halo, c->err_flag);
-    hipLaunchKernelGGL(k_fold_raw, dim3(c->t_res.blocks), dim3(kBlock), 0, c->stream, c->segs(c->t_res),
-                       c->t_res.count, c->dboxes, c->own, c->halo, sums);
-    hipLaunchKernelGGL(k_fill_dbl, dim3(1), dim3(kBlock), 0, c->stream, reinterpret_cast<double*>(maxbits),
-                       (long long)(3 * nbox), 0.0);
-    if (c->pseg_blocks > 0)
-        hipLaunchKernelGGL(k_max_mom, dim3(c->pseg_blocks), dim3(kBlock), 0, c->stream, c->pseg, c->pseg_n,
-                           c->dboxes, A.x, A.m, A.vx, A.vy, A.vz, maxbits);
-    const double* vel[3] = {A.vx, A.vy, A.vz};
-    for (int d = 0; d < 3; ++d) {
-        hipLaunchKernelGGL(k_deposit_mom, dim3(c->tseg_blocks), dim3(kBlock), lds, c->stream, c->tseg, c->tseg_n,
-                           c->dboxes, c->tbase, c->cursor, A.x, A.y, A.z, A.m, vel[d], maxbits, d, mown, mhalo,
-                           c->err_flag);
-        hipLaunchKernelGGL(k_fold_raw, dim3(c->t_res.blocks), dim3(kBlock), 0, c->stream, c->segs(c->t_res),
-                           c->t_res.count, c->dboxes, mown, mhalo, sums + (d + 1) * N);
-    }
-    const Box& B = c->boxes[box];
-    const long long nc = B.lev[0].nc;
+    const long long nc = c->boxes[box].lev[0].nc;
     const bool dev = is_device_ptr(out);
     double* dst = out;
     if (!dev && (rc = dalloc(c, &dst, 4 * nc))) {
         release();
         return rc;
     }
-    hipLaunchKernelGGL(k_vels_finish, dim3(nblocks(nc)), dim3(kBlock), 0, c->stream, c->dboxes, box, sums,
-                       sums + N, N, dst);
-    hipError_t e = hipGetLastError();
-    if (!dev && e == hipSuccess) e = hipMemcpyAsync(out, dst, 4 * nc * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    rc = density_and_vels_impl(c, box, dst, sums, mown, mhalo, maxbits);
+    hipError_t e = rc ? hipSuccess : hipGetLastError();
+    if (!rc && !dev && e == hipSuccess)
+        e = hipMemcpyAsync(out, dst, 4 * nc * sizeof(double), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (!dev) {
         dfree(dst);
@@ -2004,6 +2041,7 @@ int nyxpm_assign_density_and_vels(nyxpm_ctx* c, int box, double* out) {
     // the deposit buffers now hold raw mass, not rho
     c->density_valid = false;
     c->f1_valid = false;
+    if (rc) return rc;
     if (e != hipSuccess) FAIL(c, NYXPM_ERR_HIP, "assign_density_and_vels: %s", hipGetErrorString(e));
     return err_check(c);
 }
@@ -2246,6 +2284,7 @@ int nyxpm_pm_steps(nyxpm_ctx* c, int nsteps, double dt, double a_old, double a_n
         if ((rc = ensure_guess_buffer(c))) return rc;
         c->inc_suspended = true;
     }
+    c->iters_seen = 0;
     if (fused && nsteps > 0 && (rc = phase_kick<kKickDrift>(c, dt, a_old, a_half))) return rc;
     auto one_step = [&](int s) -> int {
         // every step of this call runs from a_old to a_new
@@ -2254,8 +2293,10 @@ int nyxpm_pm_steps(nyxpm_ctx* c, int nsteps, double dt, double a_old, double a_n
         return step_body(c, dt, a_old, a_new, four_pi_G, rel_tol, abs_tol, nyx_faithful, 0);
     };
     for (int s = 0; s < nsteps; ++s) {
-        // a replay covers two steps; in the fused form both must end in a pair kick
-        const bool replay = use_graph && (fused ? s + 2 < nsteps : s + 1 < nsteps);
+        // a replay covers two steps; in the fused form both must end in a pair
+        // kick.  The first step of a call runs eagerly: its V-cycle count sets
+        // the cycles recorded in the graph.
+        const bool replay = use_graph && s >= 1 && (fused ? s + 2 < nsteps : s + 1 < nsteps);
         if (!replay) {
             if ((rc = one_step(s))) return rc;
             if ((rc = solve_results(c, iters, nullptr, false))) return rc;
@@ -2263,8 +2304,14 @@ int nyxpm_pm_steps(nyxpm_ctx* c, int nsteps, double dt, double a_old, double a_n
         }
         const double key[6] = {dt, a_old, a_new, four_pi_G, rel_tol, abs_tol};
         const int mode = fused ? 2 : nyx_faithful ? 1 : 0;
-        bool same = c->gexec && c->g_prec == c->precision && c->g_mode == mode &&
-                    std::memcmp(key, c->g_key, sizeof(key)) == 0;
+        // the launches a capture records depend on which particle buffer is live
+        // and on the binning flags; a replay is valid only from the same state
+        const int state = c->cur | (c->keys_valid ? 2 : 0) | (c->items_valid ? 4 : 0) | (c->sorted_valid ? 8 : 0) |
+                          (c->moves_pending ? 16 : 0);
+        const int cycles = c->graph_cycles > 0 ? c->graph_cycles
+                                               : std::min(c->max_iter, std::max(2, c->iters_seen + kGraphMargin));
+        bool same = c->gexec && c->g_prec == c->precision && c->g_mode == mode && c->g_state == state &&
+                    c->g_cycles >= cycles && std::memcmp(key, c->g_key, sizeof(key)) == 0;
         if (!same) {
             if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
             if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
@@ -2277,9 +2324,8 @@ int nyxpm_pm_steps(nyxpm_ctx* c, int nsteps, double dt, double a_old, double a_n
             HIPCK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             int cur0 = c->cur;
             for (int r = 0; r < 2 && !rc; ++r)
-                rc = fused ? step_tail(c, dt, a_old, a_new, four_pi_G, rel_tol, abs_tol, c->graph_cycles, a_old, a_new)
-                           : step_body(c, dt, a_old, a_new, four_pi_G, rel_tol, abs_tol, nyx_faithful,
-                                       c->graph_cycles);
+                rc = fused ? step_tail(c, dt, a_old, a_new, four_pi_G, rel_tol, abs_tol, cycles, a_old, a_new)
+                           : step_body(c, dt, a_old, a_new, four_pi_G, rel_tol, abs_tol, nyx_faithful, cycles);
             hipGraph_t g = nullptr;
             hipError_t ee = hipStreamEndCapture(c->stream, &g);
             c->timing = was_timing;
@@ -2291,11 +2337,24 @@ int nyxpm_pm_steps(nyxpm_ctx* c, int nsteps, double dt, double a_old, double a_n
             std::memcpy(c->g_key, key, sizeof(key));
             c->g_prec = c->precision;
             c->g_mode = mode;
+            c->g_state = state;
+            c->g_cycles = cycles;
         }
+        const int nbox = (int)c->boxes.size();
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(kBlock), 0, c->stream, c->gstat, 2LL * nbox, 0);
         HIPCK(c, hipGraphLaunch(c->gexec, c->stream));
         ++s;
-        // a box still active after the recorded cycles has not converged
-        if ((rc = solve_results(c, iters, nullptr, false))) return rc;
+        // every solve of the replay: a box still active after the recorded cycles
+        // has not converged (the last solve's flags alone would hide an earlier one)
+        HIPCK(c, hipMemcpyAsync(c->h_gstat, c->gstat, 2 * nbox * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        if ((rc = solve_results(c, iters, nullptr, true))) return rc;
+        for (int b = 0; b < nbox; ++b) {
+            c->iters_seen = std::max(c->iters_seen, c->h_gstat[nbox + b]);
+            if (c->h_gstat[b])
+                FAIL(c, NYXPM_ERR_NOT_CONVERGED,
+                     "box %d: multigrid still above its target after the %d V-cycles recorded per solve in the "
+                     "captured step", b, c->g_cycles);
+        }
     }
     c->sorted_valid = false;
     c->keys_valid = false;

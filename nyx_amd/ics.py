@@ -178,3 +178,53 @@ def grf_lattice_device(n, seed, index=-1.5, rms_cells=1.0, vel_factor=0.1, devic
     if str(device).startswith("cuda"):
         torch.cuda.empty_cache()  # hand the FFT temporaries back before the library allocates
     return pos, vel, 1.0 / n ** 3
+
+
+def plummer_clustered_device(n, seed, n_spheres=2000, frac_uniform=0.2, rs_cells=(1.0, 8.0), trunc=16.0,
+                             vel_sigma=0.01, device="cuda"):
+    """plummer_clustered generated on the GPU (torch), for the 512^3 workload's
+    full-size tests: n^3 particles, frac_uniform of them uniform in the unit box,
+    the rest in equal shares over n_spheres Plummer spheres with uniform centres
+    and scale radii log-uniform on rs_cells (in cells), radii from the inverse
+    Plummer CDF truncated at trunc scale radii, isotropic directions, positions
+    wrapped periodically; velocities i.i.d. N(0, vel_sigma^2).  The draws come
+    from torch's device generator seeded with `seed` (the realisation differs
+    from the numpy one).  Returns ([x, y, z], [vx, vy, vz], m) as FP64 device
+    tensors of n^3 elements."""
+    import torch
+
+    N = n ** 3
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed))
+    f64 = torch.float64
+    n_uni = int(round(frac_uniform * N))
+    n_clu = N - n_uni
+    centres = torch.rand((n_spheres, 3), generator=g, dtype=f64, device=device)
+    lo, hi = np.log(rs_cells[0]), np.log(rs_cells[1])
+    rs = torch.exp(lo + (hi - lo) * torch.rand(n_spheres, generator=g, dtype=f64, device=device)) / n
+    per = torch.full((n_spheres,), n_clu // n_spheres, dtype=torch.int64, device=device)
+    per[: n_clu - int(per.sum().item())] += 1
+    owner = torch.repeat_interleave(torch.arange(n_spheres, device=device), per)
+    umax = trunc ** 3 / (trunc ** 2 + 1.0) ** 1.5  # enclosed mass fraction at trunc * r_s
+    uu = (torch.rand(n_clu, generator=g, dtype=f64, device=device) * umax).clamp_min(1e-300)
+    rad = rs[owner] / torch.sqrt(uu.pow_(-2.0 / 3.0) - 1.0)
+    del uu
+    cth = torch.rand(n_clu, generator=g, dtype=f64, device=device) * 2.0 - 1.0
+    ph = torch.rand(n_clu, generator=g, dtype=f64, device=device) * (2.0 * np.pi)
+    sth = torch.sqrt(1.0 - cth * cth)
+    off = (rad * sth * torch.cos(ph), rad * sth * torch.sin(ph), rad * cth)
+    del rad, cth, ph, sth
+    pos = []
+    for d in range(3):
+        x = torch.empty(N, dtype=f64, device=device)
+        x[:n_uni] = torch.rand(n_uni, generator=g, dtype=f64, device=device)
+        c = torch.remainder(centres[owner, d] + off[d], 1.0)
+        c[c >= 1.0] = 0.0
+        x[n_uni:] = c
+        del c
+        pos.append(x)
+    del off, owner
+    vel = [torch.randn(N, generator=g, dtype=f64, device=device) * vel_sigma for _ in range(3)]
+    if str(device).startswith("cuda"):
+        torch.cuda.empty_cache()
+    return pos, vel, 1.0 / N

@@ -60,7 +60,7 @@ class PMContext:
     def __init__(self, geoms, precision="fp64", tile=None, nu1=2, nu2=2, max_iter=200, smoother="fused",
                  kick="tiled", sweep_shape=None, sweep_zchunk=0, kick_chunk=0, pair_kicks=True,
                  coarse="workgroup", particle_cfl=0.0, resort_slack=0.0, phi_guess="default",
-                 residual="separate"):
+                 residual="separate", graph_cycles=0):
         """resort_slack: free slots per particle each tile gets at a full sort, for
         the incremental re-sort after kick-drifts (0: library default 0.25;
         negative: a full sort after every kick-drift).  phi_guess: initial guess of
@@ -91,6 +91,7 @@ class PMContext:
         o.resort_slack = float(resort_slack)
         o.phi_guess = {"default": 0, "previous": 1, "extrapolate": 2}[phi_guess]
         o.residual = 1 if residual == "in_sweep" else 0
+        o.graph_cycles = int(graph_cycles)
         if tile is not None:
             for d in range(3):
                 o.tile[d] = int(tile[d])
@@ -159,6 +160,12 @@ class PMContext:
 
     def num_particles(self, box=0):
         return int(self.L.nyxpm_num_particles(self.h, box))
+
+    def download_device(self, addrs, box=0):
+        """Particles of box `box` in upload order into six device arrays
+        (x, y, z, vx, vy, vz addresses; None skips a component)."""
+        self._ck(self.L.nyxpm_particles_download(self.h, box, *[ctypes.c_void_p(int(a)) if a else None
+                                                                for a in addrs]))
 
     def download(self, box=0):
         n = self.num_particles(box)
