@@ -150,6 +150,23 @@ int nyxpm_particles_upload_device(nyxpm_ctx *ctx, int box, int64_t np,
  * by file size) and uploads it as nyxpm_particles_upload does (per-particle
  * mass).  np_out (may be NULL) receives NP. */
 int nyxpm_particles_load_binary(nyxpm_ctx *ctx, int box, const char *path, int64_t *np_out);
+/* DMPC->InitFromAsciiFile(file, AMREX_SPACEDIM + 1) (NyxParticles.cpp:506-521)
+ * for box `box`: Nyx's ASCII particle file (the count, then x y z m vx vy vz per
+ * particle, whitespace separated), uploaded with per-particle masses; particles
+ * outside the box are shifted by whole periods into it.  np_out may be NULL. */
+int nyxpm_particles_load_ascii(nyxpm_ctx *ctx, int box, const char *path, int64_t *np_out);
+/* NyxParticleContainer::NyxCheckpoint (is_checkpoint != 0) / WriteNyxPlotFile
+ * (NyxParticleContainer.H:497-517, via Nyx_output.cpp:652, 735): box `box`'s
+ * particles as AMReX's particle checkpoint or plotfile <dir>/<name>/{Header,
+ * Level_0/Particle_H, Level_0/DATA_00000} (one level, one grid, FP64 reals x y z
+ * mass xvel yvel zvel, checkpoints with int32 id and cpu first; plotfiles also
+ * get Nyx's <dir>/<name>/precision file), in upload order with ids 1..np. */
+int nyxpm_particles_write_amrex(nyxpm_ctx *ctx, int box, const char *dir, const char *name, int is_checkpoint);
+/* DMPC->Restart(dir, name) (NyxParticles.cpp:586, 932) for box `box`: reads an
+ * AMReX particle checkpoint or plotfile of one level (any grids and DATA files,
+ * FP64 or FP32 reals; the first four extra reals are mass and velocity) and
+ * uploads it; particles outside the box are shifted into it.  np_out may be NULL. */
+int nyxpm_particles_read_amrex(nyxpm_ctx *ctx, int box, const char *dir, const char *name, int64_t *np_out);
 /* Copy box `box`'s particles to host or device arrays (unified addressing), in
  * upload (id) order.  Any pointer may be NULL to skip that component. */
 int nyxpm_particles_download(nyxpm_ctx *ctx, int box,

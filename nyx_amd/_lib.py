@@ -66,6 +66,9 @@ SIGNATURES = [
     ("nyxpm_particles_upload", _i, [_vp, _i, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d]),
     ("nyxpm_particles_upload_device", _i, [_vp, _i, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d]),
     ("nyxpm_particles_load_binary", _i, [_vp, _i, _s, _vp]),
+    ("nyxpm_particles_load_ascii", _i, [_vp, _i, _s, _vp]),
+    ("nyxpm_particles_write_amrex", _i, [_vp, _i, _s, _s, _i]),
+    ("nyxpm_particles_read_amrex", _i, [_vp, _i, _s, _s, _vp]),
     ("nyxpm_particles_download", _i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("nyxpm_num_particles", _i64, [_vp, _i]),
     ("nyxpm_assign_density", _i, [_vp, _i, _vp]),

@@ -12,6 +12,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <sys/stat.h>
+
+#include <cerrno>
 #include <map>
 #include <set>
 #include <string>
@@ -1938,6 +1941,253 @@ int nyxpm_particles_download(nyxpm_ctx* c, int box, double* x, double* y, double
         HIPCK(c, hipStreamSynchronize(c->stream));
     }
     return err_check(c);
+}
+
+// ---------------------------------------------------------------------------
+// particle files
+// ---------------------------------------------------------------------------
+
+// One component of box `box`'s particles (an array of the live buffer) into a
+// host or device array, in upload (id) order.
+static int download_component(nyxpm_ctx* c, int box, const double* src, double* dst) {
+    const Box& B = c->boxes[box];
+    double* tmp = c->P[1 - c->cur].x;  // the other particle buffer is free between phases
+    hipLaunchKernelGGL(k_unsort, dim3(nblocks(B.span)), dim3(kBlock), 0, c->stream, B.part_off, B.span,
+                       c->P[c->cur].x, c->P[c->cur].id, src, tmp);
+    LAUNCHCK(c);
+    HIPCK(c, hipMemcpyAsync(dst, tmp, B.np * sizeof(double), hipMemcpyDefault, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    return NYXPM_OK;
+}
+
+// After an upload of positions that may lie outside the box: the periodic shift
+// of RedistributeLocal (positions more than two periods away are flagged).
+static int shift_into_box(nyxpm_ctx* c) {
+    PartPtrs& A = c->P[c->cur];
+    hipLaunchKernelGGL(k_wrap, dim3(c->pseg_blocks), dim3(kBlock), 0, c->stream, c->pseg, c->pseg_n, c->dboxes, A.x,
+                       A.y, A.z, c->err_flag);
+    LAUNCHCK(c);
+    c->keys_valid = c->sorted_valid = c->density_valid = false;
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    return err_check(c);
+}
+
+// Nyx's ASCII particle file (particle_init_type = AsciiFile, read by
+// DMPC->InitFromAsciiFile(file, AMREX_SPACEDIM + 1), NyxParticles.cpp:506-521):
+// the particle count, then per particle its position and the AMREX_SPACEDIM + 1
+// extra values, mass and velocity (rdata(0..3), DarkMatterParticleContainer.H:
+// 14-18), all whitespace separated.  A particle outside the box is shifted by
+// whole periods into it, as AMReX's reader does before it places a particle.
+int nyxpm_particles_load_ascii(nyxpm_ctx* c, int box, const char* path, int64_t* np_out) {
+    if (!c || !path) return NYXPM_ERR_ARG;
+    int rc;
+    if ((rc = check_box(c, box))) return rc;
+    FILE* f = std::fopen(path, "rb");
+    if (!f) FAIL(c, NYXPM_ERR_ARG, "cannot open %s", path);
+    std::string text;
+    {
+        char buf[1 << 16];
+        size_t got;
+        while ((got = std::fread(buf, 1, sizeof(buf), f)) > 0) text.append(buf, got);
+        std::fclose(f);
+    }
+    const char* q = text.c_str();
+    char* end = nullptr;
+    const long long np = std::strtoll(q, &end, 10);
+    if (end == q || np < 0 || np > (long long)0x7fffffff) FAIL(c, NYXPM_ERR_ARG, "%s: no particle count", path);
+    q = end;
+    std::vector<double> a(7 * (size_t)np);
+    for (long long p = 0; p < np; ++p)
+        for (int v = 0; v < 7; ++v) {  // x y z m vx vy vz
+            const double d = std::strtod(q, &end);
+            if (end == q)
+                FAIL(c, NYXPM_ERR_ARG, "%s: particle %lld of %lld: value %d is missing or not a number", path, p,
+                     np, v);
+            q = end;
+            a[(size_t)v * np + p] = d;
+        }
+    const double* A = a.data();
+    rc = upload_impl(c, box, np, A, A + np, A + 2 * np, A + 4 * np, A + 5 * np, A + 6 * np, A + 3 * np, 0.0,
+                     hipMemcpyHostToDevice);
+    if (rc) return rc;
+    if (np > 0 && (rc = shift_into_box(c))) return rc;
+    if (np_out) *np_out = np;
+    return NYXPM_OK;
+}
+
+static int make_dir(nyxpm_ctx* c, const std::string& d) {
+    if (mkdir(d.c_str(), 0755) != 0 && errno != EEXIST) FAIL(c, NYXPM_ERR_ARG, "cannot create directory %s", d.c_str());
+    return NYXPM_OK;
+}
+
+// AMReX's particle checkpoint / plotfile of the dark-matter container, as Nyx
+// writes it through NyxParticleContainer::NyxCheckpoint (is_checkpoint = 1) and
+// WriteNyxPlotFile (0) (NyxParticleContainer.H:497-517; called by
+// Nyx::particle_check_point and Nyx::particle_plot_file, Nyx_output.cpp:652,
+// 735), one level, one grid:
+//   <dir>/<name>/Header            text: version (real type as its last "_"
+//                                  token), dim, extra real count and names
+//                                  (mass xvel yvel zvel), extra int count,
+//                                  is_checkpoint, particle count, next id,
+//                                  finest level, grids per level, then per grid
+//                                  "file count offset"
+//   <dir>/<name>/Level_0/Particle_H  the BoxArray of the level
+//   <dir>/<name>/Level_0/DATA_00000  checkpoint: count x (id, cpu) int32 first;
+//                                  then count records of x y z mass xvel yvel
+//                                  zvel, native-endian FP64
+// and for a plotfile Nyx's <dir>/<name>/precision ("NATIVE", Nyx_output.cpp:
+// 677-688).  Particles are written in upload order with ids 1..np, cpu 0.
+int nyxpm_particles_write_amrex(nyxpm_ctx* c, int box, const char* dir, const char* name, int is_checkpoint) {
+    if (!c || !dir || !name) return NYXPM_ERR_ARG;
+    int rc;
+    if ((rc = check_box(c, box))) return rc;
+    if ((rc = require_particles(c))) return rc;
+    const Box& B = c->boxes[box];
+    const long long np = B.np;
+    std::vector<double> cols(7 * (size_t)np);  // x y z m vx vy vz
+    if (np > 0) {
+        if ((rc = ensure_idle_buffer(c))) return rc;
+        PartPtrs& A = c->P[c->cur];
+        const double* src[7] = {A.x, A.y, A.z, A.m, A.vx, A.vy, A.vz};
+        for (int v = 0; v < 7; ++v) {
+            double* dst = cols.data() + (size_t)v * np;
+            if (v == 3 && !A.m) {
+                std::fill(dst, dst + np, B.m_const);
+                continue;
+            }
+            if ((rc = download_component(c, box, src[v], dst))) return rc;
+        }
+        if ((rc = err_check(c))) return rc;
+    }
+    const std::string top = std::string(dir), base = top + "/" + name, lev = base + "/Level_0";
+    if ((rc = make_dir(c, top)) || (rc = make_dir(c, base)) || (rc = make_dir(c, lev))) return rc;
+    {
+        FILE* h = std::fopen((base + "/Header").c_str(), "w");
+        if (!h) FAIL(c, NYXPM_ERR_ARG, "cannot write %s/Header", base.c_str());
+        std::fprintf(h, "Version_Two_Dot_Zero_double\n3\n4\nmass\nxvel\nyvel\nzvel\n0\n%d\n%lld\n%lld\n0\n1\n%d %lld %d\n",
+                     is_checkpoint ? 1 : 0, np, np + 1, 0, np, 0);
+        if (std::fclose(h) != 0) FAIL(c, NYXPM_ERR_ARG, "cannot write %s/Header", base.c_str());
+    }
+    {
+        FILE* h = std::fopen((lev + "/Particle_H").c_str(), "w");
+        if (!h) FAIL(c, NYXPM_ERR_ARG, "cannot write %s/Particle_H", lev.c_str());
+        std::fprintf(h, "(1 0\n((0,0,0) (%d,%d,%d) (0,0,0))\n)\n", B.n[0] - 1, B.n[1] - 1, B.n[2] - 1);
+        std::fclose(h);
+    }
+    {
+        FILE* d = std::fopen((lev + "/DATA_00000").c_str(), "wb");
+        if (!d) FAIL(c, NYXPM_ERR_ARG, "cannot write %s/DATA_00000", lev.c_str());
+        bool ok = true;
+        if (is_checkpoint) {
+            std::vector<int> ids(2 * (size_t)np);
+            for (long long p = 0; p < np; ++p) {
+                ids[2 * p] = (int)(p + 1);
+                ids[2 * p + 1] = 0;
+            }
+            ok = std::fwrite(ids.data(), sizeof(int), ids.size(), d) == ids.size();
+        }
+        std::vector<double> rec(7 * 65536);
+        for (long long p0 = 0; ok && p0 < np; p0 += 65536) {
+            const long long cnt = std::min(65536LL, np - p0);
+            for (long long q = 0; q < cnt; ++q)
+                for (int v = 0; v < 7; ++v) rec[7 * q + v] = cols[(size_t)v * np + p0 + q];
+            ok = std::fwrite(rec.data(), sizeof(double), 7 * cnt, d) == (size_t)(7 * cnt);
+        }
+        if (std::fclose(d) != 0) ok = false;
+        if (!ok) FAIL(c, NYXPM_ERR_ARG, "cannot write %s/DATA_00000", lev.c_str());
+    }
+    if (!is_checkpoint) {
+        FILE* h = std::fopen((base + "/precision").c_str(), "w");
+        if (!h) FAIL(c, NYXPM_ERR_ARG, "cannot write %s/precision", base.c_str());
+        std::fprintf(h, "NATIVE\n");
+        std::fclose(h);
+    }
+    return NYXPM_OK;
+}
+
+// DMPC->Restart(dir, name) (NyxParticles.cpp:586, 932): reads the AMReX particle
+// checkpoint or plotfile written as above (any number of grids and DATA files on
+// level 0, FP64 or FP32 reals, extra int components skipped) into box `box`.
+// The first four extra reals are taken as mass, xvel, yvel, zvel, the order
+// Nyx's containers name them (DarkMatterParticleContainer.H:14-18).
+int nyxpm_particles_read_amrex(nyxpm_ctx* c, int box, const char* dir, const char* name, int64_t* np_out) {
+    if (!c || !dir || !name) return NYXPM_ERR_ARG;
+    int rc;
+    if ((rc = check_box(c, box))) return rc;
+    const std::string base = std::string(dir) + "/" + name;
+    FILE* h = std::fopen((base + "/Header").c_str(), "r");
+    if (!h) FAIL(c, NYXPM_ERR_ARG, "cannot open %s/Header", base.c_str());
+    struct Closer {
+        FILE* f;
+        ~Closer() { std::fclose(f); }
+    } closer{h};
+    char ver[256] = {0};
+    int dim = 0, nre = 0, nie = 0, chk = 0, finest = 0, ngrids = 0;
+    long long np = 0, next_id = 0;
+    if (std::fscanf(h, "%255s %d %d", ver, &dim, &nre) != 3) FAIL(c, NYXPM_ERR_ARG, "%s/Header: bad start", base.c_str());
+    const char* us = std::strrchr(ver, '_');
+    const int rsize = us && !std::strcmp(us, "_double") ? 8 : (us && !std::strcmp(us, "_single") ? 4 : 0);
+    if (!rsize) FAIL(c, NYXPM_ERR_ARG, "%s/Header: real type of '%s' not recognised", base.c_str(), ver);
+    if (dim != 3 || nre < 4) FAIL(c, NYXPM_ERR_ARG, "%s/Header: dim %d with %d extra reals (need 3, >= 4)",
+                                  base.c_str(), dim, nre);
+    for (int q = 0; q < nre; ++q) {
+        char nm[256];
+        if (std::fscanf(h, "%255s", nm) != 1) FAIL(c, NYXPM_ERR_ARG, "%s/Header: real names", base.c_str());
+    }
+    if (std::fscanf(h, "%d", &nie) != 1) FAIL(c, NYXPM_ERR_ARG, "%s/Header: int count", base.c_str());
+    for (int q = 0; q < nie; ++q) {
+        char nm[256];
+        if (std::fscanf(h, "%255s", nm) != 1) FAIL(c, NYXPM_ERR_ARG, "%s/Header: int names", base.c_str());
+    }
+    if (std::fscanf(h, "%d %lld %lld %d", &chk, &np, &next_id, &finest) != 4)
+        FAIL(c, NYXPM_ERR_ARG, "%s/Header: counts", base.c_str());
+    if (finest != 0) FAIL(c, NYXPM_ERR_UNSUPPORTED, "%s: particles on %d levels; one level only", base.c_str(),
+                          finest + 1);
+    if (np < 0 || np > (long long)0x7fffffff) FAIL(c, NYXPM_ERR_ARG, "%s/Header: %lld particles", base.c_str(), np);
+    if (std::fscanf(h, "%d", &ngrids) != 1 || ngrids < 0) FAIL(c, NYXPM_ERR_ARG, "%s/Header: grids", base.c_str());
+    const int nint = chk ? 2 + nie : 0, nreal = 3 + nre;
+    std::vector<double> a(7 * (size_t)np);
+    long long filled = 0;
+    std::vector<unsigned char> buf;
+    for (int g = 0; g < ngrids; ++g) {
+        int which = 0;
+        long long cnt = 0, where = 0;
+        if (std::fscanf(h, "%d %lld %lld", &which, &cnt, &where) != 3)
+            FAIL(c, NYXPM_ERR_ARG, "%s/Header: grid %d", base.c_str(), g);
+        if (cnt == 0) continue;
+        if (cnt < 0 || filled + cnt > np) FAIL(c, NYXPM_ERR_ARG, "%s/Header: grid counts exceed %lld", base.c_str(), np);
+        char fn[64];
+        std::snprintf(fn, sizeof(fn), "/Level_0/DATA_%05d", which);
+        FILE* d = std::fopen((base + fn).c_str(), "rb");
+        if (!d) FAIL(c, NYXPM_ERR_ARG, "cannot open %s%s", base.c_str(), fn);
+        bool ok = std::fseek(d, (long)(where + cnt * nint * (long long)sizeof(int)), SEEK_SET) == 0;
+        buf.resize((size_t)cnt * nreal * rsize);
+        ok = ok && std::fread(buf.data(), 1, buf.size(), d) == buf.size();
+        std::fclose(d);
+        if (!ok) FAIL(c, NYXPM_ERR_ARG, "%s%s: short read (grid %d, %lld particles)", base.c_str(), fn, g, cnt);
+        for (long long q = 0; q < cnt; ++q)
+            for (int v = 0; v < 7; ++v) {
+                const unsigned char* src = buf.data() + ((size_t)q * nreal + v) * rsize;
+                double dv;
+                if (rsize == 8) {
+                    std::memcpy(&dv, src, 8);
+                } else {
+                    float fv;
+                    std::memcpy(&fv, src, 4);
+                    dv = fv;
+                }
+                a[(size_t)v * np + filled + q] = dv;
+            }
+        filled += cnt;
+    }
+    if (filled != np) FAIL(c, NYXPM_ERR_ARG, "%s: grids hold %lld of %lld particles", base.c_str(), filled, np);
+    const double* A = a.data();
+    rc = upload_impl(c, box, np, A, A + np, A + 2 * np, A + 4 * np, A + 5 * np, A + 6 * np, A + 3 * np, 0.0,
+                     hipMemcpyHostToDevice);
+    if (rc) return rc;
+    if (np > 0 && (rc = shift_into_box(c))) return rc;
+    if (np_out) *np_out = np;
+    return NYXPM_OK;
 }
 
 int nyxpm_assign_density(nyxpm_ctx* c, int box, double* rho_out) {

@@ -158,6 +158,24 @@ class PMContext:
         self._ck(self.L.nyxpm_particles_load_binary(self.h, box, str(path).encode(), ctypes.byref(n)))
         return n.value
 
+    def load_ascii(self, path, box=0):
+        """Nyx's ASCII particle file (InitFromAsciiFile) into box `box`; returns NP."""
+        n = ctypes.c_int64()
+        self._ck(self.L.nyxpm_particles_load_ascii(self.h, box, str(path).encode(), ctypes.byref(n)))
+        return n.value
+
+    def write_amrex(self, dir, name="DM", checkpoint=True, box=0):
+        """AMReX particle checkpoint (NyxCheckpoint) or plotfile (WriteNyxPlotFile)."""
+        self._ck(self.L.nyxpm_particles_write_amrex(self.h, box, str(dir).encode(), str(name).encode(),
+                                                    int(bool(checkpoint))))
+
+    def read_amrex(self, dir, name="DM", box=0):
+        """Restart from an AMReX particle checkpoint or plotfile; returns NP."""
+        n = ctypes.c_int64()
+        self._ck(self.L.nyxpm_particles_read_amrex(self.h, box, str(dir).encode(), str(name).encode(),
+                                                   ctypes.byref(n)))
+        return n.value
+
     def num_particles(self, box=0):
         return int(self.L.nyxpm_num_particles(self.h, box))
 
@@ -351,6 +369,24 @@ class DarkMatterParticleContainer:
     def InitFromArrays(self, pos, vel, mass=None, mass_const=1.0):
         """Stands in for the Ascii/Binary particle readers (NyxParticles.cpp:379-744)."""
         self.ctx.upload(self.box, pos, vel, mass, mass_const)
+
+    def InitFromAsciiFile(self, file, extradata=4):
+        """NyxParticleContainer's InitFromAsciiFile(file, AMREX_SPACEDIM + 1)."""
+        if extradata != 4:
+            raise ValueError("extradata must be AMREX_SPACEDIM + 1 (mass and velocity)")
+        return self.ctx.load_ascii(file, self.box)
+
+    def NyxCheckpoint(self, dir, name="DM"):
+        """NyxParticleContainer::NyxCheckpoint (NyxParticleContainer.H:508-517)."""
+        self.ctx.write_amrex(dir, name, True, self.box)
+
+    def WriteNyxPlotFile(self, dir, name="DM"):
+        """NyxParticleContainer::WriteNyxPlotFile (NyxParticleContainer.H:497-505)."""
+        self.ctx.write_amrex(dir, name, False, self.box)
+
+    def Restart(self, dir, name="DM"):
+        """ParticleContainer::Restart as Nyx calls it (NyxParticles.cpp:586)."""
+        return self.ctx.read_amrex(dir, name, self.box)
 
     def InitFromBinaryFile(self, file, extradata=4):
         """NyxParticleContainer's InitFromBinaryFile(file, AMREX_SPACEDIM + 1)."""
