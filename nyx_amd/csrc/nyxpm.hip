@@ -1001,16 +1001,19 @@ int phase_sort(nyxpm_ctx* c) {
 }
 
 // want_rho: also materialise the density (AssignDensity); the step needs only f.
-int phase_deposit(nyxpm_ctx* c, bool want_rho) {
+// f32: FP32 per-particle weights (the mixed-precision step); AssignDensity
+// itself, the drop-in, always deposits with FP64 weights.
+int phase_deposit(nyxpm_ctx* c, bool want_rho, bool f32 = false) {
     int rc;
     if (!c->sorted_valid && (rc = phase_sort(c))) return rc;
     PartPtrs& A = c->P[c->cur];
-    const size_t lds = 2 * sizeof(unsigned long long) * (c->tile[0] + 1) * (c->tile[1] + 1) * (c->tile[2] + 1);
+    const size_t lds = (f32 ? 1 : 2) * sizeof(unsigned long long) * (c->tile[0] + 1) * (c->tile[1] + 1) *
+                       (c->tile[2] + 1);
     {
         Timed t(c, "deposit");
-        hipLaunchKernelGGL(k_deposit, dim3(c->heavy_cap + c->tseg_blocks), dim3(kBlock), lds, c->stream, c->tseg,
-                           c->tseg_n, c->dboxes, c->tbase, c->cursor, c->heavy, c->nheavy, c->heavy_cap, A.x, A.y,
-                           A.z, A.m, c->own, c->halo, c->err_flag);
+        hipLaunchKernelGGL(f32 ? k_deposit<true> : k_deposit<false>, dim3(c->heavy_cap + c->tseg_blocks),
+                           dim3(kBlock), lds, c->stream, c->tseg, c->tseg_n, c->dboxes, c->tbase, c->cursor,
+                           c->heavy, c->nheavy, c->heavy_cap, A.x, A.y, A.z, A.m, c->own, c->halo, c->err_flag);
     }
     {
         Timed t(c, "fold");
@@ -1458,6 +1461,10 @@ int require_particles(nyxpm_ctx* c) {
     return NYXPM_OK;
 }
 
+// Whether the deposits of a step (and of nyxpm_pm_init) use FP32 per-particle
+// weights: in the mixed-precision mode.
+bool mixed_deposit(const nyxpm_ctx* c) { return c->precision == NYXPM_MIXED; }
+
 // The canonical KDK step, in stream order; with graph_cycles > 0 nothing is
 // read back (capturable).
 int step_body(nyxpm_ctx* c, double dt, double a_old, double a_new, double four_pi_G, double rel, double abs_tol,
@@ -1468,7 +1475,7 @@ int step_body(nyxpm_ctx* c, double dt, double a_old, double a_new, double four_p
         // Nyx_advance_particles.cpp:148-170: old phi re-solved from a zero guess
         if ((rc = phase_keys(c))) return rc;
         if ((rc = phase_sort(c))) return rc;
-        if ((rc = phase_deposit(c, false))) return rc;
+        if ((rc = phase_deposit(c, false, mixed_deposit(c)))) return rc;
         if ((rc = phase_rhs(c, a_old, four_pi_G))) return rc;
         hipLaunchKernelGGL(k_fill_dbl, dim3(nblocks(c->ncell_total)), dim3(kBlock), 0, c->stream, c->phi,
                            c->ncell_total, 0.0);
@@ -1477,7 +1484,7 @@ int step_body(nyxpm_ctx* c, double dt, double a_old, double a_new, double four_p
     }
     if ((rc = phase_kick<kKickDrift>(c, dt, a_old, a_half))) return rc;
     if ((rc = phase_sort(c))) return rc;
-    if ((rc = phase_deposit(c, false))) return rc;
+    if ((rc = phase_deposit(c, false, mixed_deposit(c)))) return rc;
     if ((rc = phase_rhs(c, a_new, four_pi_G))) return rc;
     if (nyx_faithful)
         invalidate_guess(c);  // Nyx's guess: the old phi just solved
@@ -1498,7 +1505,7 @@ int step_tail(nyxpm_ctx* c, double dt, double a_old, double a_new, double four_p
     int rc;
     const double a_half = 0.5 * (a_old + a_new);
     if ((rc = phase_sort(c))) return rc;
-    if ((rc = phase_deposit(c, false))) return rc;
+    if ((rc = phase_deposit(c, false, mixed_deposit(c)))) return rc;
     if ((rc = phase_rhs(c, a_new, four_pi_G))) return rc;
     if ((rc = prepare_guess(c))) return rc;
     if ((rc = solve(c, rel, abs_tol, 0, graph_cycles))) return rc;
@@ -2217,9 +2224,9 @@ static int density_and_vels_impl(nyxpm_ctx* c, int box, double* dst, double* sum
     const long long N = c->ncell_total;
     const int nbox = (int)c->boxes.size();
     const size_t lds = 2 * sizeof(unsigned long long) * (c->tile[0] + 1) * (c->tile[1] + 1) * (c->tile[2] + 1);
-    hipLaunchKernelGGL(k_deposit, dim3(c->heavy_cap + c->tseg_blocks), dim3(kBlock), lds, c->stream, c->tseg,
-                       c->tseg_n, c->dboxes, c->tbase, c->cursor, c->heavy, c->nheavy, c->heavy_cap, A.x, A.y, A.z,
-                       A.m, c->own, c->halo, c->err_flag);
+    hipLaunchKernelGGL(k_deposit<false>, dim3(c->heavy_cap + c->tseg_blocks), dim3(kBlock), lds, c->stream,
+                       c->tseg, c->tseg_n, c->dboxes, c->tbase, c->cursor, c->heavy, c->nheavy, c->heavy_cap, A.x,
+                       A.y, A.z, A.m, c->own, c->halo, c->err_flag);
     hipLaunchKernelGGL(k_fold_raw, dim3(c->t_res.blocks), dim3(kBlock), 0, c->stream, c->segs(c->t_res),
                        c->t_res.count, c->dboxes, c->own, c->halo, sums);
     // the per-box maxima start at zero on every call (the scratch keeps the last
@@ -2500,7 +2507,7 @@ int nyxpm_pm_init(nyxpm_ctx* c, double a, double four_pi_G, double rel_tol, doub
     if (!c) return NYXPM_ERR_ARG;
     int rc;
     if ((rc = require_particles(c))) return rc;
-    if ((rc = phase_deposit(c, true))) return rc;
+    if ((rc = phase_deposit(c, true, mixed_deposit(c)))) return rc;
     if ((rc = phase_rhs(c, a, four_pi_G))) return rc;
     hipLaunchKernelGGL(k_fill_dbl, dim3(nblocks(c->ncell_total)), dim3(kBlock), 0, c->stream, c->phi,
                        c->ncell_total, 0.0);

@@ -272,9 +272,33 @@ __device__ __forceinline__ int halo_index(int lx, int ly, int lz, int bx, int by
     return fx + fy + (lx - 1) * by + (ly - 1);
 }
 
+// A positive FP32 value w < 2^(E+1) in units 2^(E-32), rounded to nearest (ties
+// up): w = mant * 2^(e-150) for the biased exponent e and the 24-bit mantissa,
+// so the count is mant shifted by e - 118 - E (at most 9 up: below 2^33).
+// Integer operations only; 0 for zero and for values below half a unit.
+__device__ __forceinline__ unsigned long long fx32_units(float w, int E) {
+    const unsigned bits = __float_as_uint(w);
+    const int e = (int)(bits >> 23) & 0xff;
+    if (e == 0) return 0ull;  // zero (or a subnormal: far below the unit)
+    const unsigned long long mant = (unsigned long long)((bits & 0x7fffffu) | 0x800000u);
+    const int sh = e - 118 - E;
+    if (sh >= 0) return mant << sh;
+    if (sh < -40) return 0ull;
+    return (mant + (1ull << (-sh - 1))) >> (-sh);
+}
+
 // One block per tile.  Contributions m*sx[ii]*sy[jj]*sz[kk] (amrex_deposit_cic
 // weights, product left to right) of the tile's particles go to LDS as fixed
 // point; interior cells are written to `own`, the low halo layer to `halo`.
+//
+// F32 (the mixed-precision step): the fractional weights and the product are
+// FP32 (the cell index and the fraction come from the FP64 position as in FP64
+// mode), and each contribution is one exact 64-bit word in units 2^(E-32)
+// (fx32_units), so the LDS sum is still exact and order independent, with half
+// the atomics: a cell's sum of at most 2^31 contributions below 2^33 units
+// each stays below 2^64.  Per contribution the FP32 rounding (~6e-8 relative)
+// dominates the unit (2^-32 of the largest mass).
+template <bool F32>
 __global__ void __launch_bounds__(kBlock) k_deposit(const BoxSeg* segs, int nseg, const BoxDev* boxes,
                                                     const long long* __restrict__ tbase,
                                                     const int* __restrict__ thw,
@@ -322,7 +346,7 @@ __global__ void __launch_bounds__(kBlock) k_deposit(const BoxSeg* segs, int nseg
         }
     };
     load2(p0 + threadIdx.x);
-    for (int c = threadIdx.x; c < 2 * LC; c += kBlock) acc[c] = 0ull;
+    for (int c = threadIdx.x; c < (F32 ? LC : 2 * LC); c += kBlock) acc[c] = 0ull;
     __syncthreads();
     const int E = B.fx_exp;
     // one particle's CIC contributions as two fixed-point words: w * 2^(31-E) =
@@ -339,6 +363,23 @@ __global__ void __launch_bounds__(kBlock) k_deposit(const BoxSeg* segs, int nseg
         const int kw = wrap_checked(k, B.n[2], err) - o2;
         if (iw < 0 || iw >= bx || jw < 0 || jw >= by || kw < 0 || kw >= bz) {
             atomicOr(err, 2);
+            return;
+        }
+        if (F32) {
+            const float fx = (float)xint, fy = (float)yint, fz = (float)zint, fm = (float)mp;
+            const float sx[2] = {1.0f - fx, fx};
+            const float sy[2] = {1.0f - fy, fy};
+            const float sz[2] = {1.0f - fz, fz};
+#pragma unroll
+            for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+                for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+                    for (int ii = 0; ii < 2; ++ii) {
+                        const float w = sx[ii] * sy[jj] * sz[kk] * fm;
+                        const int idx = ((kw + kk) * Ly + (jw + jj)) * Lx + (iw + ii);
+                        atomicAdd(&acc[idx], fx32_units(w, E));
+                    }
             return;
         }
         const double sx[2] = {1.0 - xint, xint};
@@ -371,7 +412,8 @@ __global__ void __launch_bounds__(kBlock) k_deposit(const BoxSeg* segs, int nseg
     __syncthreads();
     for (int c = threadIdx.x; c < LC; c += kBlock) {
         const int lx = c % Lx, ly = (c / Lx) % Ly, lz = c / (Lx * Ly);
-        const double v = ldexp((double)acc[c], E - 31) + ldexp((double)acc[LC + c], E - 63);
+        const double v = F32 ? ldexp((double)acc[c], E - 32)
+                             : ldexp((double)acc[c], E - 31) + ldexp((double)acc[LC + c], E - 63);
         if (lx > 0 && ly > 0 && lz > 0)
             own[B.cell_off + cidx(o0 + lx - 1, o1 + ly - 1, o2 + lz - 1, B.n[0], B.n[1])] = v;
         else

@@ -150,7 +150,11 @@ def test_clustered_512_mixed_full_size(nyx):
             print(f"512^3 clustered step {s + 1}: {int(its[0])} V-cycles, accel rel-L2 {rel:.3e}, "
                   f"momentum ratio mixed {moms[-1][0]:.3e} fp64 {moms[-1][1]:.3e}", flush=True)
         assert max(rels) <= 1e-6
-        assert max(max(mm) for mm in moms) <= 1e-8
+        # exactly 0 for the exact discrete solution of an FP64 deposit; the
+        # mixed steps deposit with FP32 weights (not the exact transpose of the
+        # FP64 interpolation)
+        assert max(mm[1] for mm in moms) <= 1e-8
+        assert max(mm[0] for mm in moms) <= 1e-7
         # the step kept the particles: the mass is unchanged
         rho = torch.empty(N, dtype=torch.float64, device="cuda")
         c.assign_density_device(rho.data_ptr())

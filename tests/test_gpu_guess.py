@@ -36,8 +36,11 @@ def test_extrapolated_guess_meets_the_same_rule(nyx, precision):
     inp = _inputs(nyx, n, 41)
     prev = _run(nyx, n, inp, 8, precision, "previous")
     extr = _run(nyx, n, inp, 8, precision, "extrapolate")
+    # FP64: the solver tolerance; mixed: its steps deposit with FP32 weights, so
+    # against the FP64 density's solve the contract's 1e-6
+    bound = 1e-8 if precision == "fp64" else 1e-6
     for out, g, g64, _ in (prev, extr):
-        assert np.linalg.norm(g - g64) / np.linalg.norm(g64) <= 1e-8
+        assert np.linalg.norm(g - g64) / np.linalg.norm(g64) <= bound
     for d in range(3):
         for q in range(2):
             a, b = prev[0][q][d], extr[0][q][d]
