@@ -50,29 +50,50 @@ def tile_halo_cells(n, tile):
                for bx in sizes[0] for by in sizes[1] for bz in sizes[2])
 
 
-def kernel_bytes(name, ncell, npart, halo_cells, mixed_sz):
-    """Algorithmic (compulsory) bytes of one launch of a kernel class at level 0."""
+def coarse_top_cells(ncell):
+    """Cells of the first level of at most 4096 cells (halving every dimension
+    per level): where the one-workgroup coarse V-cycle takes over."""
+    while ncell > 4096:
+        ncell //= 8
+    return ncell
+
+
+def kernel_bytes(name, ncell, npart, halo_cells, mixed_sz, moved=0):
+    """Algorithmic (compulsory) bytes of one launch of a kernel class: one read
+    or write per array per pass.  Multigrid stages carry their depth (".dD": a
+    level of ncell / 8^D cells); `moved` is the particles an incremental re-sort
+    moves per launch."""
     sz = mixed_sz
     N, P = ncell, npart
+    base, _, dep = name.partition(".d")
+    if dep:
+        Nl = N / 8 ** int(dep)
+        per_cell = {
+            "smooth": sz + sz / 2 + sz / 2,   # read e (all), r (colour), write e (colour)
+            "sweep": 3 * sz,                  # fused red+black: read e, r; write e
+            "sweep0": 2 * sz,                 # first sweep from e = 0: read r; write e
+            "sweep_p": 3 * sz + sz / 8,       # + coarse correction read (prolongation on load)
+            "resid_restrict": 2 * sz + sz / 8,  # read e, r; write coarse r
+            "prolong": 2 * sz + sz / 8,       # read/write e, read coarse e
+            "fill": sz,
+        }
+        if base == "sweep_u" and dep == "0":
+            return (2 * sz + 16) * N          # read e, r; phi += e (read/write FP64 phi)
+        return per_cell[base] * Nl if base in per_cell else None
     table = {
-        "smooth.d0": (sz + sz / 2 + sz / 2) * N,          # read e (all), r (colour), write e (colour)
-        "sweep.d0": 3 * sz * N,                            # fused red+black: read e, r; write e
-        "sweep0.d0": 2 * sz * N,                           # first sweep from e = 0: read r; write e
-        "sweep_p.d0": (3 * sz + sz / 8) * N,               # + coarse correction read (prolongation on load)
-        "sweep_u.d0": (2 * sz + 16) * N,                   # read e, r; phi += e (read/write FP64 phi)
-        "resid_restrict.d0": (2 * sz + sz / 8) * N,        # read e, r; write coarse r
-        "prolong.d0": (2 * sz + sz / 8) * N,               # read/write e, read coarse e
-        "fill.d0": sz * N,
         "residual": (16 + sz) * N,                         # read phi, f (FP64); write r
         "res_sweep0": (16 + 2 * sz) * N,                   # residual + first sweep: write r and e
         "update": (16 + sz) * N,                           # read/write phi, read e
+        "guess": 24 * N,                                   # read phi, prev; write both
         "deposit": 24 * P + 8 * N + 8 * halo_cells,        # read positions; write own + halos
         "fold": 8 * N + 8 * halo_cells + 8 * N,            # read own + halos; write f (rho kept in own)
         "kick_drift": 96 * P + 8 * N,                      # read/write x,v; read phi
         "kick_pair": 96 * P + 8 * N,                       # kick + next kick-drift: same traffic
         "kick": 72 * P + 8 * N,                            # read x,v; write v; read phi
         "scatter": 4 * P + 2 * 56 * P,                     # key; move x,v,id
-        "keys": 24 * P + 4 * P,
+        "keys": 24 * P + 4 * P,                            # read positions; write key
+        "migrate": 2 * 52 * moved + 8 * moved,             # move x,v,id of the listed particles; the list
+        "coarse": 2 * sz * coarse_top_cells(N),            # the coarse workgroup: its top level's r in, e out
     }
     return table.get(name)
 
@@ -148,13 +169,16 @@ def run_instrumented(ctx, precision, steps, dt, tol, use_graph, names, ncell, np
     ctx.timing_reset()
     ctx.timing_select(None)
     ctx.timing_enable(True)
+    s0 = ctx.sort_stats()
     ctx.sync()
     t0 = time.perf_counter()
     ctx.pm_steps(steps, dt, rel_tol=tol, use_graph=use_graph)
     ctx.sync()
     t_inst = time.perf_counter() - t0
     ctx.timing_enable(False)
-    st = stage_table(ctx, names, ncell, npart, halo_cells, sz, t_inst * 1e3)
+    s1 = ctx.sort_stats()
+    inc = max(1, s1["incremental"] - s0["incremental"])
+    st = stage_table(ctx, names, ncell, npart, halo_cells, sz, t_inst * 1e3, (s1["moved"] - s0["moved"]) / inc)
     dname = dominant_stage(st, steps)
     if dom is None or dname not in dom.get("candidates", {}):
         # a stage outside the timed candidates leads the stage table: report it
@@ -164,13 +188,13 @@ def run_instrumented(ctx, precision, steps, dt, tol, use_graph, names, ncell, np
     return st, dom, t_inst
 
 
-def stage_table(ctx, names, ncell, npart, halo_cells, sz, elapsed_ms):
+def stage_table(ctx, names, ncell, npart, halo_cells, sz, elapsed_ms, moved=0):
     out = {}
     for nm in names:
         cnt, mean, tot = ctx.timing(nm)
         if cnt == 0:
             continue
-        b = kernel_bytes(nm, ncell, npart, halo_cells, sz)
+        b = kernel_bytes(nm, ncell, npart, halo_cells, sz, moved)
         e = {"launches": cnt, "mean_ms": round(mean, 4), "total_ms": round(tot, 2),
              "share": round(tot / elapsed_ms, 4)}
         if b is not None and mean > 0:

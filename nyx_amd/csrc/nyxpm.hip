@@ -1374,10 +1374,16 @@ int solve_results(nyxpm_ctx* c, int* iters, double* final_res, bool allow_active
     return NYXPM_OK;
 }
 
+template <int MODE>
+int phase_kick(nyxpm_ctx* c, double dt, double a_prev, double a_cur, double a_prev2 = 1.0, double a_cur2 = 1.0);
+
 // MODE is kKick (a_prev, a_cur), kKickDrift (a_prev, a_cur) or kKickPair: the
 // kick (a_prev, a_cur) of one step and the kick-drift (a_prev2, a_cur2) of the next.
-template <int MODE>
-int phase_kick(nyxpm_ctx* c, double dt, double a_prev, double a_cur, double a_prev2 = 1.0, double a_cur2 = 1.0) {
+// G is the interpolation arithmetic: double, or float in the mixed-precision
+// mode (g formed in FP64 and kept in FP32, FP32 weights and sums; positions,
+// velocities and the update stay FP64).
+template <int MODE, typename G>
+int phase_kick_g(nyxpm_ctx* c, double dt, double a_prev, double a_cur, double a_prev2, double a_cur2) {
     int rc;
     // a kick-drift's list of tile changes is used up before the particles move again
     if (c->moves_pending && (rc = phase_sort(c))) return rc;
@@ -1386,8 +1392,8 @@ int phase_kick(nyxpm_ctx* c, double dt, double a_prev, double a_cur, double a_pr
     constexpr bool MOVE = MODE != kKick;
     // staged phi tile with a halo of 2, plus the three components of g on the
     // (tile+1)^3 cells its particles interpolate from
-    const size_t lds = sizeof(double) * ((c->tile[0] + 3) * (c->tile[1] + 3) * (c->tile[2] + 3) +
-                                         3 * (c->tile[0] + 1) * (c->tile[1] + 1) * (c->tile[2] + 1));
+    const size_t lds = sizeof(double) * (c->tile[0] + 3) * (c->tile[1] + 3) * (c->tile[2] + 3) +
+                       sizeof(G) * 3 * (c->tile[0] + 1) * (c->tile[1] + 1) * (c->tile[2] + 1);
     if (c->items_valid && !c->untiled_kick && lds <= 64 * 1024) {
         Timed t(c, MODE == kKickPair ? "kick_pair" : MOVE ? "kick_drift" : "kick");
         // incremental re-sort: the kick-drift lists the particles that change tile
@@ -1408,12 +1414,12 @@ int phase_kick(nyxpm_ctx* c, double dt, double a_prev, double a_cur, double a_pr
             hipLaunchKernelGGL(k_fill_dbl, dim3(1), dim3(kBlock), 0, c->stream, reinterpret_cast<double*>(c->dtmin),
                                (long long)c->boxes.size(), 1e50);
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(kBlock), 0, c->stream, c->ndefer, 1LL, 0);
-        auto kern = cfl > 0 ? k_kick_tiled<MODE, MODE != kKickDrift> : k_kick_tiled<MODE, false>;
+        auto kern = cfl > 0 ? k_kick_tiled<MODE, MODE != kKickDrift, G> : k_kick_tiled<MODE, false, G>;
         hipLaunchKernelGGL(kern, dim3((unsigned)c->items_cap), dim3(kBlock), lds, c->stream, c->tseg, c->tseg_n,
                            c->dboxes, c->items, c->nitems, c->phi, A.x, A.y, A.z, A.vx, A.vy, A.vz, dt, a_prev,
                            a_cur, a_prev2, a_cur2, c->key, c->counts, defer, c->ndefer, cfl, c->dtmin, inc, moves,
                            c->nmove, c->err_flag);
-        hipLaunchKernelGGL(k_kick_deferred<MODE>, dim3(256), dim3(kBlock), 0, c->stream, defer, c->ndefer,
+        hipLaunchKernelGGL((k_kick_deferred<MODE, G>), dim3(256), dim3(kBlock), 0, c->stream, defer, c->ndefer,
                            c->dboxes, c->phi, A.x, A.y, A.z, A.vx, A.vy, A.vz, dt, a_prev, a_cur, a_prev2, a_cur2,
                            c->key, c->counts, cfl, c->dtmin, inc, moves, c->nmove, c->err_flag);
         LAUNCHCK(c);
@@ -1434,13 +1440,13 @@ int phase_kick(nyxpm_ctx* c, double dt, double a_prev, double a_cur, double a_pr
         Timed t(c, "kick_drift");
         hipLaunchKernelGGL(k_fill_int, dim3(nblocks(c->ntiles_total)), dim3(kBlock), 0, c->stream, c->counts,
                            c->ntiles_total, 0);
-        hipLaunchKernelGGL(k_kick<true>, dim3(c->pseg_blocks), dim3(kBlock), 0, c->stream, c->pseg, c->pseg_n,
+        hipLaunchKernelGGL((k_kick<true, G>), dim3(c->pseg_blocks), dim3(kBlock), 0, c->stream, c->pseg, c->pseg_n,
                            c->dboxes, c->phi, A.x, A.y, A.z, A.vx, A.vy, A.vz, dt, a_prev, a_cur, c->key,
                            c->counts, c->err_flag);
         c->keys_valid = true;
     } else {
         Timed t(c, "kick");
-        hipLaunchKernelGGL(k_kick<false>, dim3(c->pseg_blocks), dim3(kBlock), 0, c->stream, c->pseg, c->pseg_n,
+        hipLaunchKernelGGL((k_kick<false, G>), dim3(c->pseg_blocks), dim3(kBlock), 0, c->stream, c->pseg, c->pseg_n,
                            c->dboxes, c->phi, A.x, A.y, A.z, A.vx, A.vy, A.vz, dt, a_prev, a_cur, c->key,
                            c->counts, c->err_flag);
         c->keys_valid = false;
@@ -1449,6 +1455,12 @@ int phase_kick(nyxpm_ctx* c, double dt, double a_prev, double a_cur, double a_pr
     c->sorted_valid = false;
     c->density_valid = false;
     return NYXPM_OK;
+}
+
+template <int MODE>
+int phase_kick(nyxpm_ctx* c, double dt, double a_prev, double a_cur, double a_prev2, double a_cur2) {
+    if (c->precision == NYXPM_MIXED) return phase_kick_g<MODE, float>(c, dt, a_prev, a_cur, a_prev2, a_cur2);
+    return phase_kick_g<MODE, double>(c, dt, a_prev, a_cur, a_prev2, a_cur2);
 }
 
 int check_box(nyxpm_ctx* c, int box) {
@@ -2399,7 +2411,8 @@ int nyxpm_gather_accel(nyxpm_ctx* c, int box, double* g_out) {
     const bool dev = is_device_ptr(g_out);
     double* out = g_out;
     if (!dev && (rc = dalloc(c, &out, 3 * B.np))) return rc;
-    hipLaunchKernelGGL(k_gather, dim3(nblocks(B.span)), dim3(kBlock), 0, c->stream, c->dboxes, box, c->phi, A.x, A.y,
+    hipLaunchKernelGGL(c->precision == NYXPM_MIXED ? k_gather<float> : k_gather<double>, dim3(nblocks(B.span)),
+                       dim3(kBlock), 0, c->stream, c->dboxes, box, c->phi, A.x, A.y,
                        A.z, A.id, out, c->err_flag);
     hipError_t e = hipGetLastError();
     if (!dev && e == hipSuccess)
@@ -2425,7 +2438,7 @@ int nyxpm_est_time_step(nyxpm_ctx* c, double a, double cfl, double* dt_out) {
         hipLaunchKernelGGL(k_fill_dbl, dim3(1), dim3(kBlock), 0, c->stream, reinterpret_cast<double*>(c->dtmin),
                            (long long)nbox, 1e50);
         if (c->pseg_blocks > 0)
-            hipLaunchKernelGGL(k_est_dt, dim3(c->pseg_blocks), dim3(kBlock), 0, c->stream, c->pseg, c->pseg_n,
+            hipLaunchKernelGGL(c->precision == NYXPM_MIXED ? k_est_dt<float> : k_est_dt<double>, dim3(c->pseg_blocks), dim3(kBlock), 0, c->stream, c->pseg, c->pseg_n,
                                c->dboxes, c->phi, A.x, A.y, A.z, A.vx, A.vy, A.vz, a, cfl, c->dtmin);
         LAUNCHCK(c);
     }

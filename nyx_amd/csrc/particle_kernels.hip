@@ -262,14 +262,17 @@ __global__ void __launch_bounds__(kBlock) k_scatter(const BoxSeg* segs, int nseg
 // ---------------------------------------------------------------------------
 
 // Index of a low-halo cell (lx==0 || ly==0 || lz==0) of a tile whose LDS block is
-// (bx+1) x (by+1) x (bz+1): the x face first, then the y face without its x=0
-// column, then the z face without x=0 and y=0.
+// (bx+1) x (by+1) x (bz+1): the x face first (y fastest), then the y face without
+// its x=0 column, then the z face without x=0 and y=0 (both x fastest).  The fold
+// walks cells row-major, so the y- and z-face entries it reads for a row of a
+// tile's top face are consecutive (a strided face order fetched a separate
+// memory sector per entry).
 __device__ __forceinline__ int halo_index(int lx, int ly, int lz, int bx, int by, int bz) {
-    if (lx == 0) return ly * (bz + 1) + lz;
+    if (lx == 0) return lz * (by + 1) + ly;
     const int fx = (by + 1) * (bz + 1);
-    if (ly == 0) return fx + (lx - 1) * (bz + 1) + lz;
+    if (ly == 0) return fx + lz * bx + (lx - 1);
     const int fy = bx * (bz + 1);
-    return fx + fy + (lx - 1) * by + (ly - 1);
+    return fx + fy + (ly - 1) * bx + (lx - 1);
 }
 
 // A positive FP32 value w < 2^(E+1) in units 2^(E-32), rounded to nearest (ties
@@ -706,7 +709,14 @@ __device__ __forceinline__ double periodic_shift(double v, double lo, double rhi
 
 // val[d] = sum_{kk,jj,ii} sx[1-ii]*sy[1-jj]*sz[1-kk] * g_d(i-ii, j-jj, k-kk) with
 // g_d(c) = 0.5*(dxi*(phi(c) - phi(c-e_d)) + dxi*(phi(c+e_d) - phi(c))).
-__device__ __forceinline__ void gather_grad_inl(const BoxDev& B, const double* __restrict__ P, double x, double y,
+// G is the interpolation arithmetic: double (the drop-in) or float (the
+// mixed-precision step: g of each corner cell formed in FP64 and rounded to
+// FP32, the fractions rounded to FP32, the weights and the sum in FP32, in the
+// order of the staged kick, so the two agree bit for bit).  Forming g in FP64
+// first keeps the difference of neighbouring phi values exact: phi rounded to
+// FP32 first would lose ~n/2pi times more of g's digits to cancellation.
+template <typename G>
+__device__ __forceinline__ void gather_grad(const BoxDev& B, const double* __restrict__ P, double x, double y,
                                             double z, double val[3], int* err) {
     const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
     double xint, yint, zint;
@@ -722,12 +732,12 @@ __device__ __forceinline__ void gather_grad_inl(const BoxDev& B, const double* _
         iy[mm] = wrap1(j - 2 + mm, ny);
         iz[mm] = wrap1(k - 2 + mm, nz);
     }
-    const double sx[2] = {1.0 - xint, xint};
-    const double sy[2] = {1.0 - yint, yint};
-    const double sz[2] = {1.0 - zint, zint};
+    const G sx[2] = {G(1) - G(xint), G(xint)};
+    const G sy[2] = {G(1) - G(yint), G(yint)};
+    const G sz[2] = {G(1) - G(zint), G(zint)};
     auto ld = [&](int a, int b, int c) -> double { return P[cidx(ix[a], iy[b], iz[c], nx, ny)]; };
     const double dxi0 = B.dxi[0], dxi1 = B.dxi[1], dxi2 = B.dxi[2];
-    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+    G v0 = G(0), v1 = G(0), v2 = G(0);
 #pragma unroll
     for (int kk = 0; kk <= 1; ++kk)
 #pragma unroll
@@ -736,29 +746,24 @@ __device__ __forceinline__ void gather_grad_inl(const BoxDev& B, const double* _
             for (int ii = 0; ii <= 1; ++ii) {
                 const int a = 2 - ii, b = 2 - jj, c = 2 - kk;
                 const double pc = ld(a, b, c);
-                const double gx = 0.5 * (dxi0 * (pc - ld(a - 1, b, c)) + dxi0 * (ld(a + 1, b, c) - pc));
-                const double gy = 0.5 * (dxi1 * (pc - ld(a, b - 1, c)) + dxi1 * (ld(a, b + 1, c) - pc));
-                const double gz = 0.5 * (dxi2 * (pc - ld(a, b, c - 1)) + dxi2 * (ld(a, b, c + 1) - pc));
-                const double w = sx[1 - ii] * sy[1 - jj] * sz[1 - kk];
+                const G gx = G(0.5 * (dxi0 * (pc - ld(a - 1, b, c)) + dxi0 * (ld(a + 1, b, c) - pc)));
+                const G gy = G(0.5 * (dxi1 * (pc - ld(a, b - 1, c)) + dxi1 * (ld(a, b + 1, c) - pc)));
+                const G gz = G(0.5 * (dxi2 * (pc - ld(a, b, c - 1)) + dxi2 * (ld(a, b, c + 1) - pc)));
+                const G w = sx[1 - ii] * sy[1 - jj] * sz[1 - kk];
                 v0 += w * gx;
                 v1 += w * gy;
                 v2 += w * gz;
             }
-    val[0] = v0;
-    val[1] = v1;
-    val[2] = v2;
-}
-
-__device__ __forceinline__ void gather_grad(const BoxDev& B, const double* __restrict__ P, double x, double y,
-                                            double z, double val[3], int* err) {
-    gather_grad_inl(B, P, x, y, z, val, err);
+    val[0] = double(v0);
+    val[1] = double(v1);
+    val[2] = double(v2);
 }
 
 // moveKickDrift (DO_MOVE) or moveKick (!DO_MOVE) of update_dm_particle_single:
 //   v = (a_prev*v + (0.5*dt)*val) * (1/a_cur);  DO_MOVE: x += (dt*(1/a_cur))*v
 // DO_MOVE also bins the drifted particle (key + count for the next deposit);
 // !DO_MOVE applies the periodic shift of Redistribute.
-template <bool DO_MOVE>
+template <bool DO_MOVE, typename G>
 __global__ void __launch_bounds__(kBlock) k_kick(const BoxSeg* segs, int nseg, const BoxDev* boxes,
                                                  const double* __restrict__ phi, double* __restrict__ x,
                                                  double* __restrict__ y, double* __restrict__ z,
@@ -782,7 +787,7 @@ __global__ void __launch_bounds__(kBlock) k_kick(const BoxSeg* segs, int nseg, c
         const double dt_a_cur_inv = dt * a_cur_inv;
         double py = y[p], pz = z[p];
         double val[3];
-        gather_grad(B, phi + B.cell_off, px, py, pz, val, err);
+        gather_grad<G>(B, phi + B.cell_off, px, py, pz, val, err);
         double ux = a_prev * vx[p] + half_dt * val[0];
         ux = ux * a_cur_inv;
         double uy = a_prev * vy[p] + half_dt * val[1];
@@ -811,50 +816,6 @@ __global__ void __launch_bounds__(kBlock) k_kick(const BoxSeg* segs, int nseg, c
         }
     }
     if (DO_MOVE) agg_atomic_add(counts, t, valid);
-}
-
-// CIC-weighted central-difference gradient at a particle, evaluated in two
-// (kk, jj) loops so that only one x-line of the stencil is live at a time.
-// ld(a, b, c) returns phi at cell (i-2+a, j-2+b, k-2+c).  Summation order is that
-// of update_dm_particle_single: kk, then jj, then ii (ii = 0: cell i, ii = 1:
-// cell i-1), each term ((sx*sy)*sz)*g.
-template <class LD>
-__device__ __forceinline__ void cic_grad_lines(LD ld, double xint, double yint, double zint, double dxi0,
-                                               double dxi1, double dxi2, double val[3]) {
-    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
-#pragma unroll 1
-    for (int kk = 0; kk <= 1; ++kk) {
-        const double wz = kk ? 1.0 - zint : zint;
-#pragma unroll 1
-        for (int jj = 0; jj <= 1; ++jj) {
-            const double wy = jj ? 1.0 - yint : yint;
-            const int b = 2 - jj, c = 2 - kk;
-            const double l0 = ld(0, b, c), l1 = ld(1, b, c), l2 = ld(2, b, c), l3 = ld(3, b, c);
-            {   // ii = 0: cell i (a = 2)
-                const double pc = l2;
-                const double gx = 0.5 * (dxi0 * (pc - l1) + dxi0 * (l3 - pc));
-                const double gy = 0.5 * (dxi1 * (pc - ld(2, b - 1, c)) + dxi1 * (ld(2, b + 1, c) - pc));
-                const double gz = 0.5 * (dxi2 * (pc - ld(2, b, c - 1)) + dxi2 * (ld(2, b, c + 1) - pc));
-                const double w = xint * wy * wz;
-                v0 += w * gx;
-                v1 += w * gy;
-                v2 += w * gz;
-            }
-            {   // ii = 1: cell i-1 (a = 1)
-                const double pc = l1;
-                const double gx = 0.5 * (dxi0 * (pc - l0) + dxi0 * (l2 - pc));
-                const double gy = 0.5 * (dxi1 * (pc - ld(1, b - 1, c)) + dxi1 * (ld(1, b + 1, c) - pc));
-                const double gz = 0.5 * (dxi2 * (pc - ld(1, b, c - 1)) + dxi2 * (ld(1, b, c + 1) - pc));
-                const double w = (1.0 - xint) * wy * wz;
-                v0 += w * gx;
-                v1 += w * gy;
-                v2 += w * gz;
-            }
-        }
-    }
-    val[0] = v0;
-    val[1] = v1;
-    val[2] = v2;
 }
 
 // Work items for the tile-staged particle kernels: each tile's occupied slot
@@ -925,7 +886,9 @@ __device__ __forceinline__ double particle_dt(double ux, double uy, double uz, d
     return dtp;
 }
 
-// Cell-centred gravity (k_grav_vector's arithmetic) at cell (i, j, k) of a box.
+// Cell-centred gravity (k_grav_vector's arithmetic, rounded to G as the staged
+// kick keeps it) at cell (i, j, k) of a box.
+template <typename G>
 __device__ __forceinline__ void grav_at(const BoxDev& B, const double* __restrict__ P, int i, int j, int k,
                                         double& gx, double& gy, double& gz) {
     const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
@@ -933,12 +896,13 @@ __device__ __forceinline__ void grav_at(const BoxDev& B, const double* __restric
     const int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
     const int km = k == 0 ? nz - 1 : k - 1, kp = k == nz - 1 ? 0 : k + 1;
     const double pc = P[cidx(i, j, k, nx, ny)];
-    gx = 0.5 * (B.dxi[0] * (pc - P[cidx(im, j, k, nx, ny)]) + B.dxi[0] * (P[cidx(ip, j, k, nx, ny)] - pc));
-    gy = 0.5 * (B.dxi[1] * (pc - P[cidx(i, jm, k, nx, ny)]) + B.dxi[1] * (P[cidx(i, jp, k, nx, ny)] - pc));
-    gz = 0.5 * (B.dxi[2] * (pc - P[cidx(i, j, km, nx, ny)]) + B.dxi[2] * (P[cidx(i, j, kp, nx, ny)] - pc));
+    gx = double(G(0.5 * (B.dxi[0] * (pc - P[cidx(im, j, k, nx, ny)]) + B.dxi[0] * (P[cidx(ip, j, k, nx, ny)] - pc))));
+    gy = double(G(0.5 * (B.dxi[1] * (pc - P[cidx(i, jm, k, nx, ny)]) + B.dxi[1] * (P[cidx(i, jp, k, nx, ny)] - pc))));
+    gz = double(G(0.5 * (B.dxi[2] * (pc - P[cidx(i, j, km, nx, ny)]) + B.dxi[2] * (P[cidx(i, j, kp, nx, ny)] - pc))));
 }
 
 // The estimate from global phi (any position; cells wrapped periodically).
+template <typename G>
 __device__ __forceinline__ double particle_dt_global(const BoxDev& B, const double* __restrict__ P, double px,
                                                      double py, double pz, double ux, double uy, double uz,
                                                      double a, double cfl) {
@@ -946,7 +910,7 @@ __device__ __forceinline__ double particle_dt_global(const BoxDev& B, const doub
     const int j = wrap1((int)floor((py - B.plo[1]) * B.dxi[1]), B.n[1]);
     const int k = wrap1((int)floor((pz - B.plo[2]) * B.dxi[2]), B.n[2]);
     double gx, gy, gz;
-    grav_at(B, P, i, j, k, gx, gy, gz);
+    grav_at<G>(B, P, i, j, k, gx, gy, gz);
     return particle_dt(ux, uy, uz, B.dxi[0] / a, B.dxi[1] / a, B.dxi[2] / a, gx, gy, gz, B.dxi[0], cfl);
 }
 
@@ -970,6 +934,7 @@ __device__ __forceinline__ void block_min_to(double v, unsigned long long* slot)
 
 // Standalone estimate over every particle (nyxpm_est_time_step when no fused
 // estimate of the current state is kept).
+template <typename G>
 __global__ void __launch_bounds__(kBlock) k_est_dt(const BoxSeg* segs, int nseg, const BoxDev* boxes,
                                                    const double* __restrict__ phi, const double* __restrict__ x,
                                                    const double* __restrict__ y, const double* __restrict__ z,
@@ -983,7 +948,7 @@ __global__ void __launch_bounds__(kBlock) k_est_dt(const BoxSeg* segs, int nseg,
     if (q < B.span) {
         const long long p = B.part_off + q;
         const double px = x[p];
-        if (!isnan(px)) d = particle_dt_global(B, phi + B.cell_off, px, y[p], z[p], vx[p], vy[p], vz[p], a, cfl);
+        if (!isnan(px)) d = particle_dt_global<G>(B, phi + B.cell_off, px, y[p], z[p], vx[p], vy[p], vz[p], a, cfl);
     }
     block_min_to(d, dtmin + sg.box);
 }
@@ -1027,7 +992,7 @@ __device__ __forceinline__ long long defer_entry(long long p, int box) {
 // EST (closing kicks only): also NyxParticleContainer::estTimestep of the kicked
 // particles (cfl, a = a_cur) into dtmin, from the staged g; a compile-time
 // variant so that the kernels without it keep their register budget.
-template <int MODE, bool EST>
+template <int MODE, bool EST, typename G>
 __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int ntseg, const BoxDev* boxes,
                                                        const TileItem* __restrict__ items, const int* nitems,
                                                        const double* __restrict__ phi, double* __restrict__ x,
@@ -1039,7 +1004,8 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
                                                        long long* __restrict__ defer, int* ndefer, double cfl,
                                                        unsigned long long* dtmin, int inc,
                                                        long long* __restrict__ moves, int* nmove, int* err) {
-    extern __shared__ double sphi[];
+    extern __shared__ __align__(16) unsigned char kick_smem[];
+    double* sphi = reinterpret_cast<double*>(kick_smem);
     __shared__ int s_same[kBlock / kWave];
     // items are in (roughly) tile order; giving each XCD a contiguous range of them
     // lets neighbouring tiles share the phi halo lines in that XCD's L2
@@ -1119,9 +1085,9 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
     __syncthreads();
     const double dxi0 = B.dxi[0], dxi1 = B.dxi[1], dxi2 = B.dxi[2];
     // g at the cells the tile's particles interpolate from (local -1 .. b-1 per
-    // dimension), components of a cell adjacent
+    // dimension), components of a cell adjacent; formed in FP64, kept as G
     const int GX = bx + 1, GY = by + 1, GZ = bz + 1;
-    double* sg = sphi + SX * SY * SZ;
+    G* sg = reinterpret_cast<G*>(sphi + SX * SY * SZ);
     {
         const int rpw = kWave / GX;
         const int lrow = lane / GX, cx = lane - lrow * GX;
@@ -1133,10 +1099,10 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
             while (cz < GZ) {
                 const int o = ((cz + 1) * SY + (cy + 1)) * SX + (cx + 1);  // staged index of the cell
                 const double pc = sphi[o];
-                double* g = sg + 3 * ((cz * GY + cy) * GX + cx);
-                g[0] = 0.5 * (dxi0 * (pc - sphi[o - 1]) + dxi0 * (sphi[o + 1] - pc));
-                g[1] = 0.5 * (dxi1 * (pc - sphi[o - SX]) + dxi1 * (sphi[o + SX] - pc));
-                g[2] = 0.5 * (dxi2 * (pc - sphi[o - SX * SY]) + dxi2 * (sphi[o + SX * SY] - pc));
+                G* g = sg + 3 * ((cz * GY + cy) * GX + cx);
+                g[0] = G(0.5 * (dxi0 * (pc - sphi[o - 1]) + dxi0 * (sphi[o + 1] - pc)));
+                g[1] = G(0.5 * (dxi1 * (pc - sphi[o - SX]) + dxi1 * (sphi[o + SX] - pc)));
+                g[2] = G(0.5 * (dxi2 * (pc - sphi[o - SX * SY]) + dxi2 * (sphi[o + SX * SY] - pc)));
                 cy += dcy;
                 cz += dcz;
                 if (cy >= GY) {
@@ -1168,25 +1134,27 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
         if (li >= 0 && li < bx && lj >= 0 && lj < by && lk >= 0 && lk < bz) {
             // val_d = sum_{kk,jj,ii} sx[1-ii]*sy[1-jj]*sz[1-kk] * g_d(i-ii, j-jj, k-kk);
             // cell i-ii has g index li+1-ii
-            const double mx = 1.0 - xint, my = 1.0 - yint, mz = 1.0 - zint;
-            double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+            const G fx = G(xint), fy = G(yint), fz = G(zint);
+            const G mx = G(1) - fx, my = G(1) - fy, mz = G(1) - fz;
+            G g0 = G(0), g1 = G(0), g2 = G(0);
 #pragma unroll
             for (int kk = 0; kk <= 1; ++kk) {
-                const double wz = kk ? mz : zint;
+                const G wz = kk ? mz : fz;
 #pragma unroll
                 for (int jj = 0; jj <= 1; ++jj) {
-                    const double wy = jj ? my : yint;
-                    const double* g = sg + 3 * (((lk + 1 - kk) * GY + (lj + 1 - jj)) * GX + li);
-                    const double w0 = xint * wy * wz;  // ii = 0: cell i
-                    v0 += w0 * g[3];
-                    v1 += w0 * g[4];
-                    v2 += w0 * g[5];
-                    const double w1 = mx * wy * wz;  // ii = 1: cell i-1
-                    v0 += w1 * g[0];
-                    v1 += w1 * g[1];
-                    v2 += w1 * g[2];
+                    const G wy = jj ? my : fy;
+                    const G* g = sg + 3 * (((lk + 1 - kk) * GY + (lj + 1 - jj)) * GX + li);
+                    const G w0 = fx * wy * wz;  // ii = 0: cell i
+                    g0 += w0 * g[3];
+                    g1 += w0 * g[4];
+                    g2 += w0 * g[5];
+                    const G w1 = mx * wy * wz;  // ii = 1: cell i-1
+                    g0 += w1 * g[0];
+                    g1 += w1 * g[1];
+                    g2 += w1 * g[2];
                 }
             }
+            const double v0 = double(g0), v1 = double(g1), v2 = double(g2);
             double ux = a_prev * qx + half_dt * v0;
             ux = ux * a_cur_inv;
             double uy = a_prev * qy + half_dt * v1;
@@ -1208,8 +1176,9 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
                     const int ci = (int)floor((px - B.plo[0]) * dxi0) - (i - (li + o0));
                     const int cj = (int)floor((py - B.plo[1]) * dxi1) - (j - (lj + o1));
                     const int ck = (int)floor((pz - B.plo[2]) * dxi2) - (k - (lk + o2));
-                    const double* gc = sg + 3 * (((ck - o2 + 1) * GY + (cj - o1 + 1)) * GX + (ci - o0 + 1));
-                    const double d = particle_dt(ux, uy, uz, adxi0, adxi1, adxi2, gc[0], gc[1], gc[2], dxi0, cfl);
+                    const G* gc = sg + 3 * (((ck - o2 + 1) * GY + (cj - o1 + 1)) * GX + (ci - o0 + 1));
+                    const double d = particle_dt(ux, uy, uz, adxi0, adxi1, adxi2, double(gc[0]), double(gc[1]),
+                                                 double(gc[2]), dxi0, cfl);
                     dtmin_t = d < dtmin_t ? d : dtmin_t;
                 }
                 if (MODE == kKick) {
@@ -1279,7 +1248,7 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
 
 // The particles k_kick_tiled deferred, one thread each, phi read from global
 // memory (gather_grad: same arithmetic as the staged path).
-template <int MODE>
+template <int MODE, typename G>
 __global__ void __launch_bounds__(kBlock) k_kick_deferred(const long long* __restrict__ defer, const int* ndefer,
                                                           const BoxDev* boxes, const double* __restrict__ phi,
                                                           double* __restrict__ x, double* __restrict__ y,
@@ -1313,12 +1282,12 @@ __global__ void __launch_bounds__(kBlock) k_kick_deferred(const long long* __res
         double ux = vx[p], uy = vy[p], uz = vz[p];
         double val[3];
         if (e & kDeferDtOnly) {
-            atomicMin(dtmin + box, dbits(particle_dt_global(B, P, px, py, pz, ux, uy, uz, a_cur, cfl)));
+            atomicMin(dtmin + box, dbits(particle_dt_global<G>(B, P, px, py, pz, ux, uy, uz, a_cur, cfl)));
             continue;
         }
         if (!(e & kDeferPairKD)) {
             const double a_cur_inv = 1.0 / a_cur;
-            gather_grad(B, P, px, py, pz, val, err);
+            gather_grad<G>(B, P, px, py, pz, val, err);
             ux = a_prev * ux + half_dt * val[0];
             ux = ux * a_cur_inv;
             uy = a_prev * uy + half_dt * val[1];
@@ -1339,7 +1308,7 @@ __global__ void __launch_bounds__(kBlock) k_kick_deferred(const long long* __res
             const double syp = periodic_shift(py, B.plo[1], B.rhi[1], B.len[1], err);
             const double szp = periodic_shift(pz, B.plo[2], B.rhi[2], B.len[2], err);
             if (est)
-                atomicMin(dtmin + box, dbits(particle_dt_global(B, P, sxp, syp, szp, ux, uy, uz, a_cur, cfl)));
+                atomicMin(dtmin + box, dbits(particle_dt_global<G>(B, P, sxp, syp, szp, ux, uy, uz, a_cur, cfl)));
             if (MODE == kKick) {
                 vx[p] = ux; vy[p] = uy; vz[p] = uz;
                 if (sxp != px) x[p] = sxp;
@@ -1351,12 +1320,12 @@ __global__ void __launch_bounds__(kBlock) k_kick_deferred(const long long* __res
             px = sxp;
             py = syp;
             pz = szp;
-            if (moved) gather_grad(B, P, px, py, pz, val, err);
+            if (moved) gather_grad<G>(B, P, px, py, pz, val, err);
         } else {
             // pair whose kick is stored: its estimate, then the kick-drift at the
             // shifted position
-            if (est) atomicMin(dtmin + box, dbits(particle_dt_global(B, P, px, py, pz, ux, uy, uz, a_cur, cfl)));
-            gather_grad(B, P, px, py, pz, val, err);
+            if (est) atomicMin(dtmin + box, dbits(particle_dt_global<G>(B, P, px, py, pz, ux, uy, uz, a_cur, cfl)));
+            gather_grad<G>(B, P, px, py, pz, val, err);
         }
         const double a_cur2_inv = 1.0 / a_cur2;
         const double dt_a_cur2_inv = dt * a_cur2_inv;
@@ -1524,6 +1493,7 @@ __global__ void __launch_bounds__(kBlock) k_rhs1(const CellSeg* segs, int nseg, 
 
 // Accelerations at the particles of one box (the val of update_dm_particle_single)
 // without touching the particles; written in upload (id) order, component-major.
+template <typename G>
 __global__ void __launch_bounds__(kBlock) k_gather(const BoxDev* boxes, int box, const double* __restrict__ phi,
                                                    const double* __restrict__ x, const double* __restrict__ y,
                                                    const double* __restrict__ z, const int* __restrict__ id,
@@ -1535,7 +1505,7 @@ __global__ void __launch_bounds__(kBlock) k_gather(const BoxDev* boxes, int box,
     const double px = x[p];
     if (isnan(px)) return;
     double val[3];
-    gather_grad(B, phi + B.cell_off, px, y[p], z[p], val, err);
+    gather_grad<G>(B, phi + B.cell_off, px, y[p], z[p], val, err);
     const long long o = id[p];
     out[o] = val[0];
     out[B.np + o] = val[1];
