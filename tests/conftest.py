@@ -4,6 +4,15 @@ import sys
 import numpy as np
 import pytest
 
+# torch and the library both link libamdhip64.so.7, and the process keeps the copy
+# loaded first: torch's bundled runtime must be that copy, or torch cannot see the
+# GPU once the library (linked against /opt/rocm) has initialised HIP.  Importing
+# torch does not initialise HIP.
+try:
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover
+    torch = None
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)

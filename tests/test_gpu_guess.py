@@ -41,10 +41,14 @@ def test_extrapolated_guess_meets_the_same_rule(nyx, precision):
     bound = 1e-8 if precision == "fp64" else 1e-6
     for out, g, g64, _ in (prev, extr):
         assert np.linalg.norm(g - g64) / np.linalg.norm(g64) <= bound
+    # trajectories: FP64 to the solver tolerance; mixed interpolates g in FP32,
+    # where phi differing at the solver tolerance can round a corner's g to the
+    # neighbouring FP32 value (~6e-8 relative per step)
+    tol = 1e-9 if precision == "fp64" else 1e-7
     for d in range(3):
         for q in range(2):
             a, b = prev[0][q][d], extr[0][q][d]
-            assert np.abs(a - b).max() <= 1e-9 * max(np.abs(b).max(), 1e-300)
+            assert np.abs(a - b).max() <= tol * max(np.abs(b).max(), 1e-300)
     # no more V-cycles than the previous-phi guess over the run (the first step
     # has no phi_{n-1} and uses phi_n)
     assert sum(extr[3]) <= sum(prev[3])
