@@ -208,6 +208,16 @@ int nyxpm_solve_poisson(nyxpm_ctx *ctx, int box, const double *rhs, double *phi_
  * DarkMatterParticleContainer.cpp:276-292), without changing the particles:
  * g_out holds 3*np doubles, component-major, in upload (id) order. */
 int nyxpm_gather_accel(nyxpm_ctx *ctx, int box, double *g_out);
+/* nyxpm_gather_accel that allocates nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same results as
+ * nyxpm_gather_accel (same kernel).  A host g_out is staged through a fixed
+ * device array of 3 * 2^20 doubles (24 MiB per GPU, held until the process ends)
+ * instead of a buffer allocated per call, so the box may hold at most 2^20
+ * particles (NYXPM_ERR_ARG above that); a device g_out is written directly.
+ * Every context on a GPU shares that GPU's array: calls that overlap on one GPU
+ * (contexts on different host threads or streams) race. */
+int nyxpm_gather_accel_noalloc(nyxpm_ctx *ctx, int box, double *g_out);
 /* NyxParticleContainer::estTimestep(grav, a, 0, cfl)
  * (NyxParticleContainer.H:327-410, called by Nyx::particle_est_time_step,
  * NyxParticles.cpp:1028-1036) with grav = the cell-centred gravity of the current

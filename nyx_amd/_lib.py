@@ -78,6 +78,7 @@ SIGNATURES = [
     ("nyxpm_solve_poisson", _i, [_vp, _i, _vp, _vp, _d, _d, _i, _vp, _vp]),
     ("nyxpm_get_grav_vector", _i, [_vp, _i, _vp]),
     ("nyxpm_gather_accel", _i, [_vp, _i, _vp]),
+    ("nyxpm_gather_accel_noalloc", _i, [_vp, _i, _vp]),
     ("nyxpm_est_time_step", _i, [_vp, _d, _d, _vp]),
     ("nyxpm_get_phi", _i, [_vp, _i, _vp]),
     ("nyxpm_set_phi", _i, [_vp, _i, _vp]),

@@ -16,6 +16,7 @@
 
 #include <cerrno>
 #include <map>
+#include <mutex>
 #include <set>
 #include <string>
 #include <vector>
@@ -26,6 +27,15 @@
 #include "particle_kernels.hip"
 
 using namespace nyxpm;
+
+// Staging of nyxpm_gather_accel_noalloc: the accelerations of up to
+// kNoallocMaxParticles particles (3 doubles each, 24 MiB), one copy of the array
+// per GPU the library runs on, held until the process ends.  Host code never
+// hands the array's name to a kernel or a copy (in host code it is a host-side
+// stand-in, not a device address): it uses the device address
+// hipGetSymbolAddress returns, looked up once per device (bind_scratch).
+constexpr long long kNoallocMaxParticles = 1ll << 20;
+static __device__ __align__(256) double scratch[3 * kNoallocMaxParticles];
 
 namespace {
 
@@ -283,6 +293,10 @@ struct nyxpm_ctx {
     int* h_gstat = nullptr;
 
     long long device_bytes = 0;
+    // the device the context was created on and that device's `scratch` array
+    // (hipGetSymbolAddress; shared by every context on the device)
+    int device = 0;
+    double* scratch_dev = nullptr;
 
     CellSeg* segs(const Table& t) const { return seg_pool + t.first; }
 };
@@ -1568,6 +1582,40 @@ bool is_device_ptr(const void* p) {
     return a.type == hipMemoryTypeDevice;
 }
 
+// Device address of the current device's `scratch` array, looked up once per
+// device with that device current and kept for the process; the context keeps it.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+int bind_scratch(nyxpm_ctx* c) {
+    int dev = 0;
+    HIPCK(c, hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) FAIL(c, NYXPM_ERR_UNSUPPORTED, "device %d above %d", dev, kMaxDevices - 1);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            FAIL(c, NYXPM_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        g_scratch[dev] = static_cast<double*>(p);
+    }
+    c->device = dev;
+    c->scratch_dev = g_scratch[dev];
+    return NYXPM_OK;
+}
+
+// Accelerations of box `box`'s particles (k_gather) into `out`, device memory of
+// 3*np doubles; stream-ordered, no synchronisation.
+int launch_gather(nyxpm_ctx* c, int box, double* out) {
+    const Box& B = c->boxes[box];
+    PartPtrs& A = c->P[c->cur];
+    hipLaunchKernelGGL(c->precision == NYXPM_MIXED ? k_gather<float> : k_gather<double>, dim3(nblocks(B.span)),
+                       dim3(kBlock), 0, c->stream, c->dboxes, box, c->phi, A.x, A.y, A.z, A.id, out, c->err_flag);
+    LAUNCHCK(c);
+    return NYXPM_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1621,6 +1669,7 @@ int nyxpm_create_batch(nyxpm_ctx** out, int nbox, const nyxpm_geom* geoms, const
     if (e != hipSuccess) FAIL(c, NYXPM_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
     c->own_stream = true;
     int rc;
+    if ((rc = bind_scratch(c))) return rc;
     if ((rc = setup_boxes(c, nbox, geoms))) return rc;
     if ((rc = build_tables(c))) return rc;
     if ((rc = alloc_fields(c))) return rc;
@@ -2407,14 +2456,11 @@ int nyxpm_gather_accel(nyxpm_ctx* c, int box, double* g_out) {
     if ((rc = require_particles(c))) return rc;
     const Box& B = c->boxes[box];
     if (B.np == 0) return NYXPM_OK;
-    PartPtrs& A = c->P[c->cur];
     const bool dev = is_device_ptr(g_out);
     double* out = g_out;
     if (!dev && (rc = dalloc(c, &out, 3 * B.np))) return rc;
-    hipLaunchKernelGGL(c->precision == NYXPM_MIXED ? k_gather<float> : k_gather<double>, dim3(nblocks(B.span)),
-                       dim3(kBlock), 0, c->stream, c->dboxes, box, c->phi, A.x, A.y,
-                       A.z, A.id, out, c->err_flag);
-    hipError_t e = hipGetLastError();
+    rc = launch_gather(c, box, out);
+    hipError_t e = rc ? hipErrorLaunchFailure : hipSuccess;
     if (!dev && e == hipSuccess)
         e = hipMemcpyAsync(g_out, out, 3 * B.np * sizeof(double), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -2422,7 +2468,29 @@ int nyxpm_gather_accel(nyxpm_ctx* c, int box, double* g_out) {
         hipFree(out);
         c->device_bytes -= 3 * B.np * (long long)sizeof(double);
     }
+    if (rc) return rc;
     if (e != hipSuccess) FAIL(c, NYXPM_ERR_HIP, "gather_accel: %s", hipGetErrorString(e));
+    return err_check(c);
+}
+
+int nyxpm_gather_accel_noalloc(nyxpm_ctx* c, int box, double* g_out) {
+    if (!c || !g_out) return NYXPM_ERR_ARG;
+    int rc;
+    if ((rc = check_box(c, box))) return rc;
+    if ((rc = require_particles(c))) return rc;
+    const Box& B = c->boxes[box];
+    if (B.np > kNoallocMaxParticles)
+        FAIL(c, NYXPM_ERR_ARG, "gather_accel_noalloc: box %d holds %lld particles, above the %lld its staging array takes",
+             box, B.np, kNoallocMaxParticles);
+    if (B.np == 0) return NYXPM_OK;
+    if (!c->scratch_dev) FAIL(c, NYXPM_ERR_STATE, "gather_accel_noalloc: no staging array on device %d", c->device);
+    // device memory: written directly; host memory: through this device's array
+    // (every element of its first 3*np doubles is written before the copy reads it)
+    const bool dev = is_device_ptr(g_out);
+    if ((rc = launch_gather(c, box, dev ? g_out : c->scratch_dev))) return rc;
+    if (!dev)
+        HIPCK(c, hipMemcpyAsync(g_out, c->scratch_dev, 3 * B.np * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
     return err_check(c);
 }
 

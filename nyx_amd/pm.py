@@ -240,6 +240,13 @@ class PMContext:
         self._ck(self.L.nyxpm_gather_accel(self.h, box, ptr(g)))
         return g
 
+    def gather_accel_noalloc(self, box=0):
+        """gather_accel through nyxpm_gather_accel_noalloc (no device allocation;
+        at most 2^20 particles in the box)."""
+        g = np.empty((3, self.num_particles(box)))
+        self._ck(self.L.nyxpm_gather_accel_noalloc(self.h, box, ptr(g)))
+        return g
+
     def est_time_step(self, a=1.0, cfl=0.5):
         """NyxParticleContainer::estTimestep per box (array of nbox; -1e50 for a box
         without particles)."""
