@@ -120,6 +120,16 @@ typedef struct nyxpm_opts {
                           (nyxpm_pm_steps with use_graph); 0 = default: the most
                           V-cycles a solve of the same call needed eagerly plus 2,
                           re-captured when a replay uses up that margin */
+    int ir_first;      /* mixed precision: FP32 V-cycles of each solve's first
+                          refinement step, run back to back on one FP64 residual
+                          (each continuing from the correction of the one before,
+                          phi += e once) before the first FP64 residual and
+                          stopping test; later steps run one V-cycle each.  Saves
+                          the FP64 residual and update of the cycles in between;
+                          a solve that would stop earlier runs up to ir_first
+                          cycles.  0 = default (5); 1 = a test after every cycle.
+                          Needs nu1 + nu2 even (else 1); FP64 solves and fixed
+                          iteration counts always test after every cycle. */
 } nyxpm_opts;
 
 /* ---- lifecycle ---------------------------------------------------------- */

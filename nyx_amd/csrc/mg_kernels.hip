@@ -728,7 +728,8 @@ template <typename T>
 __global__ void __launch_bounds__(kCoarseThreads) k_coarse_vcycle(const CellSeg* segs, const int* first,
                                                                   const int* active, T* __restrict__ e,
                                                                   const T* __restrict__ r,
-                                                                  const double* __restrict__ G, int nu1, int nu2) {
+                                                                  const double* __restrict__ G, int nu1, int nu2,
+                                                                  int keep_top) {
     extern __shared__ double coarse_smem[];
     T* lds = reinterpret_cast<T*>(coarse_smem);
     const int l0 = first[blockIdx.x], l1 = first[blockIdx.x + 1];
@@ -748,13 +749,17 @@ __global__ void __launch_bounds__(kCoarseThreads) k_coarse_vcycle(const CellSeg*
     };
     auto rptr = [&](int l) -> const T* { return l == l0 ? r + segs[l0].off : lds + roff(l); };
     const int tid = threadIdx.x, nth = blockDim.x;
-    // down: zero guess, nu1 red-black sweeps, residual restricted to the next level
+    // down: zero guess (keep_top: on the box's own level, the e of the previous
+    // V-cycle of the same refinement step), nu1 red-black sweeps, residual
+    // restricted to the next level
     for (int l = l0; l + 1 < l1; ++l) {
         const CellSeg& sg = segs[l];
         T* E = lds + eoff(l);
         const T* R = rptr(l);
         const int nc = ncell(l);
-        for (int q = tid; q < nc; q += nth) E[q] = T(0);
+        const bool keep = keep_top && l == l0 && sg.lev == 0;
+        const T* Eg = e + sg.off;
+        for (int q = tid; q < nc; q += nth) E[q] = keep ? Eg[q] : T(0);
         __syncthreads();
         for (int it = 0; it < nu1; ++it)
             for (int col = 0; col < 2; ++col) {
@@ -984,7 +989,7 @@ __global__ void __launch_bounds__(kBlock) k_update0(const CellSeg* segs, int nse
 // state: [0] resnorm0 [1] rhsnorm [2] final [3] target
 __global__ void k_check(int nbox, int init, int fixed_iters, int max_iter, double rel_tol,
                         double abs_tol, int* active, int* iters, int* failed,
-                        unsigned long long* nr, unsigned long long* nf, double* state) {
+                        unsigned long long* nr, unsigned long long* nf, double* state, int inc) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nbox) return;
     unsigned long long br = 0ull, bf = 0ull;
@@ -1010,7 +1015,7 @@ __global__ void k_check(int nbox, int init, int fixed_iters, int max_iter, doubl
         active[b] = fixed_iters > 0 ? 1 : (r0 > target ? 1 : 0);
     } else if (active[b]) {
         const double rn = __longlong_as_double((long long)br);
-        const int it = iters[b] + 1;
+        const int it = iters[b] + inc;  // the V-cycles run since the last check
         iters[b] = it;
         state[4 * b + 2] = rn;
         if (fixed_iters > 0) {

@@ -139,6 +139,8 @@ constexpr int kGraphMargin = 2;
 
 }  // namespace
 
+constexpr int kIrFirstDefault = 5;
+
 struct nyxpm_ctx {
     std::string err;
     int verbose = 0;
@@ -147,6 +149,9 @@ struct nyxpm_ctx {
     // V-cycles recorded per solve in a captured step: fixed (> 0) or, with 0, the
     // most cycles of the solves since the call began plus kGraphMargin
     int graph_cycles = 0;
+    // mixed precision: FP32 V-cycles of a solve's first refinement step before its
+    // first FP64 residual check (the correction carried over, phi updated once)
+    int ir_first = kIrFirstDefault;
     int iters_seen = 0;  // most V-cycles of a box in the solves of the current call
     bool no_fused = false;  // force half-sweep smoothing (for comparisons)
     bool untiled_kick = false;  // force the thread-per-particle kick (for comparisons)
@@ -510,6 +515,7 @@ int build_tables(nyxpm_ctx* c) {
         s.off = L.off;
         s.coff = (l + 1 < B.nlev) ? B.lev[l + 1].off : 0;
         s.foff = B.cell_off;
+        s.lev = l;
         s.nblk = nblocks(per);
         return s;
     };
@@ -1151,8 +1157,12 @@ void launch_sweep(nyxpm_ctx* c, int d, const T* ein, T* eout, const T* r, const 
 // One V-cycle on every active box.  Returns in *updated whether phi += e has
 // already been applied (fused into the last finest-level sweep).  With
 // sweep0_done the first level-0 sweep was already run (by residual_sweep0).
+// first = false: a further V-cycle on the same residual (the FP32 cycles of one
+// refinement step in mixed precision): the level-0 correction starts from the
+// previous cycle's instead of zero.  last = false: phi += e is left for a later
+// cycle (no update fused into this one).
 template <typename T>
-int vcycle(nyxpm_ctx* c, bool* updated, bool sweep0_done = false) {
+int vcycle(nyxpm_ctx* c, bool* updated, bool sweep0_done = false, bool first = true, bool last = true) {
     T* r = Pools<T>::r(c);
     T* eb[2] = {Pools<T>::e(c), Pools<T>::e2(c)};
     const int pr = sizeof(T) == 8 ? 0 : 1;
@@ -1163,18 +1173,30 @@ int vcycle(nyxpm_ctx* c, bool* updated, bool sweep0_done = false) {
         auto& th = c->t_half[d];
         if (tf.count == 0) continue;
         int cur = 0;
+        const bool keep = !first && d == 0;
         if (c->sw[pr][d].tx) {
-            if (!(sweep0_done && d == 0)) {
-                Timed t(c, lvl_name("sweep0", d));
-                launch_sweep<T, kSweepZero>(c, d, eb[0], eb[0], r, eb[0], eb[1]);
-            }
-            for (int q = 1; q < c->nu1; ++q) {
-                Timed t(c, lvl_name("sweep", d));
-                launch_sweep<T, 0>(c, d, eb[cur], eb[1 - cur], r, eb[0], eb[1]);
-                cur ^= 1;
+            if (keep) {
+                // the previous cycle left e in fin_buf; nu1 + nu2 is even, so the
+                // nu1 sweeps end in pre_buf as after a cycle from zero
+                cur = c->fin_buf[0];
+                for (int q = 0; q < c->nu1; ++q) {
+                    Timed t(c, lvl_name("sweep", d));
+                    launch_sweep<T, 0>(c, d, eb[cur], eb[1 - cur], r, eb[0], eb[1]);
+                    cur ^= 1;
+                }
+            } else {
+                if (!(sweep0_done && d == 0)) {
+                    Timed t(c, lvl_name("sweep0", d));
+                    launch_sweep<T, kSweepZero>(c, d, eb[0], eb[0], r, eb[0], eb[1]);
+                }
+                for (int q = 1; q < c->nu1; ++q) {
+                    Timed t(c, lvl_name("sweep", d));
+                    launch_sweep<T, 0>(c, d, eb[cur], eb[1 - cur], r, eb[0], eb[1]);
+                    cur ^= 1;
+                }
             }
         } else {
-            {
+            if (!keep) {
                 Timed t(c, lvl_name("fill", d));
                 hipLaunchKernelGGL(k_fill<T>, dim3(tf.blocks), dim3(kBlock), 0, c->stream, c->segs(tf), tf.count,
                                    c->active, eb[0], T(0));
@@ -1207,7 +1229,7 @@ int vcycle(nyxpm_ctx* c, bool* updated, bool sweep0_done = false) {
         Timed t(c, "coarse");
         hipLaunchKernelGGL(k_coarse_vcycle<T>, dim3((unsigned)c->boxes.size()), dim3(kCoarseThreads),
                            c->cc_lds[sizeof(T) == 8 ? 0 : 1], c->stream, c->segs(c->t_cc), c->cc_first, c->active,
-                           eb[0], r, c->G64, c->nu1, c->nu2);
+                           eb[0], r, c->G64, c->nu1, c->nu2, first ? 0 : 1);
     }
     for (int d = D - 2; d >= 0; --d) {
         auto& tf = c->t_full[d];
@@ -1218,11 +1240,12 @@ int vcycle(nyxpm_ctx* c, bool* updated, bool sweep0_done = false) {
             // prolongation fused into the first sweep; phi += e into the last one
             // on the finest level
             for (int q = 0; q < c->nu2; ++q) {
-                const bool first = q == 0, last = q + 1 == c->nu2, upd = last && d == 0 && c->fuse_update;
-                Timed t(c, lvl_name(first ? "sweep_p" : (upd ? "sweep_u" : "sweep"), d));
-                if (first && upd)
+                const bool firstq = q == 0, lastq = q + 1 == c->nu2;
+                const bool upd = lastq && last && d == 0 && c->fuse_update;
+                Timed t(c, lvl_name(firstq ? "sweep_p" : (upd ? "sweep_u" : "sweep"), d));
+                if (firstq && upd)
                     launch_sweep<T, kSweepProlong | kSweepUpdate>(c, d, eb[cur], eb[1 - cur], r, eb[0], eb[1]);
-                else if (first)
+                else if (firstq)
                     launch_sweep<T, kSweepProlong>(c, d, eb[cur], eb[1 - cur], r, eb[0], eb[1]);
                 else if (upd)
                     launch_sweep<T, kSweepUpdate>(c, d, eb[cur], eb[1 - cur], r, eb[0], eb[1]);
@@ -1296,10 +1319,11 @@ int update(nyxpm_ctx* c) {
     return NYXPM_OK;
 }
 
-int check(nyxpm_ctx* c, bool init, int fixed_iters, double rel, double abs_tol) {
+// inc: the V-cycles run since the previous check (added to the per-box count)
+int check(nyxpm_ctx* c, bool init, int fixed_iters, double rel, double abs_tol, int inc = 1) {
     const int nbox = (int)c->boxes.size();
     hipLaunchKernelGGL(k_check, dim3(nblocks(nbox, 64)), dim3(64), 0, c->stream, nbox, init ? 1 : 0, fixed_iters,
-                       c->max_iter, rel, abs_tol, c->active, c->iters, c->failed, c->nr, c->nf, c->state);
+                       c->max_iter, rel, abs_tol, c->active, c->iters, c->failed, c->nr, c->nf, c->state, inc);
     LAUNCHCK(c);
     return NYXPM_OK;
 }
@@ -1317,6 +1341,29 @@ int read_flags(nyxpm_ctx* c, bool* any_active) {
 // One multigrid solve of L phi = f on every box, from the current phi.  With
 // `graph_cycles` > 0 the number of cycles issued is fixed (device flags stop
 // converged boxes) and nothing is read back, so the sequence can be captured.
+// V-cycles of the first refinement step of a solve before its first FP64
+// residual check: c->ir_first FP32 cycles in mixed precision (the correction
+// carried from cycle to cycle, phi updated once), one otherwise and with a fixed
+// iteration count.  Later steps run one cycle each.
+template <typename T>
+int first_group(const nyxpm_ctx* c, int fixed_iters) {
+    if (sizeof(T) == 8 || fixed_iters > 0 || c->ir_first <= 1 || (c->nu1 + c->nu2) % 2) return 1;
+    return std::min(c->ir_first, c->max_iter);
+}
+
+// g V-cycles on the current residual (the later ones continue from the
+// correction of the earlier ones), then phi += e; the caller forms the next
+// residual.  rs: the first level-0 sweep of the first cycle was run already.
+template <typename T>
+int vcycle_group(nyxpm_ctx* c, int g, bool rs) {
+    int rc;
+    bool upd = false;
+    for (int q = 0; q < g; ++q)
+        if ((rc = vcycle<T>(c, &upd, rs && q == 0, q == 0, q + 1 == g))) return rc;
+    if (!upd && (rc = update<T>(c))) return rc;
+    return NYXPM_OK;
+}
+
 template <typename T>
 int solve_impl(nyxpm_ctx* c, double rel, double abs_tol, int fixed_iters, int graph_cycles) {
     const int nbox = (int)c->boxes.size();
@@ -1325,16 +1372,16 @@ int solve_impl(nyxpm_ctx* c, double rel, double abs_tol, int fixed_iters, int gr
     // next V-cycle (wasted only after the last one)
     const bool rs = residual_in_sweep<T>(c);
     auto res = [&](bool with_f) { return rs ? residual_sweep0<T>(c, with_f) : residual<T>(c, with_f); };
+    const int g0 = first_group<T>(c, fixed_iters);
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(kBlock), 0, c->stream, c->active, (long long)nbox, 1);
     if ((rc = res(true))) return rc;
     if ((rc = check(c, true, fixed_iters, rel, abs_tol))) return rc;
     if (graph_cycles > 0) {
-        for (int it = 0; it < graph_cycles; ++it) {
-            bool upd;
-            if ((rc = vcycle<T>(c, &upd, rs))) return rc;
-            if (!upd && (rc = update<T>(c))) return rc;
+        // the same schedule as eagerly, graph_cycles V-cycles in all
+        for (int done = 0, g = std::min(g0, graph_cycles); done < graph_cycles; done += g, g = 1) {
+            if ((rc = vcycle_group<T>(c, g, rs))) return rc;
             if ((rc = res(false))) return rc;
-            if ((rc = check(c, false, fixed_iters, rel, abs_tol))) return rc;
+            if ((rc = check(c, false, fixed_iters, rel, abs_tol, g))) return rc;
         }
         // a box still active now did not converge within the recorded cycles:
         // kept in gstat, which the next solve of the replay does not reset
@@ -1344,15 +1391,13 @@ int solve_impl(nyxpm_ctx* c, double rel, double abs_tol, int fixed_iters, int gr
         return NYXPM_OK;
     }
     const int limit = fixed_iters > 0 ? fixed_iters : c->max_iter;
-    for (int it = 0; it <= limit; ++it) {
+    for (int it = 0, g = g0; it <= limit; it += g, g = 1) {
         bool any;
         if ((rc = read_flags(c, &any))) return rc;
         if (!any) break;
-        bool upd;
-        if ((rc = vcycle<T>(c, &upd, rs))) return rc;
-        if (!upd && (rc = update<T>(c))) return rc;
+        if ((rc = vcycle_group<T>(c, g, rs))) return rc;
         if ((rc = res(false))) return rc;
-        if ((rc = check(c, false, fixed_iters, rel, abs_tol))) return rc;
+        if ((rc = check(c, false, fixed_iters, rel, abs_tol, g))) return rc;
     }
     return NYXPM_OK;
 }
@@ -1651,6 +1696,7 @@ int nyxpm_create_batch(nyxpm_ctx** out, int nbox, const nyxpm_geom* geoms, const
         if (opts->phi_guess >= 0 && opts->phi_guess <= 2) c->phi_guess = opts->phi_guess;
         c->fuse_res = opts->residual == 1;
         if (opts->graph_cycles > 0) c->graph_cycles = opts->graph_cycles;
+        if (opts->ir_first > 0) c->ir_first = opts->ir_first;
         if (opts->resort_slack < 0) {
             c->slack_num = 0;
             c->slack_pad = 0;

@@ -62,6 +62,7 @@ struct CellSeg {
     int zchunk;            // z-planes per block of the fused sweep
     int ebuf;              // which correction buffer (0: e, 1: e2) holds the coarse
                            // level's result (prolong) / the level's result (update)
+    int lev;               // multigrid level of this piece (0: the box's own cells)
     double dh[3];          // 1/dx^2 per dimension on this level
     double gamma;          // -2 (dhx + dhy + dhz)
     double inv_gamma;      // 1/gamma
