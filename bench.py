@@ -307,6 +307,11 @@ def main():
                     help="instead of running: summarise a rocprofv3 kernel trace of a bench run over that "
                          "run's timed mixed steps (needs --from-json with the run's JSON line)")
     ap.add_argument("--from-json", default=None)
+    ap.add_argument("--pmc-summary", nargs=4, default=None,
+                    metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
+                    help="instead of running: HBM traffic per launch of each kernel (largest grid) from "
+                         "rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE counter collections of a 512^3 bench run, "
+                         "scaled by the same counters of a 1 GiB copy, against the kernel's algorithmic bytes")
     ap.add_argument("--workload", default="clustered512",
                     choices=["clustered512", "batch48", "grf256", "grf1024"],
                     help="clustered512: the headline line; batch48: 48 boxes (16 each of 64^3, 96^3, "
@@ -316,6 +321,8 @@ def main():
     a = ap.parse_args()
     if a.trace_summary:
         return trace_summary(a.trace_summary, a.from_json)
+    if a.pmc_summary:
+        return pmc_summary(*a.pmc_summary)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md, "
               "'How the path would shard'); run with --gpus 1", file=sys.stderr)
@@ -576,6 +583,52 @@ def trace_summary(csv_path, json_path):
                             f"{gbs} | {frac} |")
         out_rows.append("")
     print("\n".join(out_rows))
+
+
+def pmc_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, n=512):
+    """Per kernel (its launches with the largest grid: the finest level) the mean
+    FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of a rocprofv3 counter
+    collection, each scaled by what the counter reads for a copy of 1 GiB
+    (__amd_rocclr_copyBuffer), and the ratio to the algorithmic bytes."""
+    import csv
+
+    def load(path):
+        rows = {}
+        with open(path) as fh:
+            for r in csv.DictReader(fh):
+                nm = r["Kernel_Name"].split("(")[0].replace("void ", "").replace("nyxpm::", "")
+                rows.setdefault((nm, int(r["Grid_Size"])), []).append(float(r["Counter_Value"]))
+        return rows
+
+    def copy_kib(path):
+        with open(path) as fh:
+            for r in csv.DictReader(fh):
+                if "copyBuffer" in r["Kernel_Name"]:
+                    return float(r["Counter_Value"])
+        raise SystemExit(f"{path}: no __amd_rocclr_copyBuffer dispatch")
+
+    F, W = load(fetch_csv), load(write_csv)
+    gib_kib = 2.0 ** 30 / 1024.0
+    sf, sw = gib_kib / copy_kib(copy_fetch_csv), gib_kib / copy_kib(copy_write_csv)
+    N = n ** 3
+    halo = tile_halo_cells((n, n, n), TILE)
+    best = {}
+    for (nm, grid), v in F.items():
+        if nm not in best or grid > best[nm][0]:
+            best[nm] = (grid, v)
+    print(f"calibration: a 1 GiB copy reads FETCH_SIZE x{sf:.3f}, WRITE_SIZE x{sw:.3f} of its bytes\n")
+    print("| kernel | grid | dispatches | FETCH KiB | WRITE KiB | traffic GB (scaled) | algorithmic GB | ratio |")
+    print("|---|---|---|---|---|---|---|---|")
+    for nm, (grid, fv) in sorted(best.items()):
+        wv = W.get((nm, grid), [0.0])
+        fk, wk = sum(fv) / len(fv), sum(wv) / len(wv)
+        gb = (fk * sf + wk * sw) * 1024.0 / 1e9
+        stage, _ = _stage_of_kernel(nm)
+        sz = 4 if "float" in nm else 8
+        key = stage + ".d0" if stage and (stage.startswith("sweep") or stage == "resid_restrict") else stage
+        alg = kernel_bytes(key, N, N, halo, sz) if key else None
+        tail = f"{alg / 1e9:.3f} | {gb / (alg / 1e9):.2f} |" if alg else " | |"
+        print(f"| `{nm}` | {grid} | {len(fv)} | {fk:.4g} | {wk:.4g} | {gb:.3f} | " + tail)
 
 
 def emit(out, a):

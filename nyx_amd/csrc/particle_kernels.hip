@@ -275,19 +275,13 @@ __device__ __forceinline__ int halo_index(int lx, int ly, int lz, int bx, int by
     return fx + fy + (ly - 1) * bx + (lx - 1);
 }
 
-// A positive FP32 value w < 2^(E+1) in units 2^(E-32), rounded to nearest (ties
-// up): w = mant * 2^(e-150) for the biased exponent e and the 24-bit mantissa,
-// so the count is mant shifted by e - 118 - E (at most 9 up: below 2^33).
-// Integer operations only; 0 for zero and for values below half a unit.
-__device__ __forceinline__ unsigned long long fx32_units(float w, int E) {
-    const unsigned bits = __float_as_uint(w);
-    const int e = (int)(bits >> 23) & 0xff;
-    if (e == 0) return 0ull;  // zero (or a subnormal: far below the unit)
-    const unsigned long long mant = (unsigned long long)((bits & 0x7fffffu) | 0x800000u);
-    const int sh = e - 118 - E;
-    if (sh >= 0) return mant << sh;
-    if (sh < -40) return 0ull;
-    return (mant + (1ull << (-sh - 1))) >> (-sh);
+// A non-negative FP32 value w < 2^(E+1) in units 2^(E-31), rounded to nearest:
+// w * 2^(31-E) < 2^32 is exact in FP32 (a power-of-two scale, `scale`), and the
+// native f32 -> u32 conversion (truncating) of it plus one half rounds it (ties
+// as FP32 rounds the sum; a value that rounds up to 2^32 saturates at 2^32 - 1).
+// Four instructions per contribution.
+__device__ __forceinline__ unsigned fx32_units(float w, float scale) {
+    return __float2uint_rz(w * scale + 0.5f);
 }
 
 // One block per tile.  Contributions m*sx[ii]*sy[jj]*sz[kk] (amrex_deposit_cic
@@ -296,11 +290,11 @@ __device__ __forceinline__ unsigned long long fx32_units(float w, int E) {
 //
 // F32 (the mixed-precision step): the fractional weights and the product are
 // FP32 (the cell index and the fraction come from the FP64 position as in FP64
-// mode), and each contribution is one exact 64-bit word in units 2^(E-32)
-// (fx32_units), so the LDS sum is still exact and order independent, with half
-// the atomics: a cell's sum of at most 2^31 contributions below 2^33 units
-// each stays below 2^64.  Per contribution the FP32 rounding (~6e-8 relative)
-// dominates the unit (2^-32 of the largest mass).
+// mode), and each contribution is one 32-bit count of units 2^(E-31)
+// (fx32_units) added into the cell's 64-bit word, so the LDS sum is still exact
+// and order independent, with half the atomics: a cell's sum of at most 2^32
+// contributions below 2^32 units each stays below 2^64.  Per contribution the
+// FP32 rounding (~6e-8 relative) dominates the unit (2^-31 of the largest mass).
 template <bool F32>
 __global__ void __launch_bounds__(kBlock) k_deposit(const BoxSeg* segs, int nseg, const BoxDev* boxes,
                                                     const long long* __restrict__ tbase,
@@ -361,14 +355,22 @@ __global__ void __launch_bounds__(kBlock) k_deposit(const BoxSeg* segs, int nseg
         const int i = cic_index(px, B.plo[0], B.dxi[0], &xint);
         const int j = cic_index(py, B.plo[1], B.dxi[1], &yint);
         const int k = cic_index(pz, B.plo[2], B.dxi[2], &zint);
-        const int iw = wrap_checked(i, B.n[0], err) - o0;
-        const int jw = wrap_checked(j, B.n[1], err) - o1;
-        const int kw = wrap_checked(k, B.n[2], err) - o2;
-        if (iw < 0 || iw >= bx || jw < 0 || jw >= by || kw < 0 || kw >= bz) {
-            atomicOr(err, 2);
+        // an index more than one period out (flag 1) or a cell outside the tile
+        // the particle is binned in (flag 2): one branch for both, rarely taken
+        const bool far = (unsigned)(i + B.n[0]) >= (unsigned)(3 * B.n[0]) ||
+                         (unsigned)(j + B.n[1]) >= (unsigned)(3 * B.n[1]) ||
+                         (unsigned)(k + B.n[2]) >= (unsigned)(3 * B.n[2]);
+        const int iw = wrap1(i, B.n[0]) - o0;
+        const int jw = wrap1(j, B.n[1]) - o1;
+        const int kw = wrap1(k, B.n[2]) - o2;
+        const bool outside = (unsigned)iw >= (unsigned)bx || (unsigned)jw >= (unsigned)by ||
+                             (unsigned)kw >= (unsigned)bz;
+        if (far || outside) {
+            atomicOr(err, (far ? 1 : 0) | (outside ? 2 : 0));
             return;
         }
         if (F32) {
+            const float scale = ldexpf(1.0f, 31 - E);
             const float fx = (float)xint, fy = (float)yint, fz = (float)zint, fm = (float)mp;
             const float sx[2] = {1.0f - fx, fx};
             const float sy[2] = {1.0f - fy, fy};
@@ -381,7 +383,7 @@ __global__ void __launch_bounds__(kBlock) k_deposit(const BoxSeg* segs, int nseg
                     for (int ii = 0; ii < 2; ++ii) {
                         const float w = sx[ii] * sy[jj] * sz[kk] * fm;
                         const int idx = ((kw + kk) * Ly + (jw + jj)) * Lx + (iw + ii);
-                        atomicAdd(&acc[idx], fx32_units(w, E));
+                        atomicAdd(&acc[idx], (unsigned long long)fx32_units(w, scale));
                     }
             return;
         }
@@ -413,14 +415,31 @@ __global__ void __launch_bounds__(kBlock) k_deposit(const BoxSeg* segs, int nseg
         if (pa + kBlock < p1) dep(xb, yb, zb, mb);
     }
     __syncthreads();
-    for (int c = threadIdx.x; c < LC; c += kBlock) {
-        const int lx = c % Lx, ly = (c / Lx) % Ly, lz = c / (Lx * Ly);
-        const double v = F32 ? ldexp((double)acc[c], E - 32)
-                             : ldexp((double)acc[c], E - 31) + ldexp((double)acc[LC + c], E - 63);
-        if (lx > 0 && ly > 0 && lz > 0)
-            own[B.cell_off + cidx(o0 + lx - 1, o1 + ly - 1, o2 + lz - 1, B.n[0], B.n[1])] = v;
-        else
-            halo[gt * B.halo_stride + halo_index(lx, ly, lz, bx, by, bz)] = v;
+    // write-out by rows of Lx cells (Lx <= 33 < 64): a wave takes rpw rows per
+    // pass, so no cell needs a division for its coordinates
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int rpw = kWave / Lx;
+    const int lrow = lane / Lx, lx = lane - lrow * Lx;
+    if (lrow < rpw) {
+        const int step = (kBlock / kWave) * rpw;
+        const int r0 = wave * rpw + lrow;
+        int ly = r0 % Ly, lz = r0 / Ly;
+        const int dly = step % Ly, dlz = step / Ly;
+        while (lz < Lz) {
+            const int c = (lz * Ly + ly) * Lx + lx;
+            const double v = F32 ? ldexp((double)acc[c], E - 31)
+                                 : ldexp((double)acc[c], E - 31) + ldexp((double)acc[LC + c], E - 63);
+            if (lx > 0 && ly > 0 && lz > 0)
+                own[B.cell_off + cidx(o0 + lx - 1, o1 + ly - 1, o2 + lz - 1, B.n[0], B.n[1])] = v;
+            else
+                halo[gt * B.halo_stride + halo_index(lx, ly, lz, bx, by, bz)] = v;
+            ly += dly;
+            lz += dlz;
+            if (ly >= Ly) {
+                ly -= Ly;
+                ++lz;
+            }
+        }
     }
 }
 
