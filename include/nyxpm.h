@@ -130,6 +130,11 @@ typedef struct nyxpm_opts {
                           cycles.  0 = default (5); 1 = a test after every cycle.
                           Needs nu1 + nu2 even (else 1); FP64 solves and fixed
                           iteration counts always test after every cycle. */
+    int sweep2;        /* FP32 levels with V(2,2) cycles: each pair of smoothing
+                          sweeps as one pass over the level (e and r read once, e
+                          written once; same results bit for bit).  0 = default
+                          (on, 128 z-planes per block), > 1 = on with that many
+                          z-planes per block, < 0 = off. */
 } nyxpm_opts;
 
 /* ---- lifecycle ---------------------------------------------------------- */

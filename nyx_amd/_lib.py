@@ -45,7 +45,8 @@ class Opts(ctypes.Structure):
                 ("sweep_ty", ctypes.c_int), ("sweep_zchunk", ctypes.c_int), ("kick_chunk", ctypes.c_int),
                 ("separate_kicks", ctypes.c_int), ("coarse", ctypes.c_int), ("particle_cfl", ctypes.c_double),
                 ("resort_slack", ctypes.c_double), ("phi_guess", ctypes.c_int), ("residual", ctypes.c_int),
-                ("graph_cycles", ctypes.c_int), ("ir_first", ctypes.c_int)]
+                ("graph_cycles", ctypes.c_int), ("ir_first", ctypes.c_int),
+                ("sweep2", ctypes.c_int)]
 
 
 # (name, restype, argtypes) for every function in include/nyxpm.h

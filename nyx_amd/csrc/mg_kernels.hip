@@ -475,6 +475,201 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
     }
 }
 
+// Two full red-black Gauss-Seidel sweeps in one pass (temporal blocking): the
+// pre-smoothing pair or the post-smoothing pair of a V(2,2) cycle on a fine
+// level reads e and r once and writes e once, where two k_rb_sweep passes move
+// each of them twice.  A block owns a TX x TY column of cells and a chunk of
+// z-planes and marches up in z.  The input e planes are kept with a halo of 4
+// in x and y in a ring of 8 LDS planes, r with a halo of 3 in a ring of 5, and
+// each new plane k runs four in-place steps, one barrier apart: red of sweep 1
+// on plane k (halo-3 region), black of sweep 1 on plane k-1 (halo 2), red of
+// sweep 2 on plane k-2 (halo 1), black of sweep 2 on plane k-3 (the block's own
+// cells), after which plane k-3 is final and written out.  Each step reads only
+// values in the state the two separate sweeps would read (cells of one colour
+// change state only in their own step, and the planes a step reads are those
+// its predecessors have finished), and the halo cells are computed redundantly
+// from the same values by every block that holds them, so the results equal
+// two k_rb_sweep passes (and four half sweeps) bit for bit.  The chunk [z0, z1)
+// needs planes z0-4 .. z1+3.  (Pairing red of sweep 1 with red of sweep 2 three
+// planes behind, and the blacks likewise, needs two barriers per plane instead
+// of four but measured 13% slower: two updates per thread in a row.)
+// MODE bits as k_rb_sweep's: kSweepZero (sweep 1 from e = 0), kSweepProlong
+// (sweep 1's input is ein plus the coarse correction), kSweepUpdate (sweep 2's
+// result is added to phi instead of stored).  Needs nx % TX == 0, ny % TY == 0,
+// even nx, ny, nz and nz >= 8.  HBM bytes per cell: 3*sizeof(T) (e read, r
+// read, e write) for both sweeps.
+constexpr int kSw2TX = 32, kSw2TY = 16, kSw2NT = 512;
+template <typename T, int TX, int TY, int MODE, int NT>
+__global__ void __launch_bounds__(NT) k_rb_sweep2(const CellSeg* segs, int nseg, const int* active,
+                                                  const T* __restrict__ ein, T* __restrict__ eout,
+                                                  const T* __restrict__ r, const T* ec0, const T* ec1,
+                                                  double* phi) {
+    constexpr bool ZERO_IN = (MODE & kSweepZero) != 0;
+    constexpr bool PRO = (MODE & kSweepProlong) != 0;
+    constexpr bool UPD = (MODE & kSweepUpdate) != 0;
+    static_assert(NT >= TX * TY / 2, "one thread per cell pair of the block's own cells");
+    constexpr int EX = TX + 8, EY = TY + 8;   // e planes: halo 4
+    constexpr int RX = TX + 6, RY = TY + 6;   // r planes: halo 3
+    constexpr int NE = (EX * EY + NT - 1) / NT, NR = (RX * RY + NT - 1) / NT;
+    // ring lengths (planes): the e plane stored at the top of iteration k (plane
+    // k+1) replaces plane k-7, last read two iterations earlier; the r plane
+    // (plane k) replaces plane k-5, likewise
+    constexpr int ER = 8, RR = 5;
+    __shared__ T se[ER][EY][EX];
+    __shared__ T sr[RR][RY][RX];
+    const int bid = xcd_block(blockIdx.x, gridDim.x);
+    const int s = find_seg(segs, nseg, bid);
+    const CellSeg& sg = segs[s];
+    if (!active[sg.box]) return;
+    const int nx = sg.n[0], ny = sg.n[1], nz = sg.n[2];
+    const int ntx = nx / TX, nty = ny / TY;
+    const int zc = sg.zchunk;
+    const int b = bid - sg.blk0;
+    const int tx = b % ntx, ty = (b / ntx) % nty, tz = b / (ntx * nty);
+    const int x0 = tx * TX, y0 = ty * TY, z0 = tz * zc;
+    const int z1 = min(nz, z0 + zc);
+    const T dhx = T(sg.dh[0]), dhy = T(sg.dh[1]), dhz = T(sg.dh[2]);
+    const T gamma = T(sg.gamma), inv_gamma = T(sg.inv_gamma);
+    const T* Ein = ein + sg.off;
+    T* Eout = eout + sg.off;
+    const T* R = r + sg.off;
+    const T* Ec = (sg.ebuf ? ec1 : ec0) + sg.coff;
+    const int cnx = nx >> 1, cny = ny >> 1;
+    double* Ph = phi + sg.foff;
+    const int tid = threadIdx.x;
+    const size_t plane = (size_t)nx * ny, cplane = (size_t)cnx * cny;
+    auto wrapz = [&](int k) { return k < 0 ? k + nz : (k >= nz ? k - nz : k); };
+
+    // this thread's elements of the e and r regions (indices past the region
+    // clamped onto its last element: the extra lanes load and store the owner's
+    // value), their in-plane offsets with the periodic wrap
+    size_t eoff[NE], roff[NR], coff_[PRO ? NE : 1];
+#pragma unroll
+    for (int u = 0; u < NE; ++u) {
+        const int idx = min(tid + u * NT, EX * EY - 1);
+        const int ly = idx / EX, lx = idx - ly * EX;
+        int gx = x0 + lx - 4, gy = y0 + ly - 4;
+        gx = gx < 0 ? gx + nx : (gx >= nx ? gx - nx : gx);
+        gy = gy < 0 ? gy + ny : (gy >= ny ? gy - ny : gy);
+        eoff[u] = (size_t)gy * nx + gx;
+        if (PRO) coff_[u] = (size_t)(gy >> 1) * cnx + (gx >> 1);
+    }
+#pragma unroll
+    for (int u = 0; u < NR; ++u) {
+        const int idx = min(tid + u * NT, RX * RY - 1);
+        const int ly = idx / RX, lx = idx - ly * RX;
+        int gx = x0 + lx - 3, gy = y0 + ly - 3;
+        gx = gx < 0 ? gx + nx : (gx >= nx ? gx - nx : gx);
+        gy = gy < 0 ? gy + ny : (gy >= ny ? gy - ny : gy);
+        roff[u] = (size_t)gy * nx + gx;
+    }
+    T pe[NE], pr[NR];
+    auto fetch_e = [&](int kz) {
+        if (ZERO_IN) return;
+        const int kw = wrapz(kz);
+#pragma unroll
+        for (int u = 0; u < NE; ++u) {
+            pe[u] = Ein[(size_t)kw * plane + eoff[u]];
+            if (PRO) pe[u] = pe[u] + Ec[(size_t)(kw >> 1) * cplane + coff_[u]];
+        }
+    };
+    auto put_e = [&](int kz) {
+        if (ZERO_IN) return;  // sweep 1 reads literal zeros for its input
+        T* dst = &se[kz & (ER - 1)][0][0];
+#pragma unroll
+        for (int u = 0; u < NE; ++u) dst[min(tid + u * NT, EX * EY - 1)] = pe[u];
+    };
+    auto fetch_r = [&](int kz) {
+        const int kw = wrapz(kz);
+#pragma unroll
+        for (int u = 0; u < NR; ++u) pr[u] = R[(size_t)kw * plane + roff[u]];
+    };
+    auto put_r = [&](int slot) {
+        T* dst = &sr[slot][0][0];
+#pragma unroll
+        for (int u = 0; u < NR; ++u) dst[min(tid + u * NT, RX * RY - 1)] = pr[u];
+    };
+    // one colour (c: 0 red, 1 black) of plane p on the halo-h region: the
+    // k_gsrb_half update in place.  zred: sweep 1's red step from zero input
+    // (every e operand an input zero); zblack: sweep 1's black step from zero
+    // input (the cell's own input zero)
+    auto colour = [&](int p, int c, int h, int rslot, bool zred, bool zblack) {
+        const int sm = (p - 1) & (ER - 1), sc = p & (ER - 1), sp = (p + 1) & (ER - 1);
+        const int WX = TX + 2 * h, WY = TY + 2 * h, HX = WX / 2;
+        for (int idx = tid; idx < HX * WY; idx += NT) {
+            const int ly = idx / HX, mm = idx - ly * HX;
+            const int lx = 2 * mm + ((c + x0 + y0 + ly + p) & 1);  // (gx + gy + p) % 2 == c
+            const int ex = lx + 4 - h, ey = ly + 4 - h;
+            const int rx = lx + 3 - h, ry = ly + 3 - h;
+            if (zred) {
+                const T ec = T(0);
+                const T res = sr[rslot][ry][rx] - gamma * ec - dhx * (T(0) + T(0)) - dhy * (T(0) + T(0)) -
+                              dhz * (T(0) + T(0));
+                se[sc][ey][ex] = ec + res * inv_gamma;
+            } else {
+                const T ec = zblack ? T(0) : se[sc][ey][ex];
+                const T res = sr[rslot][ry][rx] - gamma * ec - dhx * (se[sc][ey][ex - 1] + se[sc][ey][ex + 1]) -
+                              dhy * (se[sc][ey - 1][ex] + se[sc][ey + 1][ex]) -
+                              dhz * (se[sm][ey][ex] + se[sp][ey][ex]);
+                se[sc][ey][ex] = ec + res * inv_gamma;
+            }
+        }
+    };
+    // r ring slot of plane k, kept incrementally (plane z0 - 3 -> slot 0)
+    int rk = 0;
+    auto rprev = [&](int sl, int d) { sl -= d; return sl < 0 ? sl + RR : sl; };
+
+    fetch_e(z0 - 4);
+    put_e(z0 - 4);
+    fetch_e(z0 - 3);
+    put_e(z0 - 3);
+    fetch_e(z0 - 2);
+    fetch_r(z0 - 3);
+    for (int k = z0 - 3; k <= z1 + 2; ++k) {
+        put_e(k + 1);
+        put_r(rk);
+        fetch_e(k + 2);
+        fetch_r(k + 1);
+        __syncthreads();
+        // sweep 1, red, plane k (needed for every k here)
+        colour(k, 0, 3, rk, ZERO_IN, false);
+        __syncthreads();
+        // sweep 1, black, plane k-1 (needed from z0 - 2)
+        if (k - 1 >= z0 - 2) colour(k - 1, 1, 2, rprev(rk, 1), false, ZERO_IN);
+        __syncthreads();
+        // sweep 2, red, plane k-2 (needed from z0 - 1)
+        if (k - 2 >= z0 - 1) colour(k - 2, 0, 1, rprev(rk, 2), false, false);
+        __syncthreads();
+        // sweep 2, black, plane k-3 on the block's own cells, and the plane out
+        const int kb = k - 3;
+        if (kb >= z0 && tid < TX * TY / 2) {
+            const int rs = rprev(rk, 3);
+            const int sm = (kb - 1) & (ER - 1), sc = kb & (ER - 1), sp = (kb + 1) & (ER - 1);
+            const int ly = tid / (TX / 2), m = tid - ly * (TX / 2);
+            const int lx = 2 * m;
+            const int gy = y0 + ly;
+            const int par = (x0 + lx + gy + kb) & 1;  // 0: cell lx is red, lx+1 black
+            const int bxl = lx + (1 - par);
+            const int ex = bxl + 4, ey = ly + 4;
+            const T ec = se[sc][ey][ex];
+            const T res = sr[rs][ly + 3][bxl + 3] - gamma * ec - dhx * (se[sc][ey][ex - 1] + se[sc][ey][ex + 1]) -
+                          dhy * (se[sc][ey - 1][ex] + se[sc][ey + 1][ex]) - dhz * (se[sm][ey][ex] + se[sp][ey][ex]);
+            const T eb = ec + res * inv_gamma;
+            const T er = se[sc][ey][lx + par + 4];
+            const T e0 = par == 0 ? er : eb, e1 = par == 0 ? eb : er;
+            const size_t o = cidx(x0 + lx, gy, kb, nx, ny);
+            if (UPD) {
+                Ph[o] = Ph[o] + (double)e0;
+                Ph[o + 1] = Ph[o + 1] + (double)e1;
+            } else {
+                Eout[o] = e0;
+                Eout[o + 1] = e1;
+            }
+        }
+        rk = rk + 1 == RR ? 0 : rk + 1;
+    }
+}
+
 // Residual of the correction, restricted: r_c(I) = 0.125 * sum_children (r - L e),
 // children summed k, then j, then i innermost.  One thread per coarse cell.
 template <typename T>

@@ -263,6 +263,12 @@ struct nyxpm_ctx {
         Table t;
     };
     std::vector<SweepCfg> sw[2];
+    // FP32 levels whose pre- and post-smoothing pairs run as one pass each
+    // (k_rb_sweep2): launch table per depth (count 0: two k_rb_sweep passes)
+    std::vector<Table> t_sw2;
+    bool sweep2 = true;  // opts.sweep2 >= 0
+    int sw2_zc = 128;    // z-planes per block of k_rb_sweep2
+    bool sw2_auto = true;  // default z-chunk (not set in the options)
     std::vector<int> pre_buf;    // buffer holding e after pre-smoothing, per depth
     std::vector<int> fin_buf;    // buffer holding e after post-smoothing, per depth
     Table t_bottom, t_l0, t_res;  // t_res: level 0, at most kReduceBlocks blocks per box
@@ -537,6 +543,7 @@ int build_tables(nyxpm_ctx* c) {
     // not depend on the column shape, so both precisions share it).
     c->t_full.assign(c->dmax, {});
     c->t_half.assign(c->dmax, {});
+    c->t_sw2.assign(c->dmax, {});
     for (int pr = 0; pr < 2; ++pr) {
         c->t_rrz[pr].assign(c->dmax, {});
         c->t_rrc[pr].assign(c->dmax, {});
@@ -658,6 +665,30 @@ int build_tables(nyxpm_ctx* c) {
             c->t_rrz[pr][d] = add_table(vrz[pr]);
             c->t_rrc[pr][d] = add_table(vrc[pr]);
         }
+        // the two-sweep pass (FP32) where every box's level fits its column shape
+        // and offers at least 1024 blocks, with V(2,2) cycles
+        std::vector<CellSeg> v2;
+        bool ok2 = c->sweep2 && c->sw[1][d].tx != 0 && c->nu1 == 2 && c->nu2 == 2 && !c->fuse_res;
+        long long blocks2 = 0;
+        for (int b = 0; b < nbox && ok2; ++b) {
+            const Box& B = c->boxes[b];
+            if (d >= B.dc) continue;
+            const Level& L = B.lev[d];
+            if (L.n[0] % kSw2TX || L.n[1] % kSw2TY || L.n[2] % 2 || L.n[2] < 8) {
+                ok2 = false;
+                break;
+            }
+            CellSeg w = mk(b, d, 1);
+            w.ebuf = coarse_buf(B, d);
+            w.zchunk = std::min(c->sw2_zc, L.n[2]);
+            w.nblk = (L.n[0] / kSw2TX) * (L.n[1] / kSw2TY) * ((L.n[2] + w.zchunk - 1) / w.zchunk);
+            blocks2 += w.nblk;
+            v2.push_back(w);
+        }
+        // (the default z-chunk only where the level offers enough blocks; a
+        // z-chunk set in the options wherever the shape fits)
+        if (!ok2 || (c->sw2_auto && blocks2 < 1024)) v2.clear();
+        c->t_sw2[d] = add_table(v2);
     }
     {
         std::vector<CellSeg> vb;
@@ -1154,6 +1185,21 @@ void launch_sweep(nyxpm_ctx* c, int d, const T* ein, T* eout, const T* r, const 
 #undef NYXPM_SWEEP
 }
 
+// Two red-black sweeps in one pass (k_rb_sweep2, FP32) at depth d.
+template <int MODE>
+void launch_sweep2(nyxpm_ctx* c, int d, const float* ein, float* eout, const float* r, const float* ec0,
+                   const float* ec1) {
+    const auto& t = c->t_sw2[d];
+    hipLaunchKernelGGL((k_rb_sweep2<float, kSw2TX, kSw2TY, MODE, kSw2NT>), dim3(t.blocks), dim3(kSw2NT), 0,
+                       c->stream, c->segs(t), t.count, c->active, ein, eout, r, ec0, ec1, c->phi);
+}
+
+// Whether depth d runs its smoothing pairs as k_rb_sweep2 passes.
+template <typename T>
+bool two_sweep(const nyxpm_ctx* c, int d) {
+    return sizeof(T) == 4 && d < (int)c->t_sw2.size() && c->t_sw2[d].count > 0;
+}
+
 // One V-cycle on every active box.  Returns in *updated whether phi += e has
 // already been applied (fused into the last finest-level sweep).  With
 // sweep0_done the first level-0 sweep was already run (by residual_sweep0).
@@ -1174,7 +1220,22 @@ int vcycle(nyxpm_ctx* c, bool* updated, bool sweep0_done = false, bool first = t
         if (tf.count == 0) continue;
         int cur = 0;
         const bool keep = !first && d == 0;
-        if (c->sw[pr][d].tx) {
+        if (two_sweep<T>(c, d) && !(sweep0_done && d == 0)) {
+            // both pre-smoothing sweeps in one pass, into the buffer the two
+            // post-smoothing sweeps then read (so that they end in fin_buf)
+            cur = 1 - c->pre_buf[d];
+            float* e0 = reinterpret_cast<float*>(eb[0]);
+            float* e1 = reinterpret_cast<float*>(eb[1]);
+            float* ebf[2] = {e0, e1};
+            const float* rf = reinterpret_cast<const float*>(r);
+            if (keep) {
+                Timed t(c, lvl_name("sweep2", d));
+                launch_sweep2<0>(c, d, ebf[c->fin_buf[d]], ebf[cur], rf, e0, e1);
+            } else {
+                Timed t(c, lvl_name("sweep2_0", d));
+                launch_sweep2<kSweepZero>(c, d, ebf[cur], ebf[cur], rf, e0, e1);
+            }
+        } else if (c->sw[pr][d].tx) {
             if (keep) {
                 // the previous cycle left e in fin_buf; nu1 + nu2 is even, so the
                 // nu1 sweeps end in pre_buf as after a cycle from zero
@@ -1236,7 +1297,24 @@ int vcycle(nyxpm_ctx* c, bool* updated, bool sweep0_done = false, bool first = t
         auto& th = c->t_half[d];
         if (tf.count == 0) continue;
         int cur = c->pre_buf[d];
-        if (c->sw[pr][d].tx && c->nu2 > 0) {
+        if (two_sweep<T>(c, d) && !(sweep0_done && d == 0)) {
+            // both post-smoothing sweeps in one pass (prolongation on load), from
+            // the pre-smoothing pass's buffer into fin_buf (or into phi)
+            cur = 1 - c->pre_buf[d];
+            float* e0 = reinterpret_cast<float*>(eb[0]);
+            float* e1 = reinterpret_cast<float*>(eb[1]);
+            float* ebf[2] = {e0, e1};
+            const float* rf = reinterpret_cast<const float*>(r);
+            const bool upd = last && d == 0 && c->fuse_update;
+            if (upd) {
+                Timed t(c, lvl_name("sweep2_u", d));
+                launch_sweep2<kSweepProlong | kSweepUpdate>(c, d, ebf[cur], ebf[1 - cur], rf, e0, e1);
+                *updated = true;
+            } else {
+                Timed t(c, lvl_name("sweep2_p", d));
+                launch_sweep2<kSweepProlong>(c, d, ebf[cur], ebf[1 - cur], rf, e0, e1);
+            }
+        } else if (c->sw[pr][d].tx && c->nu2 > 0) {
             // prolongation fused into the first sweep; phi += e into the last one
             // on the finest level
             for (int q = 0; q < c->nu2; ++q) {
@@ -1697,6 +1775,11 @@ int nyxpm_create_batch(nyxpm_ctx** out, int nbox, const nyxpm_geom* geoms, const
         c->fuse_res = opts->residual == 1;
         if (opts->graph_cycles > 0) c->graph_cycles = opts->graph_cycles;
         if (opts->ir_first > 0) c->ir_first = opts->ir_first;
+        if (opts->sweep2 < 0) c->sweep2 = false;
+        if (opts->sweep2 > 1) {
+            c->sw2_zc = std::max(8, opts->sweep2);
+            c->sw2_auto = false;
+        }
         if (opts->resort_slack < 0) {
             c->slack_num = 0;
             c->slack_pad = 0;

@@ -90,9 +90,14 @@ __device__ __forceinline__ int tile_div(int i, unsigned magic) {
 
 __device__ __forceinline__ int tile_of(const BoxDev& B, double x, double y, double z, int* err) {
     double f;
-    const int i = wrap_checked(cic_index(x, B.plo[0], B.dxi[0], &f), B.n[0], err);
-    const int j = wrap_checked(cic_index(y, B.plo[1], B.dxi[1], &f), B.n[1], err);
-    const int k = wrap_checked(cic_index(z, B.plo[2], B.dxi[2], &f), B.n[2], err);
+    const int ci = cic_index(x, B.plo[0], B.dxi[0], &f);
+    const int cj = cic_index(y, B.plo[1], B.dxi[1], &f);
+    const int ck = cic_index(z, B.plo[2], B.dxi[2], &f);
+    // an index more than one period out: one flag branch for the three (rare)
+    if ((unsigned)(ci + B.n[0]) >= (unsigned)(3 * B.n[0]) || (unsigned)(cj + B.n[1]) >= (unsigned)(3 * B.n[1]) ||
+        (unsigned)(ck + B.n[2]) >= (unsigned)(3 * B.n[2]))
+        atomicOr(err, 1);
+    const int i = wrap1(ci, B.n[0]), j = wrap1(cj, B.n[1]), k = wrap1(ck, B.n[2]);
     const int tx = tile_div(i, B.tdiv[0]), ty = tile_div(j, B.tdiv[1]), tz = tile_div(k, B.tdiv[2]);
     return (int)(B.tile_off + ((long long)tz * B.ntile[1] + ty) * B.ntile[0] + tx);
 }

@@ -148,6 +148,50 @@ def test_sweep_shapes_equal_half_sweeps(nyx, shape, zc, precision):
     assert np.array_equal(out[0], out[1])
 
 
+@pytest.mark.parametrize("zc", [8, 13])
+def test_two_sweep_pass_equals_two_passes(nyx, zc):
+    # FP32 smoothing pairs as one pass (k_rb_sweep2, zc planes per block, uneven
+    # last chunks) against two k_rb_sweep passes, bit for bit: fixed-iteration
+    # solves from phi = 0 on a cube (two levels in the pass) and a non-cubic box
+    r = np.random.default_rng(21)
+    for n in ((64, 64, 64), (96, 64, 32)):
+        f = r.standard_normal((n[2], n[1], n[0]))
+        f -= f.mean()
+        out = []
+        for sw2 in (zc, -1):
+            with ctx1(nyx, n, precision="mixed", sweep2=sw2) as c:
+                phi, it, _ = c.solve_poisson(f, fixed_iters=3)
+                out.append(phi)
+        assert np.array_equal(out[0], out[1])
+    # mixed steps of a batch (grouped FP32 V-cycles: the pre-smoothing pass
+    # continues from the previous cycle's correction; the last post-smoothing
+    # pass updates phi)
+    sizes = [(64, 64, 64), (96, 64, 32)]
+    res = []
+    for sw2 in (zc, -1):
+        with nyx.PMContext([nyx.BoxGeom(s) for s in sizes], precision="mixed", sweep2=sw2) as c:
+            for b, s in enumerate(sizes):
+                pos, vel, m = nyx.ics.plummer_clustered(s[0], 70 + b, n_spheres=3) if s[0] == s[1] == s[2] \
+                    else _noncubic_inputs(nyx, s, 70 + b)
+                c.upload(b, pos, vel, m_const=m)
+            c.pm_init(1.0, 1.0, 1e-10)
+            its = c.pm_steps(3, 1e-3, rel_tol=1e-10)
+            res.append(([c.download(b) for b in range(len(sizes))], [int(v) for v in its]))
+    assert res[0][1] == res[1][1]
+    for (pa, va), (pb, vb) in zip(res[0][0], res[1][0]):
+        for d in range(3):
+            assert np.array_equal(pa[d], pb[d]) and np.array_equal(va[d], vb[d])
+
+
+def _noncubic_inputs(nyx, n, seed):
+    # particles = cells, uniform in the unit box, small random velocities
+    r = np.random.default_rng(seed)
+    N = n[0] * n[1] * n[2]
+    pos = [r.random(N) for _ in range(3)]
+    vel = [0.01 * r.standard_normal(N) for _ in range(3)]
+    return pos, vel, 1.0 / N
+
+
 def test_mg_converged_meets_rule(nyx, orc):
     n = (64, 64, 64)
     dx = [1 / 64] * 3
