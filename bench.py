@@ -350,7 +350,8 @@ def main():
     if a.trace_summary:
         return trace_summary(a.trace_summary, a.from_json)
     if a.pmc_summary:
-        stages = pmc_summary(*a.pmc_summary, json_path=a.from_json)
+        jp = (a.from_json or "").split(",") if a.from_json else [None, None]
+        stages = pmc_summary(*a.pmc_summary, fetch_json=jp[0], write_json=jp[-1])
         if a.json_out:
             with open(a.json_out, "w") as fh:
                 json.dump({"source": "bench.py --pmc-summary of rocprofv3 FETCH_SIZE / WRITE_SIZE runs "
@@ -565,7 +566,7 @@ def _stage_of_kernel(name):
     import re
     s = name
     if "k_rb_sweep2" in s:
-        mt = re.search(r"k_rb_sweep2<(\w+), *(\d+), *(\d+), *(\d+)>", s)
+        mt = re.search(r"k_rb_sweep2<(\w+), *(\d+), *(\d+), *(\d+)", s)
         mode = int(mt.group(4)) if mt else -1
         return {1: "sweep2_0", 0: "sweep2", 2: "sweep2_p", 6: "sweep2_u"}.get(mode, "sweep2?"), \
             (mt.group(1) if mt else "?")
@@ -710,29 +711,40 @@ def trace_summary(csv_path, json_path):
     print("\n".join(out_rows))
 
 
-def pmc_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, json_path=None, n=512):
+def pmc_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, fetch_json=None, write_json=None, n=512):
     """Per kernel on the finest level the mean FETCH_SIZE and WRITE_SIZE (KiB per
-    dispatch) of a rocprofv3 counter collection, each scaled by what the counter
-    reads for a copy of 1 GiB (__amd_rocclr_copyBuffer), and the ratio to the
-    algorithmic bytes.  The level of a dispatch comes from its grid size and the
-    autotuned launch shapes of a bench line of the same build and workload
-    (json_path; without it, the largest grid of each kernel), so that the
+    dispatch) of two rocprofv3 counter collections of bench runs, each scaled by
+    what the counter reads for a copy of 1 GiB (__amd_rocclr_copyBuffer), and the
+    ratio to the algorithmic bytes.  The level of a dispatch comes from its grid
+    size and the autotuned launch shapes of its run's bench line (fetch_json,
+    write_json; without them, the largest grid of each kernel), so that the
     autotuner's trial launches are left out."""
     import csv
-    tuned = None
-    if json_path:
-        with open(json_path) as fh:
-            run = json.loads(fh.read().strip().splitlines()[-1])
-        n = round(run["config"]["cells"] ** (1.0 / 3.0))
-        tuned = run["config"].get("autotuned")
 
-    def load(path):
+    def tuned_of(path):
+        if not path:
+            return None, n
+        with open(path) as fh:
+            run = json.loads(fh.read().strip().splitlines()[-1])
+        return run["config"].get("autotuned"), round(run["config"]["cells"] ** (1.0 / 3.0))
+
+    def load(path, jpath):
+        tuned, nn = tuned_of(jpath)
         rows = {}
         with open(path) as fh:
             for r in csv.DictReader(fh):
                 nm = r["Kernel_Name"].split("(")[0].replace("void ", "").replace("nyxpm::", "")
                 rows.setdefault((nm, int(r["Grid_Size"])), []).append(float(r["Counter_Value"]))
-        return rows
+        out = {}
+        for (nm, grid), v in rows.items():
+            if jpath:
+                if kernel_level(nm, grid, nn, tuned) == 0:
+                    out.setdefault(nm, []).extend(v)
+            elif nm not in out or grid > out[nm][0]:
+                out[nm] = (grid, v)
+        if not jpath:
+            out = {k: v[1] for k, v in out.items()}
+        return out, nn
 
     def copy_kib(path):
         with open(path) as fh:
@@ -741,24 +753,19 @@ def pmc_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, json_path=
                     return float(r["Counter_Value"])
         raise SystemExit(f"{path}: no __amd_rocclr_copyBuffer dispatch")
 
-    F, W = load(fetch_csv), load(write_csv)
+    (F, nn), (W, _) = load(fetch_csv, fetch_json), load(write_csv, write_json)
     gib_kib = 2.0 ** 30 / 1024.0
     sf, sw = gib_kib / copy_kib(copy_fetch_csv), gib_kib / copy_kib(copy_write_csv)
-    N = n ** 3
-    halo = tile_halo_cells((n, n, n), TILE)
-    best = {}
-    for (nm, grid), v in F.items():
-        if tuned is not None:
-            if kernel_level(nm, grid, n, tuned) == 0:
-                best[nm] = (grid, v)
-        elif nm not in best or grid > best[nm][0]:
-            best[nm] = (grid, v)
+    N = nn ** 3
+    halo = tile_halo_cells((nn, nn, nn), TILE)
     stages = {}
     print(f"calibration: a 1 GiB copy reads FETCH_SIZE x{sf:.3f}, WRITE_SIZE x{sw:.3f} of its bytes\n")
-    print("| kernel | grid | dispatches | FETCH KiB | WRITE KiB | traffic GB (scaled) | algorithmic GB | ratio |")
-    print("|---|---|---|---|---|---|---|---|")
-    for nm, (grid, fv) in sorted(best.items()):
-        wv = W.get((nm, grid), [0.0])
+    print("| kernel (finest level) | dispatches | FETCH KiB | WRITE KiB | traffic GB (scaled) | algorithmic GB | ratio |")
+    print("|---|---|---|---|---|---|---|")
+    for nm in sorted(F):
+        fv, wv = F[nm], W.get(nm, [])
+        if not wv:
+            continue
         fk, wk = sum(fv) / len(fv), sum(wv) / len(wv)
         gb = (fk * sf + wk * sw) * 1024.0 / 1e9
         stage, _ = _stage_of_kernel(nm)
@@ -766,14 +773,12 @@ def pmc_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, json_path=
         key = stage + ".d0" if stage and (stage.startswith("sweep") or stage == "resid_restrict") else stage
         alg = kernel_bytes(key, N, N, halo, sz) if key else None
         tail = f"{alg / 1e9:.3f} | {gb / (alg / 1e9):.2f} |" if alg else " | |"
-        print(f"| `{nm}` | {grid} | {len(fv)} | {fk:.4g} | {wk:.4g} | {gb:.3f} | " + tail)
+        print(f"| `{nm}` | {len(fv)} | {fk:.4g} | {wk:.4g} | {gb:.3f} | " + tail)
         # the mixed-precision step's kernels (FP32 multigrid, FP32-weight deposit
         # and interpolation) for roofline.traffic
         mixed_kernel = {"fold": nm.startswith("k_fold<false>"),
                         "deposit": nm.startswith("k_deposit<true>")}.get(key, "double" not in nm)
-        # several instances can serve one stage (levels, shapes): the finest
-        # level's moves the most bytes
-        if key and alg and mixed_kernel and gb * 1e9 > stages.get(key, {}).get("traffic_bytes", -1):
+        if key and alg and mixed_kernel:
             stages[key] = {"kernel": nm, "fetch_bytes": int(fk * sf * 1024), "write_bytes": int(wk * sw * 1024),
                            "traffic_bytes": int(gb * 1e9), "algorithmic_bytes": int(alg)}
     return stages
