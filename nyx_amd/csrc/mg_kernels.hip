@@ -478,45 +478,61 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
 // Two full red-black Gauss-Seidel sweeps in one pass (temporal blocking): the
 // pre-smoothing pair or the post-smoothing pair of a V(2,2) cycle on a fine
 // level reads e and r once and writes e once, where two k_rb_sweep passes move
-// each of them twice.  A block owns a TX x TY column of cells and a chunk of
-// z-planes and marches up in z.  The input e planes are kept with a halo of 4
-// in x and y in a ring of 8 LDS planes, r with a halo of 3 in a ring of 5, and
-// each new plane k runs four in-place steps, one barrier apart: red of sweep 1
-// on plane k (halo-3 region), black of sweep 1 on plane k-1 (halo 2), red of
-// sweep 2 on plane k-2 (halo 1), black of sweep 2 on plane k-3 (the block's own
-// cells), after which plane k-3 is final and written out.  Each step reads only
-// values in the state the two separate sweeps would read (cells of one colour
-// change state only in their own step, and the planes a step reads are those
-// its predecessors have finished), and the halo cells are computed redundantly
-// from the same values by every block that holds them, so the results equal
-// two k_rb_sweep passes (and four half sweeps) bit for bit.  The chunk [z0, z1)
-// needs planes z0-4 .. z1+3.  (Pairing red of sweep 1 with red of sweep 2 three
-// planes behind, and the blacks likewise, needs two barriers per plane instead
-// of four but measured 13% slower: two updates per thread in a row.)
+// each of them twice.
+//
+// A block owns a TX x TY column of cells and a chunk of z-planes [z0, z1) and
+// marches up in z over the planes z0-4 .. z1+3.  It has one thread per pair of
+// cells (x even, x+1) of the column widened by a halo of 4 in x and y.  The
+// thread keeps its pair's e over the six planes k+1 .. k-4 and r over the four
+// planes k .. k-3 in registers, each plane split into its red and its black
+// cell (in a pair, the red cell is the first where y + z is even and the
+// second otherwise).  Iteration k runs four updates of the pair, each with
+// k_gsrb_half's arithmetic:
+//   A  sweep 1, red, plane k        B  sweep 1, black, plane k-1
+//   C  sweep 2, red, plane k-2      D  sweep 2, black, plane k-3
+// after which plane k-3 is final and the block's own cells are written out.
+// An update's z-neighbours are the same pair's cells in the planes above and
+// below (registers); its in-plane neighbours have the other colour: one is the
+// pair's other cell, three lie in the pairs left or right, above and below,
+// and are read from LDS half-planes (one array per colour and plane, indexed
+// by pair, so that a wavefront reads consecutive words).  Iteration k reads the
+// half-planes black k (input values), red k-1, black k-2 and red k-3, each
+// written in iteration k-1, and writes red k, black k-1, red k-2 and black k+1
+// (input values), so one barrier per plane orders every write before its
+// reads.  The region where the values are those of the two separate sweeps
+// shrinks by one cell per step (halo 3 after A, 2 after B, 1 after C, 0 after
+// D); every thread runs every step, and the values it computes outside that
+// region never reach a cell inside it, so the block's own cells equal two
+// k_rb_sweep passes (and four k_gsrb_half) bit for bit.
 // MODE bits as k_rb_sweep's: kSweepZero (sweep 1 from e = 0), kSweepProlong
 // (sweep 1's input is ein plus the coarse correction), kSweepUpdate (sweep 2's
 // result is added to phi instead of stored).  Needs nx % TX == 0, ny % TY == 0,
-// even nx, ny, nz and nz >= 8.  HBM bytes per cell: 3*sizeof(T) (e read, r
-// read, e write) for both sweeps.
-constexpr int kSw2TX = 32, kSw2TY = 16, kSw2NT = 512;
-template <typename T, int TX, int TY, int MODE, int NT>
-__global__ void __launch_bounds__(NT) k_rb_sweep2(const CellSeg* segs, int nseg, const int* active,
-                                                  const T* __restrict__ ein, T* __restrict__ eout,
-                                                  const T* __restrict__ r, const T* ec0, const T* ec1,
-                                                  double* phi) {
+// even nx, ny, nz, nz >= 8 and nx * ny < 2^31.  HBM bytes per cell:
+// 3*sizeof(T) (e read, r read, e write) for both sweeps (+16 with
+// kSweepUpdate: phi read and written).
+template <typename T>
+struct alignas(2 * sizeof(T)) CellPair {
+    T a, b;
+};
+constexpr int kSw2TX = 32, kSw2TY = 16;
+template <int TX, int TY>
+constexpr int sw2_threads() {
+    return (TX + 8) / 2 * (TY + 8);
+}
+template <typename T, int TX, int TY, int MODE>
+__global__ void __launch_bounds__((sw2_threads<TX, TY>())) k_rb_sweep2(const CellSeg* segs, int nseg,
+                                                                     const int* active, const T* __restrict__ ein,
+                                                                     T* __restrict__ eout, const T* __restrict__ r,
+                                                                     const T* ec0, const T* ec1, double* phi) {
     constexpr bool ZERO_IN = (MODE & kSweepZero) != 0;
     constexpr bool PRO = (MODE & kSweepProlong) != 0;
     constexpr bool UPD = (MODE & kSweepUpdate) != 0;
-    static_assert(NT >= TX * TY / 2, "one thread per cell pair of the block's own cells");
-    constexpr int EX = TX + 8, EY = TY + 8;   // e planes: halo 4
-    constexpr int RX = TX + 6, RY = TY + 6;   // r planes: halo 3
-    constexpr int NE = (EX * EY + NT - 1) / NT, NR = (RX * RY + NT - 1) / NT;
-    // ring lengths (planes): the e plane stored at the top of iteration k (plane
-    // k+1) replaces plane k-7, last read two iterations earlier; the r plane
-    // (plane k) replaces plane k-5, likewise
-    constexpr int ER = 8, RR = 5;
-    __shared__ T se[ER][EY][EX];
-    __shared__ T sr[RR][RY][RX];
+    constexpr int PX = (TX + 8) / 2, PY = TY + 8;  // pairs per row, rows
+    // LDS half-plane: PY rows of PX pairs, with a row (plus one word) of
+    // padding on either side for the out-of-region reads of the edge threads
+    constexpr int LP = (PY + 2) * PX + 2;
+    __shared__ T sred[4][LP];
+    __shared__ T sblk[4][LP];
     const int bid = xcd_block(blockIdx.x, gridDim.x);
     const int s = find_seg(segs, nseg, bid);
     const CellSeg& sg = segs[s];
@@ -534,139 +550,193 @@ __global__ void __launch_bounds__(NT) k_rb_sweep2(const CellSeg* segs, int nseg,
     T* Eout = eout + sg.off;
     const T* R = r + sg.off;
     const T* Ec = (sg.ebuf ? ec1 : ec0) + sg.coff;
-    const int cnx = nx >> 1, cny = ny >> 1;
     double* Ph = phi + sg.foff;
-    const int tid = threadIdx.x;
+    const int cnx = nx >> 1, cny = ny >> 1;
     const size_t plane = (size_t)nx * ny, cplane = (size_t)cnx * cny;
     auto wrapz = [&](int k) { return k < 0 ? k + nz : (k >= nz ? k - nz : k); };
 
-    // this thread's elements of the e and r regions (indices past the region
-    // clamped onto its last element: the extra lanes load and store the owner's
-    // value), their in-plane offsets with the periodic wrap
-    size_t eoff[NE], roff[NR], coff_[PRO ? NE : 1];
-#pragma unroll
-    for (int u = 0; u < NE; ++u) {
-        const int idx = min(tid + u * NT, EX * EY - 1);
-        const int ly = idx / EX, lx = idx - ly * EX;
-        int gx = x0 + lx - 4, gy = y0 + ly - 4;
-        gx = gx < 0 ? gx + nx : (gx >= nx ? gx - nx : gx);
-        gy = gy < 0 ? gy + ny : (gy >= ny ? gy - ny : gy);
-        eoff[u] = (size_t)gy * nx + gx;
-        if (PRO) coff_[u] = (size_t)(gy >> 1) * cnx + (gx >> 1);
-    }
-#pragma unroll
-    for (int u = 0; u < NR; ++u) {
-        const int idx = min(tid + u * NT, RX * RY - 1);
-        const int ly = idx / RX, lx = idx - ly * RX;
-        int gx = x0 + lx - 3, gy = y0 + ly - 3;
-        gx = gx < 0 ? gx + nx : (gx >= nx ? gx - nx : gx);
-        gy = gy < 0 ? gy + ny : (gy >= ny ? gy - ny : gy);
-        roff[u] = (size_t)gy * nx + gx;
-    }
-    T pe[NE], pr[NR];
-    auto fetch_e = [&](int kz) {
-        if (ZERO_IN) return;
-        const int kw = wrapz(kz);
-#pragma unroll
-        for (int u = 0; u < NE; ++u) {
-            pe[u] = Ein[(size_t)kw * plane + eoff[u]];
-            if (PRO) pe[u] = pe[u] + Ec[(size_t)(kw >> 1) * cplane + coff_[u]];
-        }
-    };
-    auto put_e = [&](int kz) {
-        if (ZERO_IN) return;  // sweep 1 reads literal zeros for its input
-        T* dst = &se[kz & (ER - 1)][0][0];
-#pragma unroll
-        for (int u = 0; u < NE; ++u) dst[min(tid + u * NT, EX * EY - 1)] = pe[u];
-    };
-    auto fetch_r = [&](int kz) {
-        const int kw = wrapz(kz);
-#pragma unroll
-        for (int u = 0; u < NR; ++u) pr[u] = R[(size_t)kw * plane + roff[u]];
-    };
-    auto put_r = [&](int slot) {
-        T* dst = &sr[slot][0][0];
-#pragma unroll
-        for (int u = 0; u < NR; ++u) dst[min(tid + u * NT, RX * RY - 1)] = pr[u];
-    };
-    // one colour (c: 0 red, 1 black) of plane p on the halo-h region: the
-    // k_gsrb_half update in place.  zred: sweep 1's red step from zero input
-    // (every e operand an input zero); zblack: sweep 1's black step from zero
-    // input (the cell's own input zero)
-    auto colour = [&](int p, int c, int h, int rslot, bool zred, bool zblack) {
-        const int sm = (p - 1) & (ER - 1), sc = p & (ER - 1), sp = (p + 1) & (ER - 1);
-        const int WX = TX + 2 * h, WY = TY + 2 * h, HX = WX / 2;
-        for (int idx = tid; idx < HX * WY; idx += NT) {
-            const int ly = idx / HX, mm = idx - ly * HX;
-            const int lx = 2 * mm + ((c + x0 + y0 + ly + p) & 1);  // (gx + gy + p) % 2 == c
-            const int ex = lx + 4 - h, ey = ly + 4 - h;
-            const int rx = lx + 3 - h, ry = ly + 3 - h;
-            if (zred) {
-                const T ec = T(0);
-                const T res = sr[rslot][ry][rx] - gamma * ec - dhx * (T(0) + T(0)) - dhy * (T(0) + T(0)) -
-                              dhz * (T(0) + T(0));
-                se[sc][ey][ex] = ec + res * inv_gamma;
-            } else {
-                const T ec = zblack ? T(0) : se[sc][ey][ex];
-                const T res = sr[rslot][ry][rx] - gamma * ec - dhx * (se[sc][ey][ex - 1] + se[sc][ey][ex + 1]) -
-                              dhy * (se[sc][ey - 1][ex] + se[sc][ey + 1][ex]) -
-                              dhz * (se[sm][ey][ex] + se[sp][ey][ex]);
-                se[sc][ey][ex] = ec + res * inv_gamma;
-            }
-        }
-    };
-    // r ring slot of plane k, kept incrementally (plane z0 - 3 -> slot 0)
-    int rk = 0;
-    auto rprev = [&](int sl, int d) { sl -= d; return sl < 0 ? sl + RR : sl; };
+    const int tid = threadIdx.x;
+    const int ly = tid / PX, m = tid - ly * PX;
+    int gx = x0 - 4 + 2 * m, gy = y0 - 4 + ly;
+    gx = gx < 0 ? gx + nx : (gx >= nx ? gx - nx : gx);
+    gy = gy < 0 ? gy + ny : (gy >= ny ? gy - ny : gy);
+    const unsigned po = (unsigned)gy * (unsigned)nx + (unsigned)gx;      // pair's in-plane offset
+    const unsigned cpo = (unsigned)(gy >> 1) * (unsigned)cnx + (unsigned)(gx >> 1);  // its parent's
+    const bool own = ly >= 4 && ly < 4 + TY && m >= 2 && m < 2 + TX / 2;
+    // the own pair nearest to this one (phi is read there by every thread, so
+    // that no load is conditional: a conditional load costs the compiler its
+    // count of loads in flight, and then every wait is a wait for all of them)
+    const unsigned pown = (unsigned)(y0 + min(max(ly - 4, 0), TY - 1)) * (unsigned)nx +
+                          (unsigned)(x0 + 2 * min(max(m - 2, 0), TX / 2 - 1));
+    const int la = (ly + 1) * PX + m + 1;  // LDS index of the pair
+    const int k0 = z0 - 3, kend = z1 + 2;  // iterations k0 .. kend
+    // In iteration k every update is of the pair's cell at index (gy + k) & 1
+    // (red of k and k-2, black of k-1 and k-3), whose outside x-neighbour lies
+    // in pair m-1 for index 0 and m+1 for index 1.  c0: that index for
+    // iterations of k0's parity.
+    const int c0 = (gy + k0) & 1;
+    const int lxe = la + (c0 ? 1 : -1), lxo = la + (c0 ? -1 : 1);
 
-    fetch_e(z0 - 4);
-    put_e(z0 - 4);
-    fetch_e(z0 - 3);
-    put_e(z0 - 3);
-    fetch_e(z0 - 2);
-    fetch_r(z0 - 3);
-    for (int k = z0 - 3; k <= z1 + 2; ++k) {
-        put_e(k + 1);
-        put_r(rk);
-        fetch_e(k + 2);
-        fetch_r(k + 1);
-        __syncthreads();
-        // sweep 1, red, plane k (needed for every k here)
-        colour(k, 0, 3, rk, ZERO_IN, false);
-        __syncthreads();
-        // sweep 1, black, plane k-1 (needed from z0 - 2)
-        if (k - 1 >= z0 - 2) colour(k - 1, 1, 2, rprev(rk, 1), false, ZERO_IN);
-        __syncthreads();
-        // sweep 2, red, plane k-2 (needed from z0 - 1)
-        if (k - 2 >= z0 - 1) colour(k - 2, 0, 1, rprev(rk, 2), false, false);
-        __syncthreads();
-        // sweep 2, black, plane k-3 on the block's own cells, and the plane out
-        const int kb = k - 3;
-        if (kb >= z0 && tid < TX * TY / 2) {
-            const int rs = rprev(rk, 3);
-            const int sm = (kb - 1) & (ER - 1), sc = kb & (ER - 1), sp = (kb + 1) & (ER - 1);
-            const int ly = tid / (TX / 2), m = tid - ly * (TX / 2);
-            const int lx = 2 * m;
-            const int gy = y0 + ly;
-            const int par = (x0 + lx + gy + kb) & 1;  // 0: cell lx is red, lx+1 black
-            const int bxl = lx + (1 - par);
-            const int ex = bxl + 4, ey = ly + 4;
-            const T ec = se[sc][ey][ex];
-            const T res = sr[rs][ly + 3][bxl + 3] - gamma * ec - dhx * (se[sc][ey][ex - 1] + se[sc][ey][ex + 1]) -
-                          dhy * (se[sc][ey - 1][ex] + se[sc][ey + 1][ex]) - dhz * (se[sm][ey][ex] + se[sp][ey][ex]);
-            const T eb = ec + res * inv_gamma;
-            const T er = se[sc][ey][lx + par + 4];
-            const T e0 = par == 0 ? er : eb, e1 = par == 0 ? eb : er;
-            const size_t o = cidx(x0 + lx, gy, kb, nx, ny);
-            if (UPD) {
-                Ph[o] = Ph[o] + (double)e0;
-                Ph[o + 1] = Ph[o + 1] + (double)e1;
+    // e and r of the pair: index i <-> plane k+1-i (e), k-i (r) at the top of
+    // iteration k
+    T Er[6], Eb[6], Fr[4], Fb[4];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) Er[i] = Eb[i] = T(0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) Fr[i] = Fb[i] = T(0);
+    // loads in flight: e of plane k+3 and r of plane k+2 issued in iteration k
+    // (slot k & 1), moved into the window at the end of iteration k+1; phi of
+    // plane k-1 for the own cells (kSweepUpdate), consumed in iteration k+2
+    CellPair<T> qe[2], qr[2];
+    T qc[2];
+    CellPair<double> qp[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        qe[i] = qr[i] = CellPair<T>{T(0), T(0)};
+        qc[i] = T(0);
+        qp[i] = CellPair<double>{0.0, 0.0};
+    }
+    // (planes past the last one needed re-read that one: every load is issued
+    // in every iteration)
+    auto load_e = [&](int p, CellPair<T>& v, T& cv) {
+        if (ZERO_IN) return;
+        const int pw = wrapz(min(p, z1 + 3));
+        v = *reinterpret_cast<const CellPair<T>*>(Ein + (size_t)pw * plane + po);
+        if (PRO) cv = Ec[(size_t)(pw >> 1) * cplane + cpo];
+    };
+    auto load_r = [&](int p, CellPair<T>& v) {
+        const int pw = wrapz(min(p, z1 + 2));
+        v = *reinterpret_cast<const CellPair<T>*>(R + (size_t)pw * plane + po);
+    };
+    // red and black cell of a loaded e pair whose red index is cr
+    auto split_e = [&](int cr, const CellPair<T>& v, T cv, T& red, T& blk) {
+        if (ZERO_IN) {
+            red = blk = T(0);
+            return;
+        }
+        T a = v.a, bb = v.b;
+        if (PRO) {
+            a = a + cv;
+            bb = bb + cv;
+        }
+        red = cr ? bb : a;
+        blk = cr ? a : bb;
+    };
+
+    // prologue: e of planes k0-1 .. k0+1 into the window, plane k0+2 and r of
+    // plane k0+1 in flight (slot of k0-1), r of plane k0 into the window, and
+    // the black input of plane k0 into LDS
+    {
+        CellPair<T> v;
+        T cv = T(0);
+        v = CellPair<T>{T(0), T(0)};
+        load_e(k0 - 1, v, cv);
+        split_e(c0 ^ 1, v, cv, Er[2], Eb[2]);
+        load_e(k0, v, cv);
+        split_e(c0, v, cv, Er[1], Eb[1]);
+        load_e(k0 + 1, v, cv);
+        split_e(c0 ^ 1, v, cv, Er[0], Eb[0]);
+        v = CellPair<T>{T(0), T(0)};
+        load_r(k0, v);
+        Fr[0] = c0 ? v.b : v.a;
+        Fb[0] = c0 ? v.a : v.b;
+        load_e(k0 + 2, qe[1], qc[1]);
+        load_r(k0 + 1, qr[1]);
+        if (!ZERO_IN) sblk[0][la] = Eb[1];
+    }
+    __syncthreads();
+
+    auto body = [&](auto JJ, int k) {
+        constexpr int J = decltype(JJ)::value;
+        // LDS slot of plane p is (p - k0) & 3: planes k, k-1, k-2, k-3 (= k+1)
+        constexpr int s0 = J & 3, s1 = (J + 3) & 3, s2 = (J + 2) & 3, s3 = (J + 1) & 3;
+        constexpr int qn = J & 1, qo = (J + 1) & 1;  // queue slot loaded now / consumed now
+        const int c = c0 ^ (J & 1);
+        const int lx = (J & 1) ? lxo : lxe;
+        load_e(k + 3, qe[qn], qc[qn]);
+        load_r(k + 2, qr[qn]);
+        // A: sweep 1, red, plane k
+        {
+            T ec = Er[1], res;
+            if (ZERO_IN) {
+                ec = T(0);
+                res = Fr[0] - gamma * ec - dhx * (T(0) + T(0)) - dhy * (T(0) + T(0)) - dhz * (T(0) + T(0));
             } else {
-                Eout[o] = e0;
-                Eout[o + 1] = e1;
+                const T* L = sblk[s0];
+                res = Fr[0] - gamma * ec - dhx * (Eb[1] + L[lx]) - dhy * (L[la - PX] + L[la + PX]) -
+                      dhz * (Eb[2] + Eb[0]);
+            }
+            Er[1] = ec + res * inv_gamma;
+            sred[s0][la] = Er[1];
+        }
+        // B: sweep 1, black, plane k-1
+        {
+            const T* L = sred[s1];
+            const T ec = Eb[2];
+            const T res = Fb[1] - gamma * ec - dhx * (Er[2] + L[lx]) - dhy * (L[la - PX] + L[la + PX]) -
+                          dhz * (Er[3] + Er[1]);
+            Eb[2] = ec + res * inv_gamma;
+            sblk[s1][la] = Eb[2];
+        }
+        // C: sweep 2, red, plane k-2
+        {
+            const T* L = sblk[s2];
+            const T ec = Er[3];
+            const T res = Fr[2] - gamma * ec - dhx * (Eb[3] + L[lx]) - dhy * (L[la - PX] + L[la + PX]) -
+                          dhz * (Eb[4] + Eb[2]);
+            Er[3] = ec + res * inv_gamma;
+            sred[s2][la] = Er[3];
+        }
+        // D: sweep 2, black, plane k-3, and the plane out
+        {
+            const T* L = sred[s3];
+            const T ec = Eb[4];
+            const T res = Fb[3] - gamma * ec - dhx * (Er[4] + L[lx]) - dhy * (L[la - PX] + L[la + PX]) -
+                          dhz * (Er[5] + Er[3]);
+            Eb[4] = ec + res * inv_gamma;
+        }
+        const int kb = k - 3;
+        if (own && kb >= z0 && kb < z1) {
+            // plane k-3: its red cell has index c ^ 1
+            const T e0 = c ? Er[4] : Eb[4], e1 = c ? Eb[4] : Er[4];
+            if (UPD) {
+                CellPair<double>* q = reinterpret_cast<CellPair<double>*>(Ph + (size_t)kb * plane + po);
+                *q = CellPair<double>{qp[qn].a + (double)e0, qp[qn].b + (double)e1};
+            } else {
+                *reinterpret_cast<CellPair<T>*>(Eout + (size_t)kb * plane + po) = CellPair<T>{e0, e1};
             }
         }
-        rk = rk + 1 == RR ? 0 : rk + 1;
+        if (UPD) {
+            const int pp = min(max(k - 1, z0), z1 - 1);
+            qp[qn] = *reinterpret_cast<const CellPair<double>*>(Ph + (size_t)pp * plane + pown);
+        }
+        // the black input of plane k+1 for the next iteration's step A
+        if (!ZERO_IN) sblk[s3][la] = Eb[0];
+        __syncthreads();
+        // the window moves down one plane: e of plane k+2 and r of plane k+1
+        // come in from the loads issued in iteration k-1
+#pragma unroll
+        for (int i = 5; i > 0; --i) {
+            Er[i] = Er[i - 1];
+            Eb[i] = Eb[i - 1];
+        }
+        split_e(c, qe[qo], qc[qo], Er[0], Eb[0]);
+#pragma unroll
+        for (int i = 3; i > 0; --i) {
+            Fr[i] = Fr[i - 1];
+            Fb[i] = Fb[i - 1];
+        }
+        Fr[0] = c ? qr[qo].a : qr[qo].b;
+        Fb[0] = c ? qr[qo].b : qr[qo].a;
+    };
+    for (int k = k0; k <= kend; k += 4) {
+        body(std::integral_constant<int, 0>{}, k);
+        if (k + 1 > kend) break;
+        body(std::integral_constant<int, 1>{}, k + 1);
+        if (k + 2 > kend) break;
+        body(std::integral_constant<int, 2>{}, k + 2);
+        if (k + 3 > kend) break;
+        body(std::integral_constant<int, 3>{}, k + 3);
     }
 }
 

@@ -674,7 +674,8 @@ int build_tables(nyxpm_ctx* c) {
             const Box& B = c->boxes[b];
             if (d >= B.dc) continue;
             const Level& L = B.lev[d];
-            if (L.n[0] % kSw2TX || L.n[1] % kSw2TY || L.n[2] % 2 || L.n[2] < 8) {
+            if (L.n[0] % kSw2TX || L.n[1] % kSw2TY || L.n[2] % 2 || L.n[2] < 8 ||
+                (long long)L.n[0] * L.n[1] >= (1LL << 31)) {
                 ok2 = false;
                 break;
             }
@@ -1190,7 +1191,8 @@ template <int MODE>
 void launch_sweep2(nyxpm_ctx* c, int d, const float* ein, float* eout, const float* r, const float* ec0,
                    const float* ec1) {
     const auto& t = c->t_sw2[d];
-    hipLaunchKernelGGL((k_rb_sweep2<float, kSw2TX, kSw2TY, MODE, kSw2NT>), dim3(t.blocks), dim3(kSw2NT), 0,
+    hipLaunchKernelGGL((k_rb_sweep2<float, kSw2TX, kSw2TY, MODE>), dim3(t.blocks),
+                       dim3(sw2_threads<kSw2TX, kSw2TY>()), 0,
                        c->stream, c->segs(t), t.count, c->active, ein, eout, r, ec0, ec1, c->phi);
 }
 
