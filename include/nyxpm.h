@@ -135,6 +135,9 @@ typedef struct nyxpm_opts {
                           written once; same results bit for bit).  0 = default
                           (on, 128 z-planes per block), > 1 = on with that many
                           z-planes per block, < 0 = off. */
+    int sweep2_tx, sweep2_ty; /* column of the one-pass smoothing pair: 32 x 16,
+                          64 x 16 or 32 x 32 cells (0 = default 32 x 16; the
+                          autotuner may pick another per level) */
 } nyxpm_opts;
 
 /* ---- lifecycle ---------------------------------------------------------- */

@@ -46,7 +46,7 @@ class Opts(ctypes.Structure):
                 ("separate_kicks", ctypes.c_int), ("coarse", ctypes.c_int), ("particle_cfl", ctypes.c_double),
                 ("resort_slack", ctypes.c_double), ("phi_guess", ctypes.c_int), ("residual", ctypes.c_int),
                 ("graph_cycles", ctypes.c_int), ("ir_first", ctypes.c_int),
-                ("sweep2", ctypes.c_int)]
+                ("sweep2", ctypes.c_int), ("sweep2_tx", ctypes.c_int), ("sweep2_ty", ctypes.c_int)]
 
 
 # (name, restype, argtypes) for every function in include/nyxpm.h

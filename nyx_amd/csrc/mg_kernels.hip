@@ -31,6 +31,29 @@ __device__ __forceinline__ T lap7(const T* a, int i, int j, int k, int nx, int n
     return lx + ly + lz;
 }
 
+// One Gauss-Seidel update of a cell with the periodic 7-point operator:
+//   e + (r - gamma*e - dhx*sx - dhy*sy - dhz*sz) / gamma
+// where sx, sy, sz are the sums of the cell's two neighbours along x, y, z.
+// FP64: the operations in the written order, unfused, as the CPU restatement
+// does them (Source/Gravity/Gravity.cpp:1318 hands the solve to AMReX's MLMG,
+// whose red-black smoother this is).  FP32 (the correction V-cycles of mixed
+// precision, which no test compares bit for bit with the CPU) as fused
+// multiply-adds: 5 operations after the three sums instead of 10.
+template <typename T>
+__device__ __forceinline__ T gs_update(T ec, T rr, T sx, T sy, T sz, T gamma, T inv_gamma, T dhx, T dhy,
+                                       T dhz) {
+    if constexpr (std::is_same<T, float>::value) {
+        float t = __builtin_fmaf(-gamma, ec, rr);
+        t = __builtin_fmaf(-dhx, sx, t);
+        t = __builtin_fmaf(-dhy, sy, t);
+        t = __builtin_fmaf(-dhz, sz, t);
+        return __builtin_fmaf(t, inv_gamma, ec);
+    } else {
+        const T res = rr - gamma * ec - dhx * sx - dhy * sy - dhz * sz;
+        return ec + res * inv_gamma;
+    }
+}
+
 // a[c] = v on every cell of each piece.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_fill(const CellSeg* segs, int nseg, const int* active, T* a, T v) {
@@ -67,12 +90,9 @@ __global__ void __launch_bounds__(kBlock) k_gsrb_half(const CellSeg* segs, int n
     int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
     int km = k == 0 ? nz - 1 : k - 1, kp = k == nz - 1 ? 0 : k + 1;
     const size_t c = cidx(i, j, k, nx, ny);
-    T ec = E[c];
-    T res = R[c] - gamma * ec
-          - dhx * (E[cidx(im, j, k, nx, ny)] + E[cidx(ip, j, k, nx, ny)])
-          - dhy * (E[cidx(i, jm, k, nx, ny)] + E[cidx(i, jp, k, nx, ny)])
-          - dhz * (E[cidx(i, j, km, nx, ny)] + E[cidx(i, j, kp, nx, ny)]);
-    E[c] = ec + res * inv_gamma;
+    E[c] = gs_update(E[c], R[c], E[cidx(im, j, k, nx, ny)] + E[cidx(ip, j, k, nx, ny)],
+                     E[cidx(i, jm, k, nx, ny)] + E[cidx(i, jp, k, nx, ny)],
+                     E[cidx(i, j, km, nx, ny)] + E[cidx(i, j, kp, nx, ny)], gamma, inv_gamma, dhx, dhy, dhz);
 }
 
 // One full red-black Gauss-Seidel sweep (red half, then black half) in a single
@@ -396,11 +416,11 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
                 // black cells of planes kr-1, kr, kr+1 are not yet updated):
                 // the same operations on literal zeros, the same bits
                 const T ec = ZERO_IN ? T(0) : se[sc][ey][ex];
-                const T res = sr[rs][ly][lx + RSH] - gamma * ec
-                            - dhx * (ZERO_IN ? T(0) + T(0) : se[sc][ey][ex - 1] + se[sc][ey][ex + 1])
-                            - dhy * (ZERO_IN ? T(0) + T(0) : se[sc][ey - 1][ex] + se[sc][ey + 1][ex])
-                            - dhz * (ZERO_IN ? T(0) + T(0) : se[sm][ey][ex] + se[sp][ey][ex]);
-                se[sc][ey][ex] = ec + res * inv_gamma;
+                se[sc][ey][ex] =
+                    gs_update(ec, sr[rs][ly][lx + RSH],
+                              ZERO_IN ? T(0) + T(0) : se[sc][ey][ex - 1] + se[sc][ey][ex + 1],
+                              ZERO_IN ? T(0) + T(0) : se[sc][ey - 1][ex] + se[sc][ey + 1][ex],
+                              ZERO_IN ? T(0) + T(0) : se[sm][ey][ex] + se[sp][ey][ex], gamma, inv_gamma, dhx, dhy, dhz);
             }
         }
         __syncthreads();
@@ -414,11 +434,9 @@ __global__ void __launch_bounds__(TX * TY / 2) k_rb_sweep(const CellSeg* segs, i
             const int bxl = lx + (1 - par);            // local x of the black cell
             const int ex = bxl + 2, ey = ly + 2;
             const T ec = ZERO_IN ? T(0) : se[sc][ey][ex];   // the black cell's input
-            const T res = sr[rs][ly + 1][bxl + 1 + RSH] - gamma * ec
-                        - dhx * (se[sc][ey][ex - 1] + se[sc][ey][ex + 1])
-                        - dhy * (se[sc][ey - 1][ex] + se[sc][ey + 1][ex])
-                        - dhz * (se[sm][ey][ex] + se[sp][ey][ex]);
-            const T eb = ec + res * inv_gamma;
+            const T eb = gs_update(ec, sr[rs][ly + 1][bxl + 1 + RSH], se[sc][ey][ex - 1] + se[sc][ey][ex + 1],
+                                   se[sc][ey - 1][ex] + se[sc][ey + 1][ex], se[sm][ey][ex] + se[sp][ey][ex], gamma,
+                                   inv_gamma, dhx, dhy, dhz);
             const T er = se[sc][ey][lx + par + 2];     // the red cell, updated this sweep
             const T e0 = par == 0 ? er : eb, e1 = par == 0 ? eb : er;
             const size_t o = cidx(x0 + lx, gy, wrapz(kb), nx, ny);
@@ -576,6 +594,11 @@ __global__ void __launch_bounds__((sw2_threads<TX, TY>())) k_rb_sweep2(const Cel
     // iterations of k0's parity.
     const int c0 = (gy + k0) & 1;
     const int lxe = la + (c0 ? 1 : -1), lxo = la + (c0 ? -1 : 1);
+    // a wavefront whose rows all lie outside the region a step must get right
+    // skips that step (B, C, D: rows 2 .. TY+5, 3 .. TY+4, 4 .. TY+3)
+    const int wv = __builtin_amdgcn_readfirstlane(tid) / 64;
+    const int wr0 = wv * 64 / PX, wr1 = min(PY - 1, (wv * 64 + 63) / PX);
+    const bool doB = wr1 >= 2 && wr0 <= TY + 5, doC = wr1 >= 3 && wr0 <= TY + 4, doD = wr1 >= 4 && wr0 <= TY + 3;
 
     // e and r of the pair: index i <-> plane k+1-i (e), k-i (r) at the top of
     // iteration k
@@ -657,43 +680,35 @@ __global__ void __launch_bounds__((sw2_threads<TX, TY>())) k_rb_sweep2(const Cel
         load_r(k + 2, qr[qn]);
         // A: sweep 1, red, plane k
         {
-            T ec = Er[1], res;
             if (ZERO_IN) {
-                ec = T(0);
-                res = Fr[0] - gamma * ec - dhx * (T(0) + T(0)) - dhy * (T(0) + T(0)) - dhz * (T(0) + T(0));
+                const T z = T(0);
+                Er[1] = gs_update(z, Fr[0], z + z, z + z, z + z, gamma, inv_gamma, dhx, dhy, dhz);
             } else {
                 const T* L = sblk[s0];
-                res = Fr[0] - gamma * ec - dhx * (Eb[1] + L[lx]) - dhy * (L[la - PX] + L[la + PX]) -
-                      dhz * (Eb[2] + Eb[0]);
+                Er[1] = gs_update(Er[1], Fr[0], Eb[1] + L[lx], L[la - PX] + L[la + PX], Eb[2] + Eb[0], gamma,
+                                  inv_gamma, dhx, dhy, dhz);
             }
-            Er[1] = ec + res * inv_gamma;
             sred[s0][la] = Er[1];
         }
         // B: sweep 1, black, plane k-1
-        {
+        if (doB) {
             const T* L = sred[s1];
-            const T ec = Eb[2];
-            const T res = Fb[1] - gamma * ec - dhx * (Er[2] + L[lx]) - dhy * (L[la - PX] + L[la + PX]) -
-                          dhz * (Er[3] + Er[1]);
-            Eb[2] = ec + res * inv_gamma;
+            Eb[2] = gs_update(Eb[2], Fb[1], Er[2] + L[lx], L[la - PX] + L[la + PX], Er[3] + Er[1], gamma, inv_gamma,
+                              dhx, dhy, dhz);
             sblk[s1][la] = Eb[2];
         }
         // C: sweep 2, red, plane k-2
-        {
+        if (doC) {
             const T* L = sblk[s2];
-            const T ec = Er[3];
-            const T res = Fr[2] - gamma * ec - dhx * (Eb[3] + L[lx]) - dhy * (L[la - PX] + L[la + PX]) -
-                          dhz * (Eb[4] + Eb[2]);
-            Er[3] = ec + res * inv_gamma;
+            Er[3] = gs_update(Er[3], Fr[2], Eb[3] + L[lx], L[la - PX] + L[la + PX], Eb[4] + Eb[2], gamma, inv_gamma,
+                              dhx, dhy, dhz);
             sred[s2][la] = Er[3];
         }
         // D: sweep 2, black, plane k-3, and the plane out
-        {
+        if (doD) {
             const T* L = sred[s3];
-            const T ec = Eb[4];
-            const T res = Fb[3] - gamma * ec - dhx * (Er[4] + L[lx]) - dhy * (L[la - PX] + L[la + PX]) -
-                          dhz * (Er[5] + Er[3]);
-            Eb[4] = ec + res * inv_gamma;
+            Eb[4] = gs_update(Eb[4], Fb[3], Er[4] + L[lx], L[la - PX] + L[la + PX], Er[5] + Er[3], gamma, inv_gamma,
+                              dhx, dhy, dhz);
         }
         const int kb = k - 3;
         if (own && kb >= z0 && kb < z1) {
@@ -980,12 +995,9 @@ __device__ __forceinline__ void coarse_half(const CellSeg& sg, T* E, const T* R,
         int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
         int km = k == 0 ? nz - 1 : k - 1, kp = k == nz - 1 ? 0 : k + 1;
         const int c = (k * ny + j) * nx + i;
-        T ec = E[c];
-        T res = R[c] - gamma * ec
-              - dhx * (E[(k * ny + j) * nx + im] + E[(k * ny + j) * nx + ip])
-              - dhy * (E[(k * ny + jm) * nx + i] + E[(k * ny + jp) * nx + i])
-              - dhz * (E[(km * ny + j) * nx + i] + E[(kp * ny + j) * nx + i]);
-        E[c] = ec + res * inv_gamma;
+        E[c] = gs_update(E[c], R[c], E[(k * ny + j) * nx + im] + E[(k * ny + j) * nx + ip],
+                         E[(k * ny + jm) * nx + i] + E[(k * ny + jp) * nx + i],
+                         E[(km * ny + j) * nx + i] + E[(kp * ny + j) * nx + i], gamma, inv_gamma, dhx, dhy, dhz);
     }
 }
 

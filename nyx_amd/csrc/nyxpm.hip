@@ -264,10 +264,18 @@ struct nyxpm_ctx {
     };
     std::vector<SweepCfg> sw[2];
     // FP32 levels whose pre- and post-smoothing pairs run as one pass each
-    // (k_rb_sweep2): launch table per depth (count 0: two k_rb_sweep passes)
-    std::vector<Table> t_sw2;
+    // (k_rb_sweep2): column shape, z-chunk and launch table per depth (count 0:
+    // two k_rb_sweep passes)
+    struct Sw2Cfg {
+        int tx = kSw2TX, ty = kSw2TY;
+        int zc = 0;        // z-chunk chosen by the autotuner (0: sw2_zc)
+        double us = 0.0;   // autotuned launch time (0: not tuned)
+        Table t;
+    };
+    std::vector<Sw2Cfg> sw2;
     bool sweep2 = true;  // opts.sweep2 >= 0
     int sw2_zc = 128;    // z-planes per block of k_rb_sweep2
+    int sw2_tx = kSw2TX, sw2_ty = kSw2TY;  // its column (before autotuning)
     bool sw2_auto = true;  // default z-chunk (not set in the options)
     std::vector<int> pre_buf;    // buffer holding e after pre-smoothing, per depth
     std::vector<int> fin_buf;    // buffer holding e after post-smoothing, per depth
@@ -498,6 +506,20 @@ void sweep_seg_shape(nyxpm_ctx* c, int pr, int d, const Level& L, CellSeg& w) {
     w.nblk = (int)(cols * nzc);
 }
 
+// Whether a level suits k_rb_sweep2 with a tx x ty column.
+bool sw2_fits(const Level& L, int tx, int ty) {
+    return L.n[0] % tx == 0 && L.n[1] % ty == 0 && L.n[2] % 2 == 0 && L.n[2] >= 8 &&
+           (long long)L.n[0] * L.n[1] < (1LL << 31);
+}
+
+// Blocks and z-chunk of one box's k_rb_sweep2 piece for depth d's current
+// shape and z-chunk.
+void sw2_seg_shape(nyxpm_ctx* c, int d, const Level& L, CellSeg& w) {
+    const auto& s2 = c->sw2[d];
+    w.zchunk = std::min(s2.zc > 0 ? s2.zc : c->sw2_zc, L.n[2]);
+    w.nblk = (L.n[0] / s2.tx) * (L.n[1] / s2.ty) * ((L.n[2] + w.zchunk - 1) / w.zchunk);
+}
+
 // Coarse tile of k_resid_restrict_z: kRrCX x CY coarse columns, CY by precision
 // (pr 0: FP64, 1: FP32) so that the LDS ring stays near 30 KB.
 constexpr int kRrCX = 32, kRrCY64 = 4, kRrCY32 = 8;
@@ -543,7 +565,11 @@ int build_tables(nyxpm_ctx* c) {
     // not depend on the column shape, so both precisions share it).
     c->t_full.assign(c->dmax, {});
     c->t_half.assign(c->dmax, {});
-    c->t_sw2.assign(c->dmax, {});
+    c->sw2.assign(c->dmax, {});
+    for (auto& s2 : c->sw2) {
+        s2.tx = c->sw2_tx;
+        s2.ty = c->sw2_ty;
+    }
     for (int pr = 0; pr < 2; ++pr) {
         c->t_rrz[pr].assign(c->dmax, {});
         c->t_rrc[pr].assign(c->dmax, {});
@@ -674,22 +700,20 @@ int build_tables(nyxpm_ctx* c) {
             const Box& B = c->boxes[b];
             if (d >= B.dc) continue;
             const Level& L = B.lev[d];
-            if (L.n[0] % kSw2TX || L.n[1] % kSw2TY || L.n[2] % 2 || L.n[2] < 8 ||
-                (long long)L.n[0] * L.n[1] >= (1LL << 31)) {
+            if (!sw2_fits(L, c->sw2_tx, c->sw2_ty)) {
                 ok2 = false;
                 break;
             }
             CellSeg w = mk(b, d, 1);
             w.ebuf = coarse_buf(B, d);
-            w.zchunk = std::min(c->sw2_zc, L.n[2]);
-            w.nblk = (L.n[0] / kSw2TX) * (L.n[1] / kSw2TY) * ((L.n[2] + w.zchunk - 1) / w.zchunk);
+            sw2_seg_shape(c, d, L, w);
             blocks2 += w.nblk;
             v2.push_back(w);
         }
         // (the default z-chunk only where the level offers enough blocks; a
         // z-chunk set in the options wherever the shape fits)
         if (!ok2 || (c->sw2_auto && blocks2 < 1024)) v2.clear();
-        c->t_sw2[d] = add_table(v2);
+        c->sw2[d].t = add_table(v2);
     }
     {
         std::vector<CellSeg> vb;
@@ -1190,16 +1214,24 @@ void launch_sweep(nyxpm_ctx* c, int d, const T* ein, T* eout, const T* r, const 
 template <int MODE>
 void launch_sweep2(nyxpm_ctx* c, int d, const float* ein, float* eout, const float* r, const float* ec0,
                    const float* ec1) {
-    const auto& t = c->t_sw2[d];
-    hipLaunchKernelGGL((k_rb_sweep2<float, kSw2TX, kSw2TY, MODE>), dim3(t.blocks),
-                       dim3(sw2_threads<kSw2TX, kSw2TY>()), 0,
-                       c->stream, c->segs(t), t.count, c->active, ein, eout, r, ec0, ec1, c->phi);
+    const auto& s2 = c->sw2[d];
+    const auto& t = s2.t;
+#define NYXPM_SW2(TX, TY)                                                                               \
+    hipLaunchKernelGGL((k_rb_sweep2<float, TX, TY, MODE>), dim3(t.blocks), dim3(sw2_threads<TX, TY>()), 0, \
+                       c->stream, c->segs(t), t.count, c->active, ein, eout, r, ec0, ec1, c->phi)
+    if (s2.tx == 64 && s2.ty == 16)
+        NYXPM_SW2(64, 16);
+    else if (s2.tx == 32 && s2.ty == 32)
+        NYXPM_SW2(32, 32);
+    else
+        NYXPM_SW2(32, 16);
+#undef NYXPM_SW2
 }
 
 // Whether depth d runs its smoothing pairs as k_rb_sweep2 passes.
 template <typename T>
 bool two_sweep(const nyxpm_ctx* c, int d) {
-    return sizeof(T) == 4 && d < (int)c->t_sw2.size() && c->t_sw2[d].count > 0;
+    return sizeof(T) == 4 && d < (int)c->sw2.size() && c->sw2[d].t.count > 0;
 }
 
 // One V-cycle on every active box.  Returns in *updated whether phi += e has
@@ -1777,6 +1809,16 @@ int nyxpm_create_batch(nyxpm_ctx** out, int nbox, const nyxpm_geom* geoms, const
         c->fuse_res = opts->residual == 1;
         if (opts->graph_cycles > 0) c->graph_cycles = opts->graph_cycles;
         if (opts->ir_first > 0) c->ir_first = opts->ir_first;
+        if (opts->sweep2_tx > 0 || opts->sweep2_ty > 0) {
+            const int tx = opts->sweep2_tx > 0 ? opts->sweep2_tx : kSw2TX;
+            const int ty = opts->sweep2_ty > 0 ? opts->sweep2_ty : kSw2TY;
+            if (!((tx == 32 && ty == 16) || (tx == 64 && ty == 16) || (tx == 32 && ty == 32))) {
+                delete c;
+                return NYXPM_ERR_ARG;
+            }
+            c->sw2_tx = tx;
+            c->sw2_ty = ty;
+        }
         if (opts->sweep2 < 0) c->sweep2 = false;
         if (opts->sweep2 > 1) {
             c->sw2_zc = std::max(8, opts->sweep2);
@@ -2889,6 +2931,23 @@ static int rebuild_sweep_table(nyxpm_ctx* c, int pr, int d) {
     return NYXPM_OK;
 }
 
+// Re-derive the k_rb_sweep2 pieces of depth d in place for its current shape
+// and z-chunk, and upload them.
+static int rebuild_sw2_table(nyxpm_ctx* c, int d) {
+    auto& t = c->sw2[d].t;
+    int blk = 0;
+    for (int q = 0; q < t.count; ++q) {
+        CellSeg& w = c->seg_host[t.first + q];
+        sw2_seg_shape(c, d, c->boxes[w.box].lev[d], w);
+        w.blk0 = blk;
+        blk += w.nblk;
+    }
+    t.blocks = blk;
+    HIPCK(c, hipMemcpy(c->seg_pool + t.first, c->seg_host.data() + t.first, t.count * sizeof(CellSeg),
+                       hipMemcpyHostToDevice));
+    return NYXPM_OK;
+}
+
 int nyxpm_autotune(nyxpm_ctx* c, int precision_mask, char* report, int report_len) {
     if (!c) return NYXPM_ERR_ARG;
     int rc;
@@ -2908,6 +2967,7 @@ int nyxpm_autotune(nyxpm_ctx* c, int precision_mask, char* report, int report_le
         for (int d = 0; d + 1 < c->dmax; ++d) {
             auto& cfg = c->sw[pr][d];
             if (!cfg.tx || cfg.t.count == 0) continue;
+            if (pr == 1 && two_sweep<float>(c, d)) continue;  // k_rb_sweep2 smooths this depth
             const int tx0 = cfg.tx, ty0 = cfg.ty, zc0 = cfg.zc;
             double best = 1e30;
             int btx = tx0, bty = ty0, bzc = zc0;
@@ -2961,6 +3021,62 @@ int nyxpm_autotune(nyxpm_ctx* c, int precision_mask, char* report, int report_le
             rep += buf;
             first_entry = false;
         }
+    }
+    // the two-sweep pass: column shape and z-chunk per depth (FP32)
+    static const int sw2_shapes[][2] = {{32, 16}, {64, 16}, {32, 32}};
+    static const int sw2_zcs[] = {32, 64, 128, 256, 512};
+    for (int d = 0; (precision_mask & 2) && d + 1 < c->dmax; ++d) {
+        auto& s2 = c->sw2[d];
+        if (s2.t.count == 0) continue;
+        const int tx0 = s2.tx, ty0 = s2.ty;
+        int maxnz = 0;
+        for (int q = 0; q < s2.t.count; ++q)
+            maxnz = std::max(maxnz, c->boxes[c->seg_host[s2.t.first + q].box].lev[d].n[2]);
+        double best = 1e30;
+        int btx = tx0, bty = ty0, bzc = s2.zc;
+        for (auto& sh : sw2_shapes) {
+            bool ok = true;
+            for (int q = 0; q < s2.t.count && ok; ++q)
+                ok = sw2_fits(c->boxes[c->seg_host[s2.t.first + q].box].lev[d], sh[0], sh[1]);
+            if (!ok) continue;
+            for (int zc : sw2_zcs) {
+                if (zc != sw2_zcs[0] && zc >= 2 * maxnz) continue;
+                s2.tx = sh[0];
+                s2.ty = sh[1];
+                s2.zc = zc;
+                if ((rc = rebuild_sw2_table(c, d))) return rc;
+                float ms = 0.f;
+                for (int rep_i = 0; rep_i < 2; ++rep_i) {  // warm-up, then a timed batch of 4
+                    HIPCK(c, hipEventRecord(e0, c->stream));
+                    for (int k = 0; k < (rep_i ? 4 : 1); ++k)
+                        launch_sweep2<0>(c, d, c->e32, c->e32b, c->r32, c->e32, c->e32b);
+                    HIPCK(c, hipEventRecord(e1, c->stream));
+                    HIPCK(c, hipEventSynchronize(e1));
+                    HIPCK(c, hipEventElapsedTime(&ms, e0, e1));
+                }
+                LAUNCHCK(c);
+                const double us = ms * 1e3 / 4.0;
+                if (c->verbose)
+                    std::fprintf(stderr, "nyxpm autotune: sweep2.d%d %dx%d z-chunk %d: %.2f us\n", d, sh[0], sh[1],
+                                 zc, us);
+                if (us < best) {
+                    best = us;
+                    btx = sh[0];
+                    bty = sh[1];
+                    bzc = zc;
+                }
+            }
+        }
+        s2.tx = btx;
+        s2.ty = bty;
+        s2.zc = bzc;
+        s2.us = best;
+        if ((rc = rebuild_sw2_table(c, d))) return rc;
+        char buf[160];
+        snprintf(buf, sizeof(buf), "%s\"sweep2.d%d\": {\"shape\": [%d, %d], \"zchunk\": %d, \"us\": %.2f}",
+                 first_entry ? "" : ", ", d, btx, bty, bzc, best);
+        rep += buf;
+        first_entry = false;
     }
     rep += "}";
     hipEventDestroy(e0);

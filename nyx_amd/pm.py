@@ -60,7 +60,7 @@ class PMContext:
     def __init__(self, geoms, precision="fp64", tile=None, nu1=2, nu2=2, max_iter=200, smoother="fused",
                  kick="tiled", sweep_shape=None, sweep_zchunk=0, kick_chunk=0, pair_kicks=True,
                  coarse="workgroup", particle_cfl=0.0, resort_slack=0.0, phi_guess="default",
-                 residual="separate", graph_cycles=0, ir_first=0, sweep2=0):
+                 residual="separate", graph_cycles=0, ir_first=0, sweep2=0, sweep2_shape=None, verbose=0):
         """resort_slack: free slots per particle each tile gets at a full sort, for
         the incremental re-sort after kick-drifts (0: library default 0.25;
         negative: a full sort after every kick-drift).  phi_guess: initial guess of
@@ -68,7 +68,9 @@ class PMContext:
         (2 phi_n - phi_{n-1}).  ir_first: mixed precision, FP32 V-cycles before a
         solve's first FP64 residual test (0: library default; 1: after every
         cycle).  sweep2: FP32 smoothing pairs as one pass (0: on, > 1: on with
-        that many z-planes per block, < 0: off)."""
+        that many z-planes per block, < 0: off); sweep2_shape: its column,
+        (32, 16), (64, 16) or (32, 32).  verbose: the library reports to stderr
+        (the autotuner's trials)."""
         if isinstance(geoms, BoxGeom):
             geoms = [geoms]
         self.geoms = list(geoms)
@@ -97,6 +99,9 @@ class PMContext:
         o.graph_cycles = int(graph_cycles)
         o.ir_first = int(ir_first)
         o.sweep2 = int(sweep2)
+        if sweep2_shape is not None:
+            o.sweep2_tx, o.sweep2_ty = int(sweep2_shape[0]), int(sweep2_shape[1])
+        o.verbose = int(verbose)
         if tile is not None:
             for d in range(3):
                 o.tile[d] = int(tile[d])

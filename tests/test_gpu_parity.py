@@ -148,28 +148,34 @@ def test_sweep_shapes_equal_half_sweeps(nyx, shape, zc, precision):
     assert np.array_equal(out[0], out[1])
 
 
-@pytest.mark.parametrize("zc", [8, 13])
-def test_two_sweep_pass_equals_two_passes(nyx, zc):
-    # FP32 smoothing pairs as one pass (k_rb_sweep2, zc planes per block, uneven
-    # last chunks) against two k_rb_sweep passes, bit for bit: fixed-iteration
-    # solves from phi = 0 on a cube (two levels in the pass) and a non-cubic box
+@pytest.mark.parametrize("zc,shape", [(8, (32, 16)), (13, (32, 16)), (13, (64, 16)), (8, (32, 32))])
+def test_two_sweep_pass_equals_two_passes(nyx, zc, shape):
+    # FP32 smoothing pairs as one pass (k_rb_sweep2 with a shape x zc column
+    # per block, uneven last chunks) against two k_rb_sweep passes, bit for
+    # bit: fixed-iteration solves from phi = 0 on a cube (two levels in the
+    # pass where the shape fits) and a non-cubic box
+    wide = (2 * shape[0], 64, 32)
     r = np.random.default_rng(21)
-    for n in ((64, 64, 64), (96, 64, 32)):
+    for n in ((64, 64, 64), wide):
         f = r.standard_normal((n[2], n[1], n[0]))
         f -= f.mean()
         out = []
         for sw2 in (zc, -1):
-            with ctx1(nyx, n, precision="mixed", sweep2=sw2) as c:
+            with ctx1(nyx, n, precision="mixed", sweep2=sw2, sweep2_shape=shape) as c:
+                c.timing_enable(True)
                 phi, it, _ = c.solve_poisson(f, fixed_iters=3)
+                c.timing_enable(False)
                 out.append(phi)
+                assert (c.timing("sweep2_0.d0")[0] > 0) == (sw2 > 0)
         assert np.array_equal(out[0], out[1])
     # mixed steps of a batch (grouped FP32 V-cycles: the pre-smoothing pass
     # continues from the previous cycle's correction; the last post-smoothing
     # pass updates phi)
-    sizes = [(64, 64, 64), (96, 64, 32)]
+    sizes = [(64, 64, 64), wide]
     res = []
     for sw2 in (zc, -1):
-        with nyx.PMContext([nyx.BoxGeom(s) for s in sizes], precision="mixed", sweep2=sw2) as c:
+        with nyx.PMContext([nyx.BoxGeom(s) for s in sizes], precision="mixed", sweep2=sw2,
+                           sweep2_shape=shape) as c:
             for b, s in enumerate(sizes):
                 pos, vel, m = nyx.ics.plummer_clustered(s[0], 70 + b, n_spheres=3) if s[0] == s[1] == s[2] \
                     else _noncubic_inputs(nyx, s, 70 + b)
