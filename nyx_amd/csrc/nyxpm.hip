@@ -270,6 +270,7 @@ struct nyxpm_ctx {
         int tx = kSw2TX, ty = kSw2TY;
         int zc = 0;        // z-chunk chosen by the autotuner (0: sw2_zc)
         double us = 0.0;   // autotuned launch time (0: not tuned)
+        bool on = false;   // the pass is used (t: its pieces wherever the level fits)
         Table t;
     };
     std::vector<Sw2Cfg> sw2;
@@ -710,10 +711,12 @@ int build_tables(nyxpm_ctx* c) {
             blocks2 += w.nblk;
             v2.push_back(w);
         }
-        // (the default z-chunk only where the level offers enough blocks; a
-        // z-chunk set in the options wherever the shape fits)
-        if (!ok2 || (c->sw2_auto && blocks2 < 1024)) v2.clear();
+        // used with the default z-chunk where the level offers enough blocks (the
+        // autotuner may switch it on elsewhere), with a z-chunk set in the
+        // options wherever the shape fits
+        if (!ok2) v2.clear();
         c->sw2[d].t = add_table(v2);
+        c->sw2[d].on = ok2 && !v2.empty() && !(c->sw2_auto && blocks2 < 1024);
     }
     {
         std::vector<CellSeg> vb;
@@ -1231,7 +1234,7 @@ void launch_sweep2(nyxpm_ctx* c, int d, const float* ein, float* eout, const flo
 // Whether depth d runs its smoothing pairs as k_rb_sweep2 passes.
 template <typename T>
 bool two_sweep(const nyxpm_ctx* c, int d) {
-    return sizeof(T) == 4 && d < (int)c->sw2.size() && c->sw2[d].t.count > 0;
+    return sizeof(T) == 4 && d < (int)c->sw2.size() && c->sw2[d].on && c->sw2[d].t.count > 0;
 }
 
 // One V-cycle on every active box.  Returns in *updated whether phi += e has
@@ -3071,10 +3074,15 @@ int nyxpm_autotune(nyxpm_ctx* c, int precision_mask, char* report, int report_le
         s2.ty = bty;
         s2.zc = bzc;
         s2.us = best;
+        // where the default left the pass off (too few blocks), it replaces the
+        // two k_rb_sweep passes if it is faster than two of them
+        const double us1 = c->sw[1][d].us;
+        if (!s2.on && us1 > 0.0 && best < 2.0 * us1) s2.on = true;
         if ((rc = rebuild_sw2_table(c, d))) return rc;
-        char buf[160];
-        snprintf(buf, sizeof(buf), "%s\"sweep2.d%d\": {\"shape\": [%d, %d], \"zchunk\": %d, \"us\": %.2f}",
-                 first_entry ? "" : ", ", d, btx, bty, bzc, best);
+        char buf[200];
+        snprintf(buf, sizeof(buf),
+                 "%s\"sweep2.d%d\": {\"shape\": [%d, %d], \"zchunk\": %d, \"us\": %.2f, \"on\": %s}",
+                 first_entry ? "" : ", ", d, btx, bty, bzc, best, s2.on ? "true" : "false");
         rep += buf;
         first_entry = false;
     }
