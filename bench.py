@@ -603,15 +603,19 @@ def kernel_level(name, grid, n, tuned, sw2_zc=128):
     bench line's config holds them); 0 for the kernels that only run on the
     finest level, None if no level matches."""
     import re
-    mt = re.search(r"k_rb_sweep2<(\w+), *(\d+), *(\d+), *(\d+), *(\d+)>", name)
+    mt = re.search(r"k_rb_sweep2<(\w+), *(\d+), *(\d+), *(\d+)>", name)
     if mt:
-        tx, ty, nt = int(mt.group(2)), int(mt.group(3)), int(mt.group(5))
+        tx, ty = int(mt.group(2)), int(mt.group(3))
+        nt = (tx + 8) // 2 * (ty + 8)
         for d in range(12):
             nd = n >> d
             if nd < 8:
                 break
-            zc = min(sw2_zc, nd)
-            if (nd // tx) * (nd // ty) * _ceil(nd, zc) * nt == grid:
+            cfg = (tuned or {}).get(f"sweep2.d{d}")
+            if cfg and tuple(cfg["shape"]) != (tx, ty):
+                continue
+            zc = min(cfg["zchunk"] if cfg else sw2_zc, nd)
+            if nd % tx == 0 and nd % ty == 0 and (nd // tx) * (nd // ty) * _ceil(nd, zc) * nt == grid:
                 return d
         return None
     mt = re.search(r"k_rb_sweep<(\w+), *(\d+), *(\d+), *(\d+)>", name)

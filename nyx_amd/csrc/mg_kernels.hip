@@ -31,23 +31,30 @@ __device__ __forceinline__ T lap7(const T* a, int i, int j, int k, int nx, int n
     return lx + ly + lz;
 }
 
+// FP32 Gauss-Seidel update from the scaled residual rs = r * (1/gamma) and the
+// scaled coefficients cx = dhx * (1/gamma), ...: gs_update<float>'s arithmetic.
+__device__ __forceinline__ float gs_update_scaled(float rs, float sx, float sy, float sz, float cx, float cy,
+                                                  float cz) {
+    float t = __builtin_fmaf(-cx, sx, rs);
+    t = __builtin_fmaf(-cy, sy, t);
+    return __builtin_fmaf(-cz, sz, t);
+}
+
 // One Gauss-Seidel update of a cell with the periodic 7-point operator:
 //   e + (r - gamma*e - dhx*sx - dhy*sy - dhz*sz) / gamma
 // where sx, sy, sz are the sums of the cell's two neighbours along x, y, z.
 // FP64: the operations in the written order, unfused, as the CPU restatement
 // does them (Source/Gravity/Gravity.cpp:1318 hands the solve to AMReX's MLMG,
 // whose red-black smoother this is).  FP32 (the correction V-cycles of mixed
-// precision, which no test compares bit for bit with the CPU) as fused
-// multiply-adds: 5 operations after the three sums instead of 10.
+// precision, which no test compares bit for bit with the CPU): the same value
+// with gamma * (1/gamma) taken as 1, r/gamma - (dhx/gamma) sx - (dhy/gamma) sy -
+// (dhz/gamma) sz, as one product and three fused multiply-adds after the sums
+// (the cell's own value drops out).
 template <typename T>
 __device__ __forceinline__ T gs_update(T ec, T rr, T sx, T sy, T sz, T gamma, T inv_gamma, T dhx, T dhy,
                                        T dhz) {
     if constexpr (std::is_same<T, float>::value) {
-        float t = __builtin_fmaf(-gamma, ec, rr);
-        t = __builtin_fmaf(-dhx, sx, t);
-        t = __builtin_fmaf(-dhy, sy, t);
-        t = __builtin_fmaf(-dhz, sz, t);
-        return __builtin_fmaf(t, inv_gamma, ec);
+        return gs_update_scaled(rr * inv_gamma, sx, sy, sz, dhx * inv_gamma, dhy * inv_gamma, dhz * inv_gamma);
     } else {
         const T res = rr - gamma * ec - dhx * sx - dhy * sy - dhz * sz;
         return ec + res * inv_gamma;
@@ -564,6 +571,21 @@ __global__ void __launch_bounds__((sw2_threads<TX, TY>())) k_rb_sweep2(const Cel
     const int z1 = min(nz, z0 + zc);
     const T dhx = T(sg.dh[0]), dhy = T(sg.dh[1]), dhz = T(sg.dh[2]);
     const T gamma = T(sg.gamma), inv_gamma = T(sg.inv_gamma);
+    // FP32: r is kept scaled by 1/gamma (gs_update_scaled)
+    constexpr bool F32 = std::is_same<T, float>::value;
+    const T cx = dhx * inv_gamma, cy = dhy * inv_gamma, cz = dhz * inv_gamma;
+    auto upd = [&](T ec, T rr, T sx, T sy, T sz) {
+        if constexpr (F32)
+            return gs_update_scaled(rr, sx, sy, sz, cx, cy, cz);
+        else
+            return gs_update(ec, rr, sx, sy, sz, gamma, inv_gamma, dhx, dhy, dhz);
+    };
+    auto rscale = [&](T v) {
+        if constexpr (F32)
+            return v * inv_gamma;
+        else
+            return v;
+    };
     const T* Ein = ein + sg.off;
     T* Eout = eout + sg.off;
     const T* R = r + sg.off;
@@ -621,15 +643,43 @@ __global__ void __launch_bounds__((sw2_threads<TX, TY>())) k_rb_sweep2(const Cel
     }
     // (planes past the last one needed re-read that one: every load is issued
     // in every iteration)
-    auto load_e = [&](int p, CellPair<T>& v, T& cv) {
-        if (ZERO_IN) return;
-        const int pw = wrapz(min(p, z1 + 3));
-        v = *reinterpret_cast<const CellPair<T>*>(Ein + (size_t)pw * plane + po);
-        if (PRO) cv = Ec[(size_t)(pw >> 1) * cplane + cpo];
+    // plane of a load: u (unwrapped, held at lim, the last plane needed), w
+    // (wrapped), its offset and its parent plane's offset.  The loop advances
+    // its cursors one plane per iteration instead of recomputing the offsets
+    // (a scalar multiply chain per load).
+    struct Cursor {
+        int u, lim, w;
+        size_t off, coff;
     };
-    auto load_r = [&](int p, CellPair<T>& v) {
-        const int pw = wrapz(min(p, z1 + 2));
-        v = *reinterpret_cast<const CellPair<T>*>(R + (size_t)pw * plane + po);
+    auto cursor = [&](int u, int lim) {
+        Cursor q;
+        q.u = min(u, lim);
+        q.lim = lim;
+        q.w = wrapz(q.u);
+        q.off = (size_t)q.w * plane;
+        q.coff = (size_t)(q.w >> 1) * cplane;
+        return q;
+    };
+    auto advance = [&](Cursor& q) {
+        if (q.u < q.lim) {
+            ++q.u;
+            if (++q.w == nz) {
+                q.w = 0;
+                q.off = 0;
+                q.coff = 0;
+            } else {
+                q.off += plane;
+                if (!(q.w & 1)) q.coff += cplane;
+            }
+        }
+    };
+    auto load_e = [&](const Cursor& q, CellPair<T>& v, T& cv) {
+        if (ZERO_IN) return;
+        v = *reinterpret_cast<const CellPair<T>*>(Ein + q.off + po);
+        if (PRO) cv = Ec[q.coff + cpo];
+    };
+    auto load_r = [&](const Cursor& q, CellPair<T>& v) {
+        v = *reinterpret_cast<const CellPair<T>*>(R + q.off + po);
     };
     // red and black cell of a loaded e pair whose red index is cr
     auto split_e = [&](int cr, const CellPair<T>& v, T cv, T& red, T& blk) {
@@ -653,21 +703,25 @@ __global__ void __launch_bounds__((sw2_threads<TX, TY>())) k_rb_sweep2(const Cel
         CellPair<T> v;
         T cv = T(0);
         v = CellPair<T>{T(0), T(0)};
-        load_e(k0 - 1, v, cv);
+        load_e(cursor(k0 - 1, z1 + 3), v, cv);
         split_e(c0 ^ 1, v, cv, Er[2], Eb[2]);
-        load_e(k0, v, cv);
+        load_e(cursor(k0, z1 + 3), v, cv);
         split_e(c0, v, cv, Er[1], Eb[1]);
-        load_e(k0 + 1, v, cv);
+        load_e(cursor(k0 + 1, z1 + 3), v, cv);
         split_e(c0 ^ 1, v, cv, Er[0], Eb[0]);
         v = CellPair<T>{T(0), T(0)};
-        load_r(k0, v);
-        Fr[0] = c0 ? v.b : v.a;
-        Fb[0] = c0 ? v.a : v.b;
-        load_e(k0 + 2, qe[1], qc[1]);
-        load_r(k0 + 1, qr[1]);
+        load_r(cursor(k0, z1 + 2), v);
+        Fr[0] = rscale(c0 ? v.b : v.a);
+        Fb[0] = rscale(c0 ? v.a : v.b);
+        load_e(cursor(k0 + 2, z1 + 3), qe[1], qc[1]);
+        load_r(cursor(k0 + 1, z1 + 2), qr[1]);
         if (!ZERO_IN) sblk[0][la] = Eb[1];
     }
     __syncthreads();
+    // the loop's loads: e of plane k+3, r of plane k+2, phi of plane k-1 (held
+    // in [z0, z1-1]); its stores: plane k-3 (from z0)
+    Cursor ce = cursor(k0 + 3, z1 + 3), cr = cursor(k0 + 2, z1 + 2);
+    size_t pho = (size_t)z0 * plane, so = (size_t)z0 * plane;
 
     auto body = [&](auto JJ, int k) {
         constexpr int J = decltype(JJ)::value;
@@ -676,54 +730,61 @@ __global__ void __launch_bounds__((sw2_threads<TX, TY>())) k_rb_sweep2(const Cel
         constexpr int qn = J & 1, qo = (J + 1) & 1;  // queue slot loaded now / consumed now
         const int c = c0 ^ (J & 1);
         const int lx = (J & 1) ? lxo : lxe;
-        load_e(k + 3, qe[qn], qc[qn]);
-        load_r(k + 2, qr[qn]);
-        // A: sweep 1, red, plane k
-        {
-            if (ZERO_IN) {
-                const T z = T(0);
-                Er[1] = gs_update(z, Fr[0], z + z, z + z, z + z, gamma, inv_gamma, dhx, dhy, dhz);
-            } else {
-                const T* L = sblk[s0];
-                Er[1] = gs_update(Er[1], Fr[0], Eb[1] + L[lx], L[la - PX] + L[la + PX], Eb[2] + Eb[0], gamma,
-                                  inv_gamma, dhx, dhy, dhz);
-            }
-            sred[s0][la] = Er[1];
+        load_e(ce, qe[qn], qc[qn]);
+        load_r(cr, qr[qn]);
+        advance(ce);
+        advance(cr);
+        // the in-plane neighbours of all four steps, issued together: each step
+        // reads a half-plane written in the previous iteration and none that
+        // this one writes
+        T xa = T(0), ya0 = T(0), ya1 = T(0);
+        if (!ZERO_IN) {
+            xa = sblk[s0][lx];
+            ya0 = sblk[s0][la - PX];
+            ya1 = sblk[s0][la + PX];
         }
+        const T xb = sred[s1][lx], yb0 = sred[s1][la - PX], yb1 = sred[s1][la + PX];
+        const T xc = sblk[s2][lx], yc0 = sblk[s2][la - PX], yc1 = sblk[s2][la + PX];
+        const T xd = sred[s3][lx], yd0 = sred[s3][la - PX], yd1 = sred[s3][la + PX];
+        // A: sweep 1, red, plane k
+        if (ZERO_IN) {
+            // every e operand an input zero: the update is r/gamma exactly
+            const T z = T(0);
+            if constexpr (F32)
+                Er[1] = Fr[0];
+            else
+                Er[1] = upd(z, Fr[0], z + z, z + z, z + z);
+        } else {
+            Er[1] = upd(Er[1], Fr[0], Eb[1] + xa, ya0 + ya1, Eb[2] + Eb[0]);
+        }
+        sred[s0][la] = Er[1];
         // B: sweep 1, black, plane k-1
         if (doB) {
-            const T* L = sred[s1];
-            Eb[2] = gs_update(Eb[2], Fb[1], Er[2] + L[lx], L[la - PX] + L[la + PX], Er[3] + Er[1], gamma, inv_gamma,
-                              dhx, dhy, dhz);
+            Eb[2] = upd(Eb[2], Fb[1], Er[2] + xb, yb0 + yb1, Er[3] + Er[1]);
             sblk[s1][la] = Eb[2];
         }
         // C: sweep 2, red, plane k-2
         if (doC) {
-            const T* L = sblk[s2];
-            Er[3] = gs_update(Er[3], Fr[2], Eb[3] + L[lx], L[la - PX] + L[la + PX], Eb[4] + Eb[2], gamma, inv_gamma,
-                              dhx, dhy, dhz);
+            Er[3] = upd(Er[3], Fr[2], Eb[3] + xc, yc0 + yc1, Eb[4] + Eb[2]);
             sred[s2][la] = Er[3];
         }
         // D: sweep 2, black, plane k-3, and the plane out
-        if (doD) {
-            const T* L = sred[s3];
-            Eb[4] = gs_update(Eb[4], Fb[3], Er[4] + L[lx], L[la - PX] + L[la + PX], Er[5] + Er[3], gamma, inv_gamma,
-                              dhx, dhy, dhz);
-        }
+        if (doD) Eb[4] = upd(Eb[4], Fb[3], Er[4] + xd, yd0 + yd1, Er[5] + Er[3]);
         const int kb = k - 3;
         if (own && kb >= z0 && kb < z1) {
             // plane k-3: its red cell has index c ^ 1
             const T e0 = c ? Er[4] : Eb[4], e1 = c ? Eb[4] : Er[4];
             if (UPD) {
-                CellPair<double>* q = reinterpret_cast<CellPair<double>*>(Ph + (size_t)kb * plane + po);
+                CellPair<double>* q = reinterpret_cast<CellPair<double>*>(Ph + so + po);
                 *q = CellPair<double>{qp[qn].a + (double)e0, qp[qn].b + (double)e1};
             } else {
-                *reinterpret_cast<CellPair<T>*>(Eout + (size_t)kb * plane + po) = CellPair<T>{e0, e1};
+                *reinterpret_cast<CellPair<T>*>(Eout + so + po) = CellPair<T>{e0, e1};
             }
         }
+        if (kb >= z0) so += plane;
         if (UPD) {
-            const int pp = min(max(k - 1, z0), z1 - 1);
-            qp[qn] = *reinterpret_cast<const CellPair<double>*>(Ph + (size_t)pp * plane + pown);
+            qp[qn] = *reinterpret_cast<const CellPair<double>*>(Ph + pho + pown);
+            if (k - 1 >= z0 && k - 1 < z1 - 1) pho += plane;
         }
         // the black input of plane k+1 for the next iteration's step A
         if (!ZERO_IN) sblk[s3][la] = Eb[0];
@@ -741,16 +802,17 @@ __global__ void __launch_bounds__((sw2_threads<TX, TY>())) k_rb_sweep2(const Cel
             Fr[i] = Fr[i - 1];
             Fb[i] = Fb[i - 1];
         }
-        Fr[0] = c ? qr[qo].a : qr[qo].b;
-        Fb[0] = c ? qr[qo].b : qr[qo].a;
+        Fr[0] = rscale(c ? qr[qo].a : qr[qo].b);
+        Fb[0] = rscale(c ? qr[qo].b : qr[qo].a);
     };
+    // whole groups of four iterations (up to three past kend, whose loads
+    // re-read planes already loaded and whose planes are not written out): one
+    // exit from the loop, so that the loads in flight across its back edge keep
+    // their registers
     for (int k = k0; k <= kend; k += 4) {
         body(std::integral_constant<int, 0>{}, k);
-        if (k + 1 > kend) break;
         body(std::integral_constant<int, 1>{}, k + 1);
-        if (k + 2 > kend) break;
         body(std::integral_constant<int, 2>{}, k + 2);
-        if (k + 3 > kend) break;
         body(std::integral_constant<int, 3>{}, k + 3);
     }
 }

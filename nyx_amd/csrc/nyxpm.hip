@@ -3075,9 +3075,14 @@ int nyxpm_autotune(nyxpm_ctx* c, int precision_mask, char* report, int report_le
         s2.zc = bzc;
         s2.us = best;
         // where the default left the pass off (too few blocks), it replaces the
-        // two k_rb_sweep passes if it is faster than two of them
+        // two k_rb_sweep passes on a level of at least 2^22 cells if it is faster
+        // than two of them (on smaller levels the launches take a few
+        // microseconds, and a timing is mostly launch overhead, which a tracer
+        // inflates)
         const double us1 = c->sw[1][d].us;
-        if (!s2.on && us1 > 0.0 && best < 2.0 * us1) s2.on = true;
+        long long cells = 0;
+        for (int q = 0; q < s2.t.count; ++q) cells += c->boxes[c->seg_host[s2.t.first + q].box].lev[d].nc;
+        if (!s2.on && us1 > 0.0 && best < 2.0 * us1 && cells >= (1LL << 22)) s2.on = true;
         if ((rc = rebuild_sw2_table(c, d))) return rc;
         char buf[200];
         snprintf(buf, sizeof(buf),
