@@ -293,13 +293,25 @@ int nyxpm_pm_steps_comoving(nyxpm_ctx *ctx, int nsteps, double dt, double *a_io,
                             int *iters);
 
 /* ---- tuning ------------------------------------------------------------- */
-/* Autotune the fused smoother: for every multigrid depth (and precision in
- * precision_mask: bit 0 FP64, bit 1 FP32) time each column shape the level
- * allows (128x8, 64x16, 64x8, 32x8, 16x8) and z-chunk (8..256 planes) on this
- * context's own level arrays, and keep the fastest.  Results never depend on the
- * choice (every shape is bit-identical).  report (may be NULL) receives a JSON
- * object {"fp64.d0": {"shape": [tx, ty], "zchunk": z, "us": t}, ...}.  No
- * counterpart in the reference (AMReX's MLMG has fixed launch shapes). */
+/* Autotune launch shapes on this context's own arrays and keep the fastest.
+ * precision_mask bit 0 (FP64) and bit 1 (FP32): for every multigrid depth, the
+ * fused smoother's column shape the level allows (128x8, 64x16, 64x8, 32x8,
+ * 16x8) and z-chunk (2..256 planes); with bit 1 also the one-pass smoothing
+ * pair's column (32x16, 64x16, 32x32) and z-chunk (32..512), switched on at a
+ * level of 2^22 cells or more where it beats two single sweeps.  Bit 2, with
+ * particles uploaded: the particle tile of the deposit and the kicks (16x8x8,
+ * 12x8x8, 24x8x8, 16x16x8, 16x8x4, 32x8x4, 8x8x8), timed on a deposit, fold,
+ * re-sort and paired kick of dt = 0 in the context's precision; the particles
+ * are restored from a copy taken first, so the state is unchanged (the phi
+ * extrapolation history starts over).  The smoother choices never change a
+ * result (every shape gives the same bits); the particle tile changes the
+ * density in its last bit at tile boundaries (the fold adds a cell's parts
+ * from the tiles that hold them in FP64), as the tile option does.  report
+ * (may be NULL) receives a JSON
+ * object {"fp64.d0": {"shape": [tx, ty], "zchunk": z, "us": t}, ...,
+ * "tile": {"shape": [tx, ty, tz], "us": t}, "sweep2.d0": {...}}.  No
+ * counterpart in the reference (AMReX's MLMG and particle tiling have fixed
+ * launch shapes). */
 int nyxpm_autotune(nyxpm_ctx *ctx, int precision_mask, char *report, int report_len);
 
 /* ---- measurement -------------------------------------------------------- */

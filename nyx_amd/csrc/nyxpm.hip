@@ -60,6 +60,7 @@ struct Box {
     long long part_off = 0, np = 0;
     long long span = 0;  // particle slots (np plus the tile slack)
     bool has_mass = false;
+    bool mass_array = false;  // uploaded with per-particle masses (else m_const)
     double m_const = 0.0, mass_total = 0.0, mass_offset = 0.0;
     int fx_exp = 0;
     long long G_off;
@@ -421,6 +422,8 @@ const char* lvl_name(const char* base, int d) {
 // setup
 // ---------------------------------------------------------------------------
 
+void tile_boxes(nyxpm_ctx* c);
+
 int setup_boxes(nyxpm_ctx* c, int nbox, const nyxpm_geom* geoms) {
     c->boxes.resize(nbox);
     long long cell = 0, tiles = 0, mg = 0, Gtot = 0;
@@ -440,8 +443,6 @@ int setup_boxes(nyxpm_ctx* c, int nbox, const nyxpm_geom* geoms) {
             B.dxi[d] = 1.0 / B.dx[d];
             B.rhi[d] = domain_rhi(B.plo[d], B.phi[d], B.n[d]);
             B.len[d] = B.phi[d] - B.plo[d];
-            B.tile[d] = std::min(c->tile[d], B.n[d]);
-            B.ntile[d] = (B.n[d] + B.tile[d] - 1) / B.tile[d];
         }
         B.dv = B.dx[0] * B.dx[1] * B.dx[2];
         B.inv_vol = 1.0 / B.dv;
@@ -473,17 +474,30 @@ int setup_boxes(nyxpm_ctx* c, int nbox, const nyxpm_geom* geoms) {
         Gtot += nb;
         B.cell_off = cell;
         cell += B.lev[0].nc;
-        B.tile_off = tiles;
-        B.ntiles = (long long)B.ntile[0] * B.ntile[1] * B.ntile[2];
-        tiles += B.ntiles;
         c->dmax = std::max(c->dmax, B.nlev);
     }
     c->ncell_total = cell;
-    c->ntiles_total = tiles;
     c->mg_total = mg;
     c->G_total = Gtot;
-    c->halo_stride = (c->tile[0] + 1) * (c->tile[1] + 1) * (c->tile[2] + 1) - c->tile[0] * c->tile[1] * c->tile[2];
+    tile_boxes(c);
     return NYXPM_OK;
+}
+
+// Particle tiles of every box for c->tile (a box smaller than the tile is one
+// tile): counts, offsets, and the halo slots a tile's deposit leaves.
+void tile_boxes(nyxpm_ctx* c) {
+    long long tiles = 0;
+    for (auto& B : c->boxes) {
+        for (int d = 0; d < 3; ++d) {
+            B.tile[d] = std::min(c->tile[d], B.n[d]);
+            B.ntile[d] = (B.n[d] + B.tile[d] - 1) / B.tile[d];
+        }
+        B.tile_off = tiles;
+        B.ntiles = (long long)B.ntile[0] * B.ntile[1] * B.ntile[2];
+        tiles += B.ntiles;
+    }
+    c->ntiles_total = tiles;
+    c->halo_stride = (c->tile[0] + 1) * (c->tile[1] + 1) * (c->tile[2] + 1) - c->tile[0] * c->tile[1] * c->tile[2];
 }
 
 // Blocks and z-chunk of one box's fused-sweep piece for the current shape of
@@ -525,6 +539,8 @@ void sw2_seg_shape(nyxpm_ctx* c, int d, const Level& L, CellSeg& w) {
 // (pr 0: FP64, 1: FP32) so that the LDS ring stays near 30 KB.
 constexpr int kRrCX = 32, kRrCY64 = 4, kRrCY32 = 8;
 inline int rrz_cy(int pr) { return pr == 0 ? kRrCY64 : kRrCY32; }
+
+int build_tseg(nyxpm_ctx* c);
 
 int build_tables(nyxpm_ctx* c) {
     std::vector<CellSeg>& S = c->seg_host;
@@ -783,7 +799,12 @@ int build_tables(nyxpm_ctx* c) {
     if (rc) return rc;
     HIPCK(c, hipMemcpy(c->seg_pool, S.data(), S.size() * sizeof(CellSeg), hipMemcpyHostToDevice));
 
-    // tiles (one block per tile)
+    return build_tseg(c);
+}
+
+// The tile table (one block per tile of each box).
+int build_tseg(nyxpm_ctx* c) {
+    const int nbox = (int)c->boxes.size();
     std::vector<BoxSeg> ts;
     int blk = 0;
     for (int b = 0; b < nbox; ++b) {
@@ -791,8 +812,10 @@ int build_tables(nyxpm_ctx* c) {
         blk += s.nblk;
         ts.push_back(s);
     }
-    rc = dalloc(c, &c->tseg, ts.size());
-    if (rc) return rc;
+    if (!c->tseg) {
+        int rc = dalloc(c, &c->tseg, ts.size());
+        if (rc) return rc;
+    }
     HIPCK(c, hipMemcpy(c->tseg, ts.data(), ts.size() * sizeof(BoxSeg), hipMemcpyHostToDevice));
     c->tseg_n = nbox;
     c->tseg_blocks = blk;
@@ -853,6 +876,27 @@ int build_pseg(nyxpm_ctx* c) {
     return NYXPM_OK;
 }
 
+// Buffers sized by the tile count: the deposit's halo slots, per-tile counts,
+// cursors, starts, bases and capacities, the scan's block sums.
+int alloc_tile_buffers(nyxpm_ctx* c) {
+    int rc;
+    if ((rc = dalloc(c, &c->halo, (size_t)c->ntiles_total * c->halo_stride))) return rc;
+    if ((rc = dalloc(c, &c->counts, c->ntiles_total))) return rc;
+    if ((rc = dalloc(c, &c->cursor, c->ntiles_total))) return rc;
+    if ((rc = dalloc(c, &c->tstart, c->ntiles_total + 1))) return rc;
+    if ((rc = dalloc(c, &c->tbase, c->ntiles_total))) return rc;
+    if ((rc = dalloc(c, &c->tcap, c->ntiles_total))) return rc;
+    c->scan_blocks = nblocks(c->ntiles_total, kBlock * kScanItems);
+    if ((rc = dalloc(c, &c->bsum, c->scan_blocks + 1))) return rc;
+    HIPCK(c, hipMemset(c->counts, 0, c->ntiles_total * sizeof(int)));
+    return NYXPM_OK;
+}
+
+void free_tile_buffers(nyxpm_ctx* c) {
+    dfree(c->halo); dfree(c->counts); dfree(c->cursor); dfree(c->tstart);
+    dfree(c->tbase); dfree(c->tcap); dfree(c->bsum);
+}
+
 int alloc_fields(nyxpm_ctx* c) {
     const int nbox = (int)c->boxes.size();
     int rc;
@@ -860,20 +904,13 @@ int alloc_fields(nyxpm_ctx* c) {
     c->rho = c->own;  // the fold turns the deposit's own cells into rho in place
     if ((rc = dalloc(c, &c->f, c->ncell_total))) return rc;
     if ((rc = dalloc(c, &c->phi, c->ncell_total))) return rc;
-    if ((rc = dalloc(c, &c->halo, (size_t)c->ntiles_total * c->halo_stride))) return rc;
+    if ((rc = alloc_tile_buffers(c))) return rc;
     if ((rc = dalloc(c, &c->partial, (size_t)c->t_l0.blocks))) return rc;
-    if ((rc = dalloc(c, &c->counts, c->ntiles_total))) return rc;
-    if ((rc = dalloc(c, &c->cursor, c->ntiles_total))) return rc;
-    if ((rc = dalloc(c, &c->tstart, c->ntiles_total + 1))) return rc;
-    if ((rc = dalloc(c, &c->tbase, c->ntiles_total))) return rc;
-    if ((rc = dalloc(c, &c->tcap, c->ntiles_total))) return rc;
     if ((rc = dalloc(c, &c->nmove, 1))) return rc;
     if ((rc = dalloc(c, &c->mig, 2))) return rc;
     if ((rc = dalloc(c, &c->guess_ok, 1))) return rc;
     HIPCK(c, hipMemset(c->guess_ok, 0, sizeof(int)));
     HIPCK(c, hipHostMalloc((void**)&c->h_mig, 2 * sizeof(int), hipHostMallocDefault));
-    c->scan_blocks = nblocks(c->ntiles_total, kBlock * kScanItems);
-    if ((rc = dalloc(c, &c->bsum, c->scan_blocks + 1))) return rc;
     if ((rc = dalloc(c, &c->r64, c->mg_total))) return rc;
     if ((rc = dalloc(c, &c->e64, c->mg_total))) return rc;
     if ((rc = dalloc(c, &c->e64b, c->mg_total))) return rc;
@@ -901,7 +938,6 @@ int alloc_fields(nyxpm_ctx* c) {
     HIPCK(c, hipMemset(c->nr, 0, (size_t)nbox * kNormSlots * sizeof(unsigned long long)));
     HIPCK(c, hipMemset(c->nf, 0, (size_t)nbox * kNormSlots * sizeof(unsigned long long)));
     HIPCK(c, hipMemset(c->err_flag, 0, sizeof(int)));
-    HIPCK(c, hipMemset(c->counts, 0, c->ntiles_total * sizeof(int)));
     // bottom Green's functions
     std::vector<double> G(c->G_total);
     for (auto& B : c->boxes) {
@@ -2082,6 +2118,7 @@ static int upload_impl(nyxpm_ctx* c, int box, int64_t np, const double* x, const
     // every box reads the array
     for (auto& BB : c->boxes) BB.has_mass = need_mass;
     B.m_const = m ? 0.0 : m_const;
+    B.mass_array = m != nullptr;
     B.mass_total = total;
     B.mass_offset = total / B.prob_size;
     B.fx_exp = mref > 0.0 ? std::ilogb(mref) : 0;
@@ -2951,6 +2988,72 @@ static int rebuild_sw2_table(nyxpm_ctx* c, int d) {
     return NYXPM_OK;
 }
 
+// Every box's particles in upload (id) order, in device arrays: x, y, z, vx,
+// vy, vz and (for a box uploaded with per-particle masses) m, 7 per box.
+struct ParticleSnapshot {
+    std::vector<double*> a;
+};
+
+static void snapshot_free(ParticleSnapshot& s) {
+    for (double*& p : s.a) dfree(p);
+    s.a.clear();
+}
+
+static int snapshot_take(nyxpm_ctx* c, ParticleSnapshot& s) {
+    int rc;
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    if ((rc = ensure_idle_buffer(c))) return rc;
+    const int nbox = (int)c->boxes.size();
+    s.a.assign(7 * (size_t)nbox, nullptr);
+    const PartPtrs& A = c->P[c->cur];
+    for (int b = 0; b < nbox; ++b) {
+        const Box& B = c->boxes[b];
+        if (B.np == 0) continue;
+        const double* src[7] = {A.x, A.y, A.z, A.vx, A.vy, A.vz, B.mass_array ? A.m : nullptr};
+        for (int q = 0; q < 7; ++q) {
+            if (!src[q]) continue;
+            if ((rc = dalloc(c, &s.a[7 * b + q], B.np))) return rc;
+            if ((rc = download_component(c, b, src[q], s.a[7 * b + q]))) return rc;
+        }
+    }
+    return err_check(c);
+}
+
+// Re-tiles every box with particle tile t and puts the snapshot's particles back
+// through the upload path, which lays out the slots for the new tile count.
+// Positions, velocities and phi come back bit for bit; the next deposit adds a
+// tile-boundary cell's parts in the new tiles' FP64 fold, so the density may
+// differ in its last bit from the old tiling's; the phi extrapolation history
+// starts over, as after any upload.
+static int retile_from(nyxpm_ctx* c, const int t[3], const ParticleSnapshot& s) {
+    int rc;
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    const int nbox = (int)c->boxes.size();
+    std::vector<long long> np(nbox);
+    std::vector<double> mc(nbox);
+    for (int b = 0; b < nbox; ++b) {
+        np[b] = c->boxes[b].np;
+        mc[b] = c->boxes[b].m_const;
+    }
+    for (int d = 0; d < 3; ++d) c->tile[d] = t[d];
+    tile_boxes(c);
+    free_tile_buffers(c);
+    if ((rc = alloc_tile_buffers(c))) return rc;
+    if ((rc = build_tseg(c))) return rc;
+    // an empty layout, then the boxes one by one
+    for (auto& B : c->boxes) {
+        B.np = 0;
+        B.span = 0;
+    }
+    for (int b = 0; b < nbox; ++b) {
+        const double* const* a = s.a.data() + 7 * (size_t)b;
+        if ((rc = upload_impl(c, b, np[b], a[0], a[1], a[2], a[3], a[4], a[5], a[6], mc[b],
+                              hipMemcpyDeviceToDevice)))
+            return rc;
+    }
+    return NYXPM_OK;
+}
+
 int nyxpm_autotune(nyxpm_ctx* c, int precision_mask, char* report, int report_len) {
     if (!c) return NYXPM_ERR_ARG;
     int rc;
@@ -3024,6 +3127,54 @@ int nyxpm_autotune(nyxpm_ctx* c, int precision_mask, char* report, int report_le
             rep += buf;
             first_entry = false;
         }
+    }
+    // the particle tile (deposit, fold, re-sort and paired kick of the current
+    // precision; the trial kicks have dt = 0, and the particles come back from a
+    // snapshot at the end)
+    if ((precision_mask & 4) && c->npart_total > 0) {
+        static const int tiles[][3] = {{16, 8, 8}, {12, 8, 8}, {24, 8, 8}, {16, 16, 8},
+                                       {16, 8, 4}, {32, 8, 4}, {8, 8, 8}};
+        const bool f32 = c->precision == NYXPM_MIXED;
+        ParticleSnapshot snap;
+        if ((rc = snapshot_take(c, snap))) {
+            snapshot_free(snap);
+            return rc;
+        }
+        int best_t[3] = {c->tile[0], c->tile[1], c->tile[2]};
+        double best = 1e30;
+        for (auto& t : tiles) {
+            if ((t[0] + 1) * (t[1] + 1) * (t[2] + 1) * 16 > 64 * 1024) continue;
+            if ((rc = retile_from(c, t, snap))) break;
+            float ms = 0.f;
+            for (int rep_i = 0; rep_i < 2 && !rc; ++rep_i) {  // warm-up, then a timed pair of steps
+                HIPCK(c, hipEventRecord(e0, c->stream));
+                for (int k = 0; k < (rep_i ? 2 : 1) && !rc; ++k) {
+                    if ((rc = phase_deposit(c, false, f32))) break;
+                    rc = phase_kick<kKickPair>(c, 0.0, 1.0, 1.0, 1.0, 1.0);
+                }
+                HIPCK(c, hipEventRecord(e1, c->stream));
+                HIPCK(c, hipEventSynchronize(e1));
+                HIPCK(c, hipEventElapsedTime(&ms, e0, e1));
+            }
+            if (rc) break;
+            const double us = ms * 1e3 / 2.0;
+            if (c->verbose)
+                std::fprintf(stderr, "nyxpm autotune: particle tile %dx%dx%d: %.2f us\n", t[0], t[1], t[2], us);
+            if (us < best) {
+                best = us;
+                for (int d = 0; d < 3; ++d) best_t[d] = t[d];
+            }
+        }
+        if (!rc) rc = retile_from(c, best_t, snap);
+        snapshot_free(snap);
+        if (rc) return rc;
+        // the trial deposits overwrote the density and rhs buffers
+        c->f1_valid = c->f_raw = c->density_valid = false;
+        char buf[160];
+        snprintf(buf, sizeof(buf), "%s\"tile\": {\"shape\": [%d, %d, %d], \"us\": %.2f}",
+                 first_entry ? "" : ", ", best_t[0], best_t[1], best_t[2], best);
+        rep += buf;
+        first_entry = false;
     }
     // the two-sweep pass: column shape and z-chunk per depth (FP32)
     static const int sw2_shapes[][2] = {{32, 16}, {64, 16}, {32, 32}};

@@ -328,12 +328,15 @@ class PMContext:
                                                 float(rel_tol), float(abs_tol), int(bool(nyx_faithful)), ptr(it)))
         return av.value, it
 
-    def autotune(self, fp64=True, fp32=True):
+    def autotune(self, fp64=True, fp32=True, particles=True):
         """Time every fused-sweep column shape and z-chunk per multigrid depth on
-        this context's levels and keep the fastest; returns the choices."""
+        this context's levels, and (particles, with particles loaded) the particle
+        tile for the deposit and kicks, and keep the fastest; returns the choices.
+        The smoother shapes give the same bits; the particle tile changes the
+        density's last bit at tile boundaries (as the tile option does)."""
         import json
         buf = ctypes.create_string_buffer(8192)
-        mask = (1 if fp64 else 0) | (2 if fp32 else 0)
+        mask = (1 if fp64 else 0) | (2 if fp32 else 0) | (4 if particles else 0)
         self._ck(self.L.nyxpm_autotune(self.h, mask, buf, len(buf)))
         return json.loads(buf.value.decode())
 

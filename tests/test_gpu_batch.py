@@ -115,6 +115,47 @@ def test_autotune_keeps_results(nyx):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_particle_tile_autotune_keeps_state(nyx, precision):
+    # the particle tile re-chosen in the middle of a run: the particles go out to
+    # a copy and come back in upload order, so positions, velocities and phi are
+    # bit for bit those before it; the steps after it agree with the run that kept
+    # its tile to rounding (the fold adds a boundary cell's parts from the tiles
+    # that hold them, in FP64, so the density's last bit depends on the tiling);
+    # one box with per-particle masses
+    sizes = [(32, 32, 32), (48, 32, 24)]
+    inputs = [_inputs(nyx, sizes[0], 500), _inputs(nyx, sizes[1], 501, mass=True)]
+    outs = []
+    for tune in (False, True):
+        with nyx.PMContext([nyx.BoxGeom(g) for g in sizes], precision=precision) as c:
+            for b, inp in enumerate(inputs):
+                _upload(c, b, inp)
+            c.pm_init(1.0, 1.0, 1e-10)
+            c.pm_steps(2, 1e-3, rel_tol=1e-10)
+            if tune:
+                before = ([c.download(b) for b in range(len(sizes))], [c.get_phi(b) for b in range(len(sizes))])
+                rep = c.autotune(fp64=False, fp32=False)
+                assert set(rep) == {"tile"} and rep["tile"]["us"] > 0 and len(rep["tile"]["shape"]) == 3
+                for b in range(len(sizes)):
+                    assert np.array_equal(before[1][b], c.get_phi(b))
+                    pa = c.download(b)
+                    for d in range(3):
+                        assert np.array_equal(before[0][b][0][d], pa[0][d])
+                        assert np.array_equal(before[0][b][1][d], pa[1][d])
+                assert c.sum_particle_mass(1) == pytest.approx(inputs[1][2].sum(), rel=1e-15)
+            its = c.pm_steps(2, 1e-3, rel_tol=1e-10)
+            outs.append(([c.download(b) for b in range(len(sizes))], [c.get_phi(b) for b in range(len(sizes))],
+                         [int(v) for v in its]))
+    assert outs[0][2] == outs[1][2]
+    for b in range(len(sizes)):
+        phi0, phi1 = outs[0][1][b], outs[1][1][b]
+        assert np.abs(phi0 - phi1).max() <= 1e-9 * np.abs(phi0).max()
+        for d in range(3):
+            assert np.abs(outs[0][0][b][0][d] - outs[1][0][b][0][d]).max() <= 1e-12
+            v0, v1 = outs[0][0][b][1][d], outs[1][0][b][1][d]
+            assert np.abs(v0 - v1).max() <= 1e-9 * np.abs(v0).max()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("steps,use_graph", [(1, False), (3, False), (6, True)])
 def test_pair_kicks_equal_separate(nyx, precision, steps, use_graph):
     # the closing kick of a step fused with the next step's kick-drift gives the
