@@ -636,6 +636,7 @@ def kernel_level(name, grid, n, tuned, sw2_zc=128):
         return None
     mt = re.search(r"k_resid_restrict_z<(\w+), *(\d+), *(\d+)>", name)
     if mt:
+        pr = "fp32" if mt.group(1) == "float" else "fp64"
         cx, cy = int(mt.group(2)), int(mt.group(3))
         for d in range(12):
             nd = n >> d
@@ -643,9 +644,13 @@ def kernel_level(name, grid, n, tuned, sw2_zc=128):
                 break
             cols = (nd // (2 * cx)) * (nd // (2 * cy))
             cz = nd // 2
-            zc = min(16, cz)
-            while zc > 2 and cols * _ceil(cz, zc) < 1024:
-                zc //= 2
+            cfg = (tuned or {}).get(f"rrz.{pr}.d{d}")
+            if cfg:
+                zc = min(cfg["zchunk"], cz)
+            else:
+                zc = min(16, cz)
+                while zc > 2 and cols * _ceil(cz, zc) < 1024:
+                    zc //= 2
             if cols * _ceil(cz, zc) * cx * cy == grid:
                 return d
         return None

@@ -2971,6 +2971,28 @@ static int rebuild_sweep_table(nyxpm_ctx* c, int pr, int d) {
     return NYXPM_OK;
 }
 
+// Re-derive the residual-restriction pieces of (pr, d) in place with zc coarse
+// planes per block, and upload them.
+static int rebuild_rrz_table(nyxpm_ctx* c, int pr, int d, int zc) {
+    auto& t = c->t_rrz[pr][d];
+    const int cy = rrz_cy(pr);
+    int blk = 0;
+    for (int q = 0; q < t.count; ++q) {
+        CellSeg& w = c->seg_host[t.first + q];
+        const Level& L = c->boxes[w.box].lev[d];
+        const long long cols = (long long)(L.n[0] / (2 * kRrCX)) * (L.n[1] / (2 * cy));
+        const int cz = L.n[2] / 2;
+        w.zchunk = std::min(zc, cz);
+        w.nblk = (int)(cols * ((cz + w.zchunk - 1) / w.zchunk));
+        w.blk0 = blk;
+        blk += w.nblk;
+    }
+    t.blocks = blk;
+    HIPCK(c, hipMemcpy(c->seg_pool + t.first, c->seg_host.data() + t.first, t.count * sizeof(CellSeg),
+                       hipMemcpyHostToDevice));
+    return NYXPM_OK;
+}
+
 // Re-derive the k_rb_sweep2 pieces of depth d in place for its current shape
 // and z-chunk, and upload them.
 static int rebuild_sw2_table(nyxpm_ctx* c, int d) {
@@ -3124,6 +3146,54 @@ int nyxpm_autotune(nyxpm_ctx* c, int precision_mask, char* report, int report_le
             char buf[160];
             snprintf(buf, sizeof(buf), "%s\"%s.d%d\": {\"shape\": [%d, %d], \"zchunk\": %d, \"us\": %.2f}",
                      first_entry ? "" : ", ", pr == 0 ? "fp64" : "fp32", d, btx, bty, bzc, best);
+            rep += buf;
+            first_entry = false;
+        }
+        // the z-chunk (coarse planes per block) of the marching residual-restriction
+        for (int d = 0; d + 1 < c->dmax; ++d) {
+            auto& tz = c->t_rrz[pr][d];
+            if (tz.count == 0) continue;
+            static const int rzcs[] = {2, 4, 8, 16, 32, 64, 128};
+            int maxcz = 0;
+            for (int q = 0; q < tz.count; ++q)
+                maxcz = std::max(maxcz, c->boxes[c->seg_host[tz.first + q].box].lev[d].n[2] / 2);
+            const int zc0 = c->seg_host[tz.first].zchunk;
+            double best = 1e30;
+            int bzc = zc0;
+            for (int zc : rzcs) {
+                if (zc != rzcs[0] && zc >= 2 * maxcz) continue;
+                if ((rc = rebuild_rrz_table(c, pr, d, zc))) return rc;
+                float ms = 0.f;
+                for (int rep_i = 0; rep_i < 2; ++rep_i) {
+                    HIPCK(c, hipEventRecord(e0, c->stream));
+                    for (int k = 0; k < (rep_i ? 4 : 1); ++k) {
+                        if (pr == 0)
+                            hipLaunchKernelGGL((k_resid_restrict_z<double, kRrCX, kRrCY64>), dim3(tz.blocks),
+                                               dim3(kRrCX * kRrCY64), 0, c->stream, c->segs(tz), tz.count, c->active,
+                                               c->e64, c->r64);
+                        else
+                            hipLaunchKernelGGL((k_resid_restrict_z<float, kRrCX, kRrCY32>), dim3(tz.blocks),
+                                               dim3(kRrCX * kRrCY32), 0, c->stream, c->segs(tz), tz.count, c->active,
+                                               c->e32, c->r32);
+                    }
+                    HIPCK(c, hipEventRecord(e1, c->stream));
+                    HIPCK(c, hipEventSynchronize(e1));
+                    HIPCK(c, hipEventElapsedTime(&ms, e0, e1));
+                }
+                LAUNCHCK(c);
+                const double us = ms * 1e3 / 4.0;
+                if (c->verbose)
+                    std::fprintf(stderr, "nyxpm autotune: resid_restrict %s.d%d z-chunk %d: %.2f us\n",
+                                 pr == 0 ? "fp64" : "fp32", d, zc, us);
+                if (us < best) {
+                    best = us;
+                    bzc = zc;
+                }
+            }
+            if ((rc = rebuild_rrz_table(c, pr, d, bzc))) return rc;
+            char buf[160];
+            snprintf(buf, sizeof(buf), "%s\"rrz.%s.d%d\": {\"zchunk\": %d, \"us\": %.2f}", first_entry ? "" : ", ",
+                     pr == 0 ? "fp64" : "fp32", d, bzc, best);
             rep += buf;
             first_entry = false;
         }
