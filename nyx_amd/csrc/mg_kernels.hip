@@ -774,11 +774,17 @@ __global__ void __launch_bounds__((sw2_threads<TX, TY>())) k_rb_sweep2(const Cel
         if (own && kb >= z0 && kb < z1) {
             // plane k-3: its red cell has index c ^ 1
             const T e0 = c ? Er[4] : Eb[4], e1 = c ? Eb[4] : Er[4];
+            // streaming stores: the output is read again only by the next pass,
+            // and lines kept in L2 would push out the input halos the
+            // neighbouring blocks are about to read
             if (UPD) {
-                CellPair<double>* q = reinterpret_cast<CellPair<double>*>(Ph + so + po);
-                *q = CellPair<double>{qp[qn].a + (double)e0, qp[qn].b + (double)e1};
+                typedef double v2d __attribute__((ext_vector_type(2)));
+                v2d o = {qp[qn].a + (double)e0, qp[qn].b + (double)e1};
+                __builtin_nontemporal_store(o, reinterpret_cast<v2d*>(Ph + so + po));
             } else {
-                *reinterpret_cast<CellPair<T>*>(Eout + so + po) = CellPair<T>{e0, e1};
+                typedef T v2t __attribute__((ext_vector_type(2)));
+                v2t o = {e0, e1};
+                __builtin_nontemporal_store(o, reinterpret_cast<v2t*>(Eout + so + po));
             }
         }
         if (kb >= z0) so += plane;
@@ -1280,7 +1286,7 @@ __global__ void __launch_bounds__(kResTX * kResTY) k_residual0(const CellSeg* se
                 fv = (t * Ggravity) * inv_a;
             }
             const double v = fv - (lxv + lyv + lzv);
-            Rr[(size_t)k * plane + col] = T(v);
+            __builtin_nontemporal_store(T(v), &Rr[(size_t)k * plane + col]);  // read by the next pass only
             ar = fmax(ar, fabs(v));
             af = fmax(af, fabs(fv));
         }

@@ -1229,13 +1229,15 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
                     pz = pz + dt_a_cur2_inv * uz;
                 }
             }
-            VX[q] = ux;
-            VY[q] = uy;
-            VZ[q] = uz;
+            // streaming stores (each particle is written once per pass; the
+            // L2 keeps the phi lines the neighbouring tiles stage)
+            __builtin_nontemporal_store(ux, &VX[q]);
+            __builtin_nontemporal_store(uy, &VY[q]);
+            __builtin_nontemporal_store(uz, &VZ[q]);
             if (MODE != kKick && stay) {
-                X[q] = px;
-                Y[q] = py;
-                Z[q] = pz;
+                __builtin_nontemporal_store(px, &X[q]);
+                __builtin_nontemporal_store(py, &Y[q]);
+                __builtin_nontemporal_store(pz, &Z[q]);
                 const int t = tile_of(B, px, py, pz, err);
                 if (inc) {
                     // incremental re-sort: only the particles that left the tile
