@@ -1600,10 +1600,13 @@ int phase_kick_g(nyxpm_ctx* c, double dt, double a_prev, double a_cur, double a_
     if ((rc = ensure_idle_buffer(c))) return rc;
     PartPtrs& A = c->P[c->cur];
     constexpr bool MOVE = MODE != kKick;
-    // staged phi tile with a halo of 2, plus the three components of g on the
-    // (tile+1)^3 cells its particles interpolate from
-    const size_t lds = sizeof(double) * (c->tile[0] + 3) * (c->tile[1] + 3) * (c->tile[2] + 3) +
-                       sizeof(G) * 3 * (c->tile[0] + 1) * (c->tile[1] + 1) * (c->tile[2] + 1);
+    // staged phi tile with a halo of 2, and the three components of g on the
+    // (tile+1)^3 cells its particles interpolate from: written over the staged
+    // phi where a thread's share of g fits in registers, else beside it
+    const size_t lds_phi = sizeof(double) * (c->tile[0] + 3) * (c->tile[1] + 3) * (c->tile[2] + 3);
+    const long long ng = (long long)(c->tile[0] + 1) * (c->tile[1] + 1) * (c->tile[2] + 1);
+    const size_t lds_g = sizeof(G) * 3 * ng;
+    const size_t lds = ng <= (long long)kBlock * kKickG ? std::max(lds_phi, lds_g) : lds_phi + lds_g;
     if (c->items_valid && !c->untiled_kick && lds <= 64 * 1024) {
         Timed t(c, MODE == kKickPair ? "kick_pair" : MOVE ? "kick_drift" : "kick");
         // incremental re-sort: the kick-drift lists the particles that change tile

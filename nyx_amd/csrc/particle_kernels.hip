@@ -978,6 +978,9 @@ __global__ void __launch_bounds__(kBlock) k_est_dt(const BoxSeg* segs, int nseg,
 }
 
 // Kick variants of the tile-staged particle kernel.
+// Cells of g per thread that the tile-staged kernel keeps in registers to
+// write g over the staged phi (tiles of up to kBlock * kKickG g cells).
+constexpr int kKickG = 8;
 constexpr int kKick = 0;       // moveKick: kick, then the periodic shift of Redistribute
 constexpr int kKickDrift = 1;  // moveKickDrift: kick, drift, new tile key
 constexpr int kKickPair = 2;   // moveKick of one step followed by moveKickDrift of the next
@@ -1061,7 +1064,8 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
     int* __restrict__ KEY = key + it.p0;
     // one particle per thread at a time, the first loaded before the staging of
     // phi so that its latency overlaps the staging.  (Two per thread in flight
-    // need ~20 more VGPRs: 3 waves per SIMD instead of 4, 14% slower.)
+    // need 12-27 more VGPRs, one wave per SIMD fewer: 2% slower in mixed
+    // precision, 14% in FP64.)
     double ax = 0.0, ay = 0.0, az = 0.0, aux = 0.0, auy = 0.0, auz = 0.0;
     auto load1 = [&](int q) {
         ax = X[q]; ay = Y[q]; az = Z[q];
@@ -1109,10 +1113,52 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
     __syncthreads();
     const double dxi0 = B.dxi[0], dxi1 = B.dxi[1], dxi2 = B.dxi[2];
     // g at the cells the tile's particles interpolate from (local -1 .. b-1 per
-    // dimension), components of a cell adjacent; formed in FP64, kept as G
+    // dimension), components of a cell adjacent; formed in FP64, kept as G.
+    // Where every thread's share fits in registers (kKickG cells) g is formed
+    // there and, after a barrier, written over the staged phi, which it no
+    // longer needs: the block holds the larger of the two arrays instead of both
+    // (more blocks per CU); otherwise g goes beside the staged phi.
     const int GX = bx + 1, GY = by + 1, GZ = bz + 1;
-    G* sg = reinterpret_cast<G*>(sphi + SX * SY * SZ);
-    {
+    const int ng = GX * GY * GZ;
+    const bool alias = ng <= kBlock * kKickG;
+    G* sg = reinterpret_cast<G*>(alias ? sphi : sphi + SX * SY * SZ);
+    if (alias) {
+        G gv[kKickG][3];
+        // cell index q = tid + u * kBlock, as (cx, cy, cz), advanced incrementally
+        int cx = threadIdx.x % GX, cy = (threadIdx.x / GX) % GY, cz = threadIdx.x / (GX * GY);
+        const int dcx = kBlock % GX, dcy = (kBlock / GX) % GY, dcz = kBlock / (GX * GY);
+#pragma unroll
+        for (int u = 0; u < kKickG; ++u) {
+            if ((int)threadIdx.x + u * kBlock < ng) {
+                const int o = ((cz + 1) * SY + (cy + 1)) * SX + (cx + 1);  // staged index of the cell
+                const double pc = sphi[o];
+                gv[u][0] = G(0.5 * (dxi0 * (pc - sphi[o - 1]) + dxi0 * (sphi[o + 1] - pc)));
+                gv[u][1] = G(0.5 * (dxi1 * (pc - sphi[o - SX]) + dxi1 * (sphi[o + SX] - pc)));
+                gv[u][2] = G(0.5 * (dxi2 * (pc - sphi[o - SX * SY]) + dxi2 * (sphi[o + SX * SY] - pc)));
+            }
+            cx += dcx;
+            cy += dcy;
+            cz += dcz;
+            if (cx >= GX) {
+                cx -= GX;
+                ++cy;
+            }
+            if (cy >= GY) {
+                cy -= GY;
+                ++cz;
+            }
+        }
+        __syncthreads();  // every read of the staged phi is done
+#pragma unroll
+        for (int u = 0; u < kKickG; ++u) {
+            const int q = (int)threadIdx.x + u * kBlock;
+            if (q < ng) {
+                sg[3 * q] = gv[u][0];
+                sg[3 * q + 1] = gv[u][1];
+                sg[3 * q + 2] = gv[u][2];
+            }
+        }
+    } else {
         const int rpw = kWave / GX;
         const int lrow = lane / GX, cx = lane - lrow * GX;
         if (lrow < rpw) {
