@@ -817,10 +817,10 @@ def main_batch(a):
     res = {}
     for prec in ("fp64", "mixed"):
         ctx = nyx_amd.PMContext([nyx_amd.BoxGeom((n, n, n)) for n in sizes], precision=prec)
-        tuned = ctx.autotune(fp64=prec == "fp64", fp32=prec == "mixed")
-        log(prec, "autotuned", tuned)
         for b, (pos, vel, m) in enumerate(inputs):
             ctx.upload(b, pos, vel, m_const=m)
+        tuned = ctx.autotune(fp64=prec == "fp64", fp32=prec == "mixed")  # with the particle tile
+        log(prec, "autotuned", tuned)
         ctx.pm_init(1.0, 1.0, a.tol)
         ctx.pm_steps(2, 1e-3, rel_tol=a.tol, use_graph=True)  # capture + warm up
         ctx.sync()
@@ -848,9 +848,8 @@ def main_batch(a):
         tseq = 0.0
         for b, (pos, vel, m) in enumerate(inputs):
             c1 = nyx_amd.PMContext(nyx_amd.BoxGeom((sizes[b],) * 3), precision=prec)
-            if b in (0, 16, 32):  # tune once per box size; the context is rebuilt per box
-                seq_tuned = c1.autotune(fp64=prec == "fp64", fp32=prec == "mixed")
             c1.upload(0, pos, vel, m_const=m)
+            c1.autotune(fp64=prec == "fp64", fp32=prec == "mixed")  # every box's own context
             c1.pm_init(1.0, 1.0, a.tol)
             c1.pm_steps(2, 1e-3, rel_tol=a.tol, use_graph=True)
             c1.sync()

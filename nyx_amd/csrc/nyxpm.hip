@@ -3065,17 +3065,77 @@ static int retile_from(nyxpm_ctx* c, const int t[3], const ParticleSnapshot& s) 
     free_tile_buffers(c);
     if ((rc = alloc_tile_buffers(c))) return rc;
     if ((rc = build_tseg(c))) return rc;
-    // an empty layout, then the boxes one by one
-    for (auto& B : c->boxes) {
-        B.np = 0;
-        B.span = 0;
+    // every box's slots laid out for the new tile counts, each box copied once
+    // from the snapshot (the mass bookkeeping of the boxes is unchanged)
+    std::vector<long long> off(nbox), span(nbox);
+    long long slots = 0;
+    for (int b = 0; b < nbox; ++b) {
+        off[b] = slots;
+        span[b] = slot_count(c, np[b], c->boxes[b].ntiles);
+        slots += span[b];
     }
+    PartPtrs N = {};
+    int* nkey = nullptr;
+    const long long cap = std::max(slots, 1LL);
+    if ((rc = alloc_particle_set(c, N, cap, c->any_mass))) return rc;
+    if ((rc = dalloc(c, &nkey, cap))) return rc;
     for (int b = 0; b < nbox; ++b) {
         const double* const* a = s.a.data() + 7 * (size_t)b;
-        if ((rc = upload_impl(c, b, np[b], a[0], a[1], a[2], a[3], a[4], a[5], a[6], mc[b],
-                              hipMemcpyDeviceToDevice)))
-            return rc;
+        const long long d = off[b];
+        const size_t n8 = (size_t)np[b] * sizeof(double);
+        if (np[b] > 0) {
+            double* dst[6] = {N.x, N.y, N.z, N.vx, N.vy, N.vz};
+            for (int q = 0; q < 6; ++q)
+                HIPCK(c, hipMemcpyAsync(dst[q] + d, a[q], n8, hipMemcpyDeviceToDevice, c->stream));
+            if (c->any_mass) {
+                if (a[6]) {
+                    HIPCK(c, hipMemcpyAsync(N.m + d, a[6], n8, hipMemcpyDeviceToDevice, c->stream));
+                } else {
+                    hipLaunchKernelGGL(k_fill_dbl, dim3(nblocks(np[b])), dim3(kBlock), 0, c->stream, N.m + d,
+                                       (long long)np[b], mc[b]);
+                }
+            }
+            hipLaunchKernelGGL(k_iota, dim3(nblocks(np[b])), dim3(kBlock), 0, c->stream, d, (long long)np[b],
+                               N.id);
+            LAUNCHCK(c);
+        }
+        if (span[b] > np[b]) {
+            hipLaunchKernelGGL(k_fill_dbl, dim3(nblocks(span[b] - np[b])), dim3(kBlock), 0, c->stream,
+                               N.x + d + np[b], span[b] - np[b], std::nan(""));
+            LAUNCHCK(c);
+        }
     }
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    const long long icap = c->ntiles_total + slots / c->kick_chunk + 1;
+    if (icap > 0x7fffffffLL) FAIL(c, NYXPM_ERR_ARG, "too many tile chunks");
+    TileItem* nitm = nullptr;
+    if ((rc = dalloc(c, &nitm, icap))) return rc;
+    const long long hcap = std::min<long long>(c->ntiles_total, slots / kHeavyTile + 1);
+    int* nheavy_list = nullptr;
+    if ((rc = dalloc(c, &nheavy_list, hcap))) return rc;
+    free_particles(c);
+    c->P[0] = N;
+    c->P[1] = PartPtrs{};
+    c->key = nkey;
+    c->items = nitm;
+    c->items_cap = icap;
+    c->heavy = nheavy_list;
+    c->heavy_cap = (int)hcap;
+    c->items_valid = false;
+    c->dt_valid = false;
+    c->cur = 0;
+    c->part_cap = cap;
+    for (int b = 0; b < nbox; ++b) {
+        c->boxes[b].part_off = off[b];
+        c->boxes[b].span = span[b];
+    }
+    c->moves_pending = false;
+    invalidate_guess(c);
+    if ((rc = upload_boxdev(c))) return rc;
+    if ((rc = build_pseg(c))) return rc;
+    c->keys_valid = c->sorted_valid = c->density_valid = false;
+    if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
     return NYXPM_OK;
 }
 
