@@ -36,7 +36,13 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-TILE = (12, 8, 8)  # the library's default deposit/kick tile
+TILE = (16, 8, 8)  # the library's default deposit/kick tile
+
+
+def tile_of(tuned):
+    """The deposit/kick tile of a run: the autotuned one if tuned, else the default."""
+    t = (tuned or {}).get("tile")
+    return tuple(t["shape"]) if t else TILE
 
 
 def tile_halo_cells(n, tile):
@@ -424,7 +430,7 @@ def main():
             [f"{k}.d{d}" for k in ("sweep0", "sweep", "sweep_p", "sweep_u", "smooth", "resid_restrict", "prolong",
                                     "fill", "sweep2", "sweep2_0", "sweep2_p", "sweep2_u")
              for d in range(0, 12)]
-    halo_cells = tile_halo_cells((n, n, n), TILE)
+    halo_cells = tile_halo_cells((n, n, n), tile_of(tuned))
 
     def settle(prec):
         # same state, then one untimed step of this precision (its first sort is
@@ -596,7 +602,42 @@ def _ceil(a, b):
     return -(-a // b)
 
 
-def kernel_level(name, grid, n, tuned, sw2_zc=128):
+def _split_by_level(values, levels, counts=None):
+    """Dispatches of one kernel and grid size that several levels launch alike,
+    split over those levels, the largest values (durations or counter readings,
+    ~8x apart from one level to the next) on the finest: by the launch count of
+    each level where known (counts), else at the gaps of more than 3x."""
+    import math
+    levels = sorted(levels)
+    v = sorted(values, reverse=True)
+    if len(levels) == 1:
+        return {levels[0]: v}
+    out, start = {}, 0
+    if counts and all(l in counts for l in levels):
+        for i, lev in enumerate(levels):
+            end = len(v) if i + 1 == len(levels) else min(len(v), start + counts[lev])
+            out[lev] = v[start:end]
+            start = end
+        return out
+    gaps = [i for i in range(1, len(v)) if v[i - 1] > 3.0 * max(v[i], 1e-30)]
+    gaps = sorted(sorted(gaps, key=lambda i: -math.log(v[i - 1] / max(v[i], 1e-30)))[:len(levels) - 1])
+    for lev, end in zip(levels, gaps + [len(v)]):
+        out[lev] = v[start:end]
+        start = end
+    return out
+
+
+def kernel_levels(name, grid, n, tuned, sw2_zc=128):
+    """Every level kernel_level could assign (the finest first)."""
+    levs = []
+    for d in range(12):
+        lev = kernel_level(name, grid, n, tuned, sw2_zc, only=d)
+        if lev is not None:
+            levs.append(lev)
+    return levs or [kernel_level(name, grid, n, tuned, sw2_zc)]
+
+
+def kernel_level(name, grid, n, tuned, sw2_zc=128, only=None):
     """Multigrid level (0 = finest) of a launch of a level kernel of a box of n^3
     cells, from its grid size (threads) and the launch shapes the library uses
     (the autotuned fused-sweep shape and z-chunk per precision and level, as the
@@ -607,7 +648,7 @@ def kernel_level(name, grid, n, tuned, sw2_zc=128):
     if mt:
         tx, ty = int(mt.group(2)), int(mt.group(3))
         nt = (tx + 8) // 2 * (ty + 8)
-        for d in range(12):
+        for d in (range(12) if only is None else [only]):
             nd = n >> d
             if nd < 8:
                 break
@@ -622,7 +663,7 @@ def kernel_level(name, grid, n, tuned, sw2_zc=128):
     if mt:
         pr = "fp32" if mt.group(1) == "float" else "fp64"
         tx, ty = int(mt.group(2)), int(mt.group(3))
-        for d in range(12):
+        for d in (range(12) if only is None else [only]):
             nd = n >> d
             if nd < 2:
                 break
@@ -638,7 +679,7 @@ def kernel_level(name, grid, n, tuned, sw2_zc=128):
     if mt:
         pr = "fp32" if mt.group(1) == "float" else "fp64"
         cx, cy = int(mt.group(2)), int(mt.group(3))
-        for d in range(12):
+        for d in (range(12) if only is None else [only]):
             nd = n >> d
             if nd < 2 * cx:
                 break
@@ -655,14 +696,14 @@ def kernel_level(name, grid, n, tuned, sw2_zc=128):
                 return d
         return None
     if "k_resid_restrict<" in name:
-        for d in range(12):
+        for d in (range(12) if only is None else [only]):
             nd = n >> d
             if nd < 2:
                 break
             if _ceil(nd ** 3 // 8, 256) * 256 == grid:
                 return d
         return None
-    return 0
+    return 0 if only in (None, 0) else None
 
 
 def trace_summary(csv_path, json_path):
@@ -676,7 +717,7 @@ def trace_summary(csv_path, json_path):
     n = round(run["config"]["cells"] ** (1.0 / 3.0))
     tuned = run["config"].get("autotuned")
     steps = run["steps"]
-    halo_cells = tile_halo_cells((n, n, n), TILE)
+    halo_cells = tile_halo_cells((n, n, n), tile_of(tuned))
     out_rows = []
     for prec in ("mixed", "fp64"):
         w = run.get("timed_windows_monotonic_ns", {}).get(prec)
@@ -689,21 +730,30 @@ def trace_summary(csv_path, json_path):
                 if t0 < w[0] or t1 > w[1]:
                     continue
                 key = (r["Kernel_Name"], int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"]))
-                e = agg.setdefault(key, [0, 0.0])
-                e[0] += 1
-                e[1] += (t1 - t0) * 1e-6
+                agg.setdefault(key, []).append((t1 - t0) * 1e-6)
         sz = 4 if prec == "mixed" else 8
+        stab = run.get(f"stages_{prec}") or {}
         rows = []
-        for (nm, grid), (cnt, tot) in agg.items():
+        for (nm, grid), durs in agg.items():
             stage, tprec = _stage_of_kernel(nm)
-            b, lev = None, None
+            groups = {None: durs}
             if stage:
-                lev = kernel_level(nm, grid, n, tuned)
-                key = stage
-                if lev is not None and (stage.startswith("sweep") or stage == "resid_restrict"):
-                    key = f"{stage}.d{lev}"
-                b = kernel_bytes(key, n ** 3, n ** 3, halo_cells, 4 if tprec == "float" else sz)
-            rows.append((tot, nm, grid, stage, lev, cnt, tot / cnt, b))
+                levs = kernel_levels(nm, grid, n, tuned)
+                # the launches per level of the instrumented steps (the same number of steps)
+                known = [l for l in levs if f"{stage}.d{l}" in stab]
+                counts = {l: stab[f"{stage}.d{l}"]["launches"] for l in known}
+                groups = _split_by_level(durs, known or levs, counts if known else None)
+            for lev, dl in groups.items():
+                if not dl:
+                    continue
+                b = None
+                if stage:
+                    key = stage
+                    if lev is not None and (stage.startswith("sweep") or stage == "resid_restrict"):
+                        key = f"{stage}.d{lev}"
+                    b = kernel_bytes(key, n ** 3, n ** 3, halo_cells, 4 if tprec == "float" else sz)
+                cnt, tot = len(dl), sum(dl)
+                rows.append((tot, nm, grid, stage, lev, cnt, tot / cnt, b))
         rows.sort(reverse=True)
         tot_all = sum(r[0] for r in rows)
         out_rows.append(f"### {prec} timed steps ({steps} steps, window {w[1] - w[0]} ns, kernels "
@@ -732,23 +782,28 @@ def pmc_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, fetch_json
 
     def tuned_of(path):
         if not path:
-            return None, n
+            return None, n, []
         with open(path) as fh:
             run = json.loads(fh.read().strip().splitlines()[-1])
-        return run["config"].get("autotuned"), round(run["config"]["cells"] ** (1.0 / 3.0))
+        wins = [tuple(w) for w in run.get("timed_windows_monotonic_ns", {}).values() if w]
+        return run["config"].get("autotuned"), round(run["config"]["cells"] ** (1.0 / 3.0)), wins
 
     def load(path, jpath):
-        tuned, nn = tuned_of(jpath)
+        tuned, nn, wins = tuned_of(jpath)
         rows = {}
         with open(path) as fh:
             for r in csv.DictReader(fh):
+                # only the dispatches of the timed steps (not the autotuner's trials)
+                if wins and not any(a0 <= int(r["Start_Timestamp"]) <= a1 for a0, a1 in wins):
+                    continue
                 nm = r["Kernel_Name"].split("(")[0].replace("void ", "").replace("nyxpm::", "")
                 rows.setdefault((nm, int(r["Grid_Size"])), []).append(float(r["Counter_Value"]))
         out = {}
         for (nm, grid), v in rows.items():
             if jpath:
-                if kernel_level(nm, grid, nn, tuned) == 0:
-                    out.setdefault(nm, []).extend(v)
+                levs = kernel_levels(nm, grid, nn, tuned)
+                if 0 in levs:
+                    out.setdefault(nm, []).extend(_split_by_level(v, levs)[0])
             elif nm not in out or grid > out[nm][0]:
                 out[nm] = (grid, v)
         if not jpath:
@@ -763,10 +818,11 @@ def pmc_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, fetch_json
         raise SystemExit(f"{path}: no __amd_rocclr_copyBuffer dispatch")
 
     (F, nn), (W, _) = load(fetch_csv, fetch_json), load(write_csv, write_json)
+    tuned_f = tuned_of(fetch_json)[0]
     gib_kib = 2.0 ** 30 / 1024.0
     sf, sw = gib_kib / copy_kib(copy_fetch_csv), gib_kib / copy_kib(copy_write_csv)
     N = nn ** 3
-    halo = tile_halo_cells((nn, nn, nn), TILE)
+    halo = tile_halo_cells((nn, nn, nn), tile_of(tuned_f))
     stages = {}
     print(f"calibration: a 1 GiB copy reads FETCH_SIZE x{sf:.3f}, WRITE_SIZE x{sw:.3f} of its bytes\n")
     print("| kernel (finest level) | dispatches | FETCH KiB | WRITE KiB | traffic GB (scaled) | algorithmic GB | ratio |")

@@ -78,7 +78,7 @@ typedef struct nyxpm_opts {
     int precision;     /* NYXPM_FP64 or NYXPM_MIXED */
     int nu1, nu2;      /* pre/post smoothing sweeps (MLMG defaults 2, 2); 0 = default */
     int max_iter;      /* V-cycle limit (MLMG default 200); 0 = default */
-    int tile[3];       /* deposit/gather tile shape in cells; 0 = default (12,8,8) */
+    int tile[3];       /* deposit/gather tile shape in cells; 0 = default (16,8,8) */
     int verbose;       /* 0 = silent */
     int smoother;      /* 0: fused red-black sweeps where the level allows (default);
                           1: separate red and black half sweeps everywhere */

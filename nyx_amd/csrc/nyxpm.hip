@@ -169,7 +169,7 @@ struct nyxpm_ctx {
     int dmax = 1;  // max levels over boxes
     long long ncell_total = 0, ntiles_total = 0, npart_total = 0, mg_total = 0, G_total = 0;
     int halo_stride = 0;
-    int tile[3] = {12, 8, 8};  // measured best at 512^3 clustered (LDS of the kick fits 4 blocks per CU)
+    int tile[3] = {16, 8, 8};  // measured best at 512^3 clustered (the autotuner may pick another)
 
     // particles (two buffers for the tile sort)
     PartPtrs P[2] = {};
