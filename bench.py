@@ -498,13 +498,23 @@ def main():
         time and `per_sweep_equivalent` the bytes of the two single-sweep passes
         it replaces per time."""
         stt = stt or {}
+
+        def hbm(key, mean_ms):
+            # the bytes the pass moves to and from HBM (committed PMC summary)
+            tb = measured_traffic(key, n)
+            if not tb:
+                return {}
+            g = tb / (mean_ms * 1e-3) / 1e9
+            return {"hbm_traffic_per_launch": tb, "hbm_achieved": round(g, 1), "hbm_frac": round(g / HBM_PEAK_GBS, 4)}
+
         e = stt.get("sweep.d0")
         if e:
             bb = kernel_bytes("sweep.d0", ncell, npart, halo_cells, sz)
             ach = bb / (e["mean_ms"] * 1e-3) / 1e9
             return {"kernel": "sweep.d0 (red+black sweep, finest level)", "achieved": round(ach, 1),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                    "bytes_per_launch": int(bb), "mean_launch_ms": e["mean_ms"]}
+                    "bytes_per_launch": int(bb), "mean_launch_ms": e["mean_ms"],
+                    **hbm(("fp64:" if sz == 8 else "") + "sweep.d0", e["mean_ms"])}
         e = stt.get("sweep2.d0")
         if not e:
             return None
@@ -514,7 +524,8 @@ def main():
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                 "bytes_per_launch": int(bb), "mean_launch_ms": e["mean_ms"],
                 "per_sweep_equivalent": {"achieved": round(2 * ach, 1), "frac": round(2 * ach / HBM_PEAK_GBS, 4),
-                                         "bytes_of_two_passes": int(2 * bb)}}
+                                         "bytes_of_two_passes": int(2 * bb)},
+                **hbm("sweep2.d0", e["mean_ms"])}
 
     copy = stream_copy()
     log(f"device copy {copy['gbs']} GB/s")
@@ -843,9 +854,12 @@ def pmc_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, fetch_json
         # and interpolation) for roofline.traffic
         mixed_kernel = {"fold": nm.startswith("k_fold<false>"),
                         "deposit": nm.startswith("k_deposit<true>")}.get(key, "double" not in nm)
-        if key and alg and mixed_kernel:
-            stages[key] = {"kernel": nm, "fetch_bytes": int(fk * sf * 1024), "write_bytes": int(wk * sw * 1024),
-                           "traffic_bytes": int(gb * 1e9), "algorithmic_bytes": int(alg)}
+        if key and alg:
+            # the mixed step's kernels under the stage name, the FP64 step's
+            # under "fp64:" + the stage name
+            stages[key if mixed_kernel else "fp64:" + key] = {
+                "kernel": nm, "fetch_bytes": int(fk * sf * 1024), "write_bytes": int(wk * sw * 1024),
+                "traffic_bytes": int(gb * 1e9), "algorithmic_bytes": int(alg)}
     return stages
 
 
