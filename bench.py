@@ -1023,8 +1023,10 @@ def main_grf1024(a):
         torch.cuda.empty_cache()
 
     log(f"grf1024: n={n}, {steps} steps, dt={dt}")
-    # dense tiles (a full sort after every kick-drift): the per-step accuracy
-    # check holds 60 GB of comparison arrays, and slack slots would add ~35 GB
+    # the checked trajectory with dense tiles (a full sort after every
+    # kick-drift): its per-step accuracy check holds 60 GB of comparison arrays,
+    # and slack slots would add ~35 GB; the two timed trajectories keep the
+    # library's default slack (incremental re-sorts)
     slack = -1.0
     ctx = nyx_amd.PMContext(nyx_amd.BoxGeom((n, n, n)), precision="mixed", resort_slack=slack)
     tuned = ctx.autotune(fp64=True, fp32=True) if not a.no_tune else None
@@ -1064,7 +1066,7 @@ def main_grf1024(a):
     torch.cuda.empty_cache()
     # the mixed trajectory again from the same initial conditions, all steps in
     # one call (paired kicks), as the FP64 trajectory below is timed
-    ctx = nyx_amd.PMContext(nyx_amd.BoxGeom((n, n, n)), precision="mixed", resort_slack=slack)
+    ctx = nyx_amd.PMContext(nyx_amd.BoxGeom((n, n, n)), precision="mixed")
     if not a.no_tune:
         ctx.autotune(fp64=False, fp32=True)
     fresh_ics(ctx)
@@ -1077,7 +1079,7 @@ def main_grf1024(a):
     ctx.close()
     torch.cuda.empty_cache()
     log(f"mixed: {t_mix / steps * 1e3:.2f} ms/step in one call; FP64 trajectory")
-    ctx = nyx_amd.PMContext(nyx_amd.BoxGeom((n, n, n)), precision="fp64", resort_slack=slack)
+    ctx = nyx_amd.PMContext(nyx_amd.BoxGeom((n, n, n)), precision="fp64")
     if not a.no_tune:
         ctx.autotune(fp64=True, fp32=False)
     fresh_ics(ctx)
