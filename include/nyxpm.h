@@ -127,7 +127,10 @@ typedef struct nyxpm_opts {
                           stopping test; later steps run one V-cycle each.  Saves
                           the FP64 residual and update of the cycles in between;
                           a solve that would stop earlier runs up to ir_first
-                          cycles.  0 = default (5); 1 = a test after every cycle.
+                          cycles.  0 = default: 5, or 4 with phi_guess = 2 (a
+                          smaller first residual: the FP32 correction reaches
+                          its limit, ~1e-7 of the residual it starts from, a
+                          cycle earlier); 1 = a test after every cycle.
                           Needs nu1 + nu2 even (else 1); FP64 solves and fixed
                           iteration counts always test after every cycle. */
     int sweep2;        /* FP32 levels with V(2,2) cycles: each pair of smoothing

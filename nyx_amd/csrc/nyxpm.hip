@@ -141,6 +141,10 @@ constexpr int kGraphMargin = 2;
 }  // namespace
 
 constexpr int kIrFirstDefault = 5;
+// with the extrapolated guess (phi_guess = 2) the first residual is about ten
+// times smaller, and the FP32 correction reaches its precision limit (about
+// 1e-7 of the residual it starts from) one cycle earlier
+constexpr int kIrFirstExtrapolated = 4;
 
 struct nyxpm_ctx {
     std::string err;
@@ -1850,7 +1854,10 @@ int nyxpm_create_batch(nyxpm_ctx** out, int nbox, const nyxpm_geom* geoms, const
         if (opts->phi_guess >= 0 && opts->phi_guess <= 2) c->phi_guess = opts->phi_guess;
         c->fuse_res = opts->residual == 1;
         if (opts->graph_cycles > 0) c->graph_cycles = opts->graph_cycles;
-        if (opts->ir_first > 0) c->ir_first = opts->ir_first;
+        if (opts->ir_first > 0)
+            c->ir_first = opts->ir_first;
+        else if (c->phi_guess == 2)
+            c->ir_first = kIrFirstExtrapolated;
         if (opts->sweep2_tx > 0 || opts->sweep2_ty > 0) {
             const int tx = opts->sweep2_tx > 0 ? opts->sweep2_tx : kSw2TX;
             const int ty = opts->sweep2_ty > 0 ? opts->sweep2_ty : kSw2TY;
