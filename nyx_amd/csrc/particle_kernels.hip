@@ -555,35 +555,29 @@ __device__ __forceinline__ double block_sum_fixed(double v) {
     return t;
 }
 
-// Cell (i, j, k) of a box after SumBoundary: its own deposit plus the low-halo
-// entries of the up-to-7 upper neighbour tiles that hold it, added in a fixed
-// order (neighbour in x, y, z, xy, xz, yz, xyz).
-__device__ __forceinline__ double fold_cell(const BoxDev& B, const double* own, const double* __restrict__ halo,
-                                            int i, int j, int k, size_t q) {
+// The edge and corner terms of cell (i, j, k) after SumBoundary: the low-halo
+// entries of the upper neighbour tiles across two or three of the tile's top
+// faces (neighbour in xy, xz, yz, xyz, in that order), added to v.
+__device__ __forceinline__ double fold_edges(const BoxDev& B, const double* __restrict__ halo, double v, int tx,
+                                             int ty, int tz, int li, int lj, int lk, bool topx, bool topy,
+                                             bool topz) {
     const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
     const int T0 = B.tile[0], T1 = B.tile[1], T2 = B.tile[2];
     const int nt0 = B.ntile[0], nt1 = B.ntile[1], nt2 = B.ntile[2];
-    const int tx = tile_div(i, B.tdiv[0]), ty = tile_div(j, B.tdiv[1]), tz = tile_div(k, B.tdiv[2]);
-    const int li = i - tx * T0, lj = j - ty * T1, lk = k - tz * T2;
-    const int bx = min(T0, nx - tx * T0), by = min(T1, ny - ty * T1), bz = min(T2, nz - tz * T2);
-    const bool topx = li == bx - 1, topy = lj == by - 1, topz = lk == bz - 1;
-    double v = own[q];
-    if (topx || topy || topz) {
-        const int sel[7] = {1, 2, 4, 3, 5, 6, 7};
+    const int sel[4] = {3, 5, 6, 7};
 #pragma unroll
-        for (int s = 0; s < 7; ++s) {
-            const int S = sel[s];
-            const bool sx = S & 1, sy = S & 2, sz = S & 4;
-            if ((sx && !topx) || (sy && !topy) || (sz && !topz)) continue;
-            int ntx = tx + (sx ? 1 : 0), nty = ty + (sy ? 1 : 0), ntz = tz + (sz ? 1 : 0);
-            if (ntx == nt0) ntx = 0;
-            if (nty == nt1) nty = 0;
-            if (ntz == nt2) ntz = 0;
-            const int nbx = min(T0, nx - ntx * T0), nby = min(T1, ny - nty * T1), nbz = min(T2, nz - ntz * T2);
-            const int hx = sx ? 0 : li + 1, hy = sy ? 0 : lj + 1, hz = sz ? 0 : lk + 1;
-            const long long nt = B.tile_off + ((long long)ntz * nt1 + nty) * nt0 + ntx;
-            v += halo[nt * B.halo_stride + halo_index(hx, hy, hz, nbx, nby, nbz)];
-        }
+    for (int s = 0; s < 4; ++s) {
+        const int S = sel[s];
+        const bool sx = S & 1, sy = S & 2, sz = S & 4;
+        if ((sx && !topx) || (sy && !topy) || (sz && !topz)) continue;
+        int ntx = tx + (sx ? 1 : 0), nty = ty + (sy ? 1 : 0), ntz = tz + (sz ? 1 : 0);
+        if (ntx == nt0) ntx = 0;
+        if (nty == nt1) nty = 0;
+        if (ntz == nt2) ntz = 0;
+        const int nbx = min(T0, nx - ntx * T0), nby = min(T1, ny - nty * T1), nbz = min(T2, nz - ntz * T2);
+        const int hx = sx ? 0 : li + 1, hy = sy ? 0 : lj + 1, hz = sz ? 0 : lk + 1;
+        const long long nt = B.tile_off + ((long long)ntz * nt1 + nty) * nt0 + ntx;
+        v += halo[nt * B.halo_stride + halo_index(hx, hy, hz, nbx, nby, nbz)];
     }
     return v;
 }
@@ -623,7 +617,108 @@ __device__ __forceinline__ void for_block_cells(const CellSeg& sg, const BoxDev&
     }
 }
 
-// rho = (own + halos, fold_cell) * 1/vol; f = rho + (-mass_offset); per-block sum
+// The folded deposit (SumBoundary) of a block's share of a box's cells, in
+// for_block_cells' order (the same cells per thread, in the same sequence):
+// own deposit plus the low-halo entries of the up-to-7 upper neighbour tiles
+// that hold the cell, added in a fixed order (neighbour in x, y, z, xy, xz, yz,
+// xyz); emit(q, v) per cell in order.  U cells per thread at a time: their own
+// values and face terms (x, y, z) are loaded before any is used, so a thread
+// has up to 4U loads in flight; the edge and corner terms (cells on two or
+// three top faces of a tile) follow them.  Coordinates advance by kBlock cells
+// with block-uniform carries instead of divisions.
+template <int U, class F>
+__device__ __forceinline__ void fold_walk(const CellSeg& sg, const BoxDev& B, const double* own,
+                                          const double* __restrict__ halo, F emit) {
+    const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
+    const int T0 = B.tile[0], T1 = B.tile[1], T2 = B.tile[2];
+    const int nt0 = B.ntile[0], nt1 = B.ntile[1], nt2 = B.ntile[2];
+    const long long N = (long long)nx * ny * nz;
+    long long per = (N + sg.nblk - 1) / sg.nblk;
+    per = (per + kBlock - 1) / kBlock * kBlock;
+    const long long c0 = (long long)(blockIdx.x - sg.blk0) * per;
+    const long long c1 = min(N, c0 + per);
+    long long c = c0 + threadIdx.x;
+    if (c >= c1) return;
+    const long long plane = (long long)nx * ny;
+    int k = (int)(c / plane);
+    const long long rem = c - (long long)k * plane;
+    int j = (int)(rem / nx);
+    int i = (int)(rem - (long long)j * nx);
+    // kBlock cells = dk planes + dj rows + di cells (block-uniform)
+    const int di = kBlock % nx, dq = kBlock / nx;
+    const int dj = dq % ny, dk = dq / ny;
+    const long long hbase = B.tile_off;
+    const long long hs = B.halo_stride;
+    for (; c < c1; c += (long long)U * kBlock) {
+        double v[U], hx[U], hy[U], hz[U];
+        int cx[U], cy[U], cz[U];
+        int fl[U];  // bit 0, 1, 2: on the tile's top x, y, z face; -1: past the block's cells
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            cx[u] = i;
+            cy[u] = j;
+            cz[u] = k;
+            fl[u] = -1;
+            hx[u] = hy[u] = hz[u] = 0.0;
+            if (c + (long long)u * kBlock < c1) {
+                v[u] = own[B.cell_off + c + (long long)u * kBlock];
+                const int tx = tile_div(i, B.tdiv[0]), ty = tile_div(j, B.tdiv[1]), tz = tile_div(k, B.tdiv[2]);
+                const int li = i - tx * T0, lj = j - ty * T1, lk = k - tz * T2;
+                const int bx = min(T0, nx - tx * T0), by = min(T1, ny - ty * T1), bz = min(T2, nz - tz * T2);
+                const int lt = (tz * nt1 + ty) * nt0 + tx;
+                fl[u] = (li == bx - 1 ? 1 : 0) | (lj == by - 1 ? 2 : 0) | (lk == bz - 1 ? 4 : 0);
+                if (li == bx - 1) {  // x face of tile tx + 1 (same rows and planes)
+                    const int nlt = tx + 1 == nt0 ? lt - tx : lt + 1;
+                    hx[u] = halo[(hbase + nlt) * hs + (lk + 1) * (by + 1) + (lj + 1)];
+                }
+                if (lj == by - 1) {  // y face of tile ty + 1
+                    const int nty = ty + 1 == nt1 ? 0 : ty + 1;
+                    const int nby = min(T1, ny - nty * T1);
+                    const int nlt = lt + (nty - ty) * nt0;
+                    hy[u] = halo[(hbase + nlt) * hs + (nby + 1) * (bz + 1) + (lk + 1) * bx + li];
+                }
+                if (lk == bz - 1) {  // z face of tile tz + 1
+                    const int ntz = tz + 1 == nt2 ? 0 : tz + 1;
+                    const int nbz = min(T2, nz - ntz * T2);
+                    const int nlt = lt + (ntz - tz) * nt0 * nt1;
+                    hz[u] = halo[(hbase + nlt) * hs + (by + 1) * (nbz + 1) + bx * (nbz + 1) + lj * bx + li];
+                }
+            }
+            i += di;
+            j += dj;
+            k += dk;
+            if (i >= nx) {
+                i -= nx;
+                ++j;
+            }
+            if (j >= ny) {
+                j -= ny;
+                ++k;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int f = fl[u];
+            if (f < 0) continue;
+            double w = v[u];
+            if (f & 1) w += hx[u];
+            if (f & 2) w += hy[u];
+            if (f & 4) w += hz[u];
+            if (__builtin_popcount(f) >= 2) {
+                const int tx = tile_div(cx[u], B.tdiv[0]), ty = tile_div(cy[u], B.tdiv[1]),
+                          tz = tile_div(cz[u], B.tdiv[2]);
+                w = fold_edges(B, halo, w, tx, ty, tz, cx[u] - tx * T0, cy[u] - ty * T1, cz[u] - tz * T2, f & 1,
+                               f & 2, f & 4);
+            }
+            emit(B.cell_off + (size_t)(c + (long long)u * kBlock), w);
+        }
+    }
+}
+
+// Cells per thread in flight in the fold.
+constexpr int kFoldU = 4;
+
+// rho = (own + halos, fold_walk) * 1/vol; f = rho + (-mass_offset); per-block sum
 // of f for the solvability correction.  Cells in row-major order (coalesced reads
 // of own and writes of f, whole cache lines).  rho itself is written only when the
 // caller wants the density (AssignDensity); the step needs only f.
@@ -636,9 +731,7 @@ __global__ void __launch_bounds__(kBlock) k_fold(const CellSeg* segs, int nseg, 
     const CellSeg& sg = segs[find_seg(segs, nseg, blockIdx.x)];
     const BoxDev B = boxes[__builtin_amdgcn_readfirstlane(sg.box)];
     double acc = 0.0;  // this thread's cells, in a fixed order
-    for_block_cells(sg, B, [&](int i, int j, int k, long long c) {
-        const size_t q = B.cell_off + (size_t)c;
-        const double v = fold_cell(B, own, halo, i, j, k, q);
+    fold_walk<kFoldU>(sg, B, own, halo, [&](size_t q, double v) {
         const double rv = v * B.inv_vol;
         if (WRITE_RHO) rho[q] = rv;
         const double fv = rv + (-B.mass_offset);
@@ -649,16 +742,13 @@ __global__ void __launch_bounds__(kBlock) k_fold(const CellSeg* segs, int nseg, 
     if (threadIdx.x == 0) partial[blockIdx.x] = bs;
 }
 
-// out = own + halos (fold_cell, no scaling): the folded raw sums of a deposit.
+// out = own + halos (fold_walk, no scaling): the folded raw sums of a deposit.
 __global__ void __launch_bounds__(kBlock) k_fold_raw(const CellSeg* segs, int nseg, const BoxDev* boxes,
                                                      const double* __restrict__ own,
                                                      const double* __restrict__ halo, double* __restrict__ out) {
     const CellSeg& sg = segs[find_seg(segs, nseg, blockIdx.x)];
     const BoxDev B = boxes[__builtin_amdgcn_readfirstlane(sg.box)];
-    for_block_cells(sg, B, [&](int i, int j, int k, long long c) {
-        const size_t q = B.cell_off + (size_t)c;
-        out[q] = fold_cell(B, own, halo, i, j, k, q);
-    });
+    fold_walk<kFoldU>(sg, B, own, halo, [&](size_t q, double v) { out[q] = v; });
 }
 
 // AssignDensityAndVels for box `box`: rho = mass * 1/vol, v_d = mom_d / mass (0
