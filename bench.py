@@ -364,8 +364,8 @@ def main():
                                      "(profiles/pm_step_512_clustered.md)", "stages": stages}, fh, indent=1)
         return None
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md, "
-              "'How the path would shard'); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: the library runs each box on one GPU and has no "
+              "path that spreads a step over several; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if a.workload == "batch48":
         return main_batch(a)
