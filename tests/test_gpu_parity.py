@@ -68,7 +68,9 @@ def test_deposit_clustered_tiles(nyx, orc, tile):
     assert np.array_equal(a, b)  # order-independent fixed-point tile sums
 
 
-@pytest.mark.parametrize("n", [(20, 12, 6), (10, 10, 10), (2, 2, 2)])
+# (300, 6, 4): rows longer than a fold block's stride of 256 cells; (40, 24, 18):
+# partial tiles in x and z with several rows per stride
+@pytest.mark.parametrize("n", [(20, 12, 6), (10, 10, 10), (2, 2, 2), (300, 6, 4), (40, 24, 18)])
 def test_deposit_partial_tiles_and_edges(nyx, orc, n):
     r = np.random.default_rng(5)
     NP = 2000
