@@ -1327,10 +1327,11 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
                 py = py + dt_a_cur_inv * uy;
                 pz = pz + dt_a_cur_inv * uz;
             } else {
-                const double sxp = periodic_shift(px, B.plo[0], B.rhi[0], B.len[0], err);
-                const double syp = periodic_shift(py, B.plo[1], B.rhi[1], B.len[1], err);
-                const double szp = periodic_shift(pz, B.plo[2], B.rhi[2], B.len[2], err);
-                const bool moved = sxp != px || syp != py || szp != pz;
+                // periodic_shift changes exactly the coordinates outside [plo, rhi]
+                // (and flags the non-finite ones): the shifted values are formed
+                // only on that rare path, so they hold no registers on the other
+                const bool moved = !(px >= B.plo[0] && px <= B.rhi[0]) || !(py >= B.plo[1] && py <= B.rhi[1]) ||
+                                   !(pz >= B.plo[2] && pz <= B.rhi[2]);
                 if (est && !moved) {
                     // cell = floor((x - plo) dxi) is i or i-1 of the CIC index: in the staged g
                     const int ci = (int)floor((px - B.plo[0]) * dxi0) - (i - (li + o0));
@@ -1341,19 +1342,24 @@ __global__ void __launch_bounds__(kBlock) k_kick_tiled(const BoxSeg* tsegs, int 
                                                  double(gc[2]), dxi0, cfl);
                     dtmin_t = d < dtmin_t ? d : dtmin_t;
                 }
-                if (MODE == kKick) {
-                    if (sxp != px) X[q] = sxp;
-                    if (syp != py) Y[q] = syp;
-                    if (szp != pz) Z[q] = szp;
-                    if (est && moved) defer[atomicAdd(ndefer, 1)] = defer_entry(it.p0 + q, box) | kDeferDtOnly;
-                } else if (moved) {
-                    // store the kick; the kick-drift follows in k_kick_deferred
-                    X[q] = sxp;
-                    Y[q] = syp;
-                    Z[q] = szp;
-                    stay = false;
-                    defer[atomicAdd(ndefer, 1)] = defer_entry(it.p0 + q, box) | kDeferPairKD;
-                } else {
+                if (moved) {
+                    const double sxp = periodic_shift(px, B.plo[0], B.rhi[0], B.len[0], err);
+                    const double syp = periodic_shift(py, B.plo[1], B.rhi[1], B.len[1], err);
+                    const double szp = periodic_shift(pz, B.plo[2], B.rhi[2], B.len[2], err);
+                    if (MODE == kKick) {
+                        if (sxp != px) X[q] = sxp;
+                        if (syp != py) Y[q] = syp;
+                        if (szp != pz) Z[q] = szp;
+                        if (est) defer[atomicAdd(ndefer, 1)] = defer_entry(it.p0 + q, box) | kDeferDtOnly;
+                    } else {
+                        // store the kick; the kick-drift follows in k_kick_deferred
+                        X[q] = sxp;
+                        Y[q] = syp;
+                        Z[q] = szp;
+                        stay = false;
+                        defer[atomicAdd(ndefer, 1)] = defer_entry(it.p0 + q, box) | kDeferPairKD;
+                    }
+                } else if (MODE != kKick) {
                     double wx = a_prev2 * ux + half_dt * v0;
                     ux = wx * a_cur2_inv;
                     double wy = a_prev2 * uy + half_dt * v1;
